@@ -1,0 +1,124 @@
+"""Benchmark of the pllcart3d time step on one MI355X.
+
+metric: cell-steps/s at 256x128x128 (interior cells x time steps per second);
+a step is the CFL reduction plus the three RK3-CN substages.  Inputs are
+resident in HBM before the timed region; the timed region is K whole steps
+bracketed by stream synchronisation, timed with HIP events on the solver's
+stream.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s at 256x128x128, FP64 plain vs mixed/batched, L2 err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of each kernel class, in full-field touches of the
+# interior cells per launch (reads + writes of FP64 fields; the viscosity
+# and the A(mu) coefficients count as recomputed from c, scratch round trips
+# of the line solvers do not count).
+TOUCHES = {
+    "rhs_c": 7, "rhs_u": 8, "rhs_v": 8, "rhs_w": 8,
+    "sweep_y": None, "sweep_z": None, "sweep_x": None,   # per-field, see below
+    "div": 4, "velo_corr": 7, "pg": 8, "cfl": 1.5, "visc": 5,
+}
+# sweeps run for c, u, v, w in that order each substage: y/z (2,3,3,3), x (3,4,4,4)
+SWEEP_TOUCHES = {"sweep_y": 11 / 4.0, "sweep_z": 11 / 4.0, "sweep_x": 15 / 4.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(prm, c0, cells, threads):
+    """The FP64 oracle on the host cores: a bounded sample of the same workload."""
+    from oracle import oracle
+    oracle.set_threads(threads)
+    orc = oracle.Oracle(prm)
+    orc.init(c0)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.run(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or n >= 5:
+            break
+    return {"value": cells * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d step(s) of the %dx%dx%d workload from the same initial state, %.1f s" %
+                      (n, prm.nx, prm.ny, prm.nz, el)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import pllcart3d_amd as P
+    from pllcart3d_amd.configs import CONFIGS, config_params
+    cfg = CONFIGS[a.config]
+    prm = config_params(a.config)
+    cells = prm.nx * prm.ny * prm.nz
+    c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
+    sim = P.Sim(prm)
+    sim.init(c0)
+    ms = sim.time_steps(a.warmup, a.steps)
+    value = cells * a.steps / (ms * 1e-3)
+
+    roof = None
+    kernels = {}
+    if not a.no_kernels:
+        names = P.load().pllc_kernel_names().decode().split(";")
+        ksteps = max(2, min(5, a.steps))
+        for nm in names:
+            avg, n = sim.time_kernel(nm, ksteps)
+            if n:
+                kernels[nm] = {"avg_ms": avg, "launches_per_step": n / float(ksteps), "ms_per_step": avg * n / ksteps}
+        dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+        t = TOUCHES.get(dom) or SWEEP_TOUCHES.get(dom)
+        if t is not None:
+            ab = t * cells * 8.0
+            ach = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "algorithmic_bytes_per_launch": ab, "avg_launch_ms": kernels[dom]["avg_ms"]}
+        else:
+            roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                    "traffic": None, "kernel": dom}
+    step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
+    cpu = None
+    if not a.no_cpu:
+        thr = a.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(prm, c0, cells, thr)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%s: %dx%dx%d box 4x1x1, R=%g Pe=%g, plain FP64 RK3-CN step" %
+                   (a.config, prm.nx, prm.ny, prm.nz, prm.Vratio, prm.Pe), "cells": cells,
+                   "precision": "plain-fp64"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "step_hbm_fraction": round(step_ab / (ms / a.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,261 @@
+/*
+ * pllcart3d.h -- C-ABI of libpllcart3d_amd.so, the MI355X (gfx950) time
+ * step of pllcart3d.
+ *
+ * Part 1 is the drop-in surface: the reference's types (structures_3d.h),
+ * parameters (global_variables.h), allocators (allocate_3d_calloc.h), entry
+ * points (global_function_decl.h), line solver (pll_tridiag.c) and DCT
+ * (fftw_3d.h), with the same names, argument meaning and void returns.  A
+ * maintainer includes this header instead of those five and links the
+ * library; the driver's call sequence does not change.  Every double***
+ * made by alloc3d_dble (and the alloc_velo / alloc_conc / alloc_visc
+ * wrappers) is a host index table over one contiguous block, mirrored by a
+ * device buffer with the same [i][j][k] linear index; the time-step entry
+ * points compute on the device copy.  pllc_to_device / pllc_to_host move
+ * data across when host code reads or writes the arrays.
+ *
+ * Part 2 is this library's own handle API (prefix pllc_): simulations
+ * owned by the library, used by the fused RK3 step, the tests and the
+ * benchmark.  It has no counterpart in the reference.
+ *
+ * Errors: the reference's entry points return void and have no error
+ * codes.  Here a HIP failure is recorded and reported by pllc_last_error();
+ * handle-API calls return 0 on success and a negative value on failure.
+ */
+#ifndef PLLCART3D_AMD_H
+#define PLLCART3D_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+/* the reference names a member `new`, a C++ keyword: same slot, other name */
+#define PLLC_NEW new_
+#else
+#define PLLC_NEW new
+#endif
+
+/* ================================================================ types */
+
+/* structures_3d.h:2-10 and pll_tridiag.c:12-18 */
+typedef struct tridiagonal_matrix_system {
+    double *acof;
+    double *bcof;
+    double *ccof;
+    double *rhs;
+} Tds;
+
+/* structures_3d.h:12-19 */
+typedef struct tri_variables {
+    double a;
+    double b;
+    double c;
+} Tri_abc;
+
+/* structures_3d.h:21-30 */
+typedef struct momentum_equation {
+    int nx_max, ny_max, nz_max;
+    int istart, iend, jstart, jend, kstart, kend;
+    double ***PLLC_NEW, ***old;
+    double ***pg;
+    double ***explicit_n, ***explicit_nm1;
+    Tri_abc ***Ax, ***Ay, ***Az;
+} Velo;
+
+/* structures_3d.h:32-41 */
+typedef struct concentration_equation {
+    int nx_max, ny_max, nz_max;
+    int istart, iend, jstart, jend, kstart, kend;
+    double ***PLLC_NEW, ***old;
+    double ***explicit_n, ***explicit_nm1;
+    Tri_abc ***Ax, ***Ay, ***Az;
+} Conc;
+
+/* structures_3d.h:43-52 */
+typedef struct viscosity_cellcenter_nodalpoints {
+    int nx_max, ny_max, nz_max;
+    double ***cell;
+    double ***xy;
+    double ***xz;
+    double ***yz;
+} Visc;
+
+/* structures_3d.h:54-62 */
+typedef struct Grid_data {
+    int istart, iend, jstart, jend, kstart, kend;
+    int nx_cell, ny_cell, nz_cell;
+    int nx_max, ny_max, nz_max;
+    int nmax_xyz, nx_global_offset;
+    double dx, dy, dz;
+    double *X, *Y, *Z;
+    double *Xc, *Yc, *Zc;
+} Grid;
+
+/* ====================================== parameters (global_variables.h) */
+
+/* global_variables.h:2-14, pllcart3d.c:36: defined in the library */
+extern int Nx_cell_global, Ny_cell_global, Nz_cell_global;
+extern int Nx_ghost, Ny_ghost, Nz_ghost;
+extern int Initial, Ugrid, Vdiff;
+extern int Niter_print;
+extern double X1, X2, Y1, Y2, Z1, Z2;
+extern double Re, Pe, Sc, Vratio, Fx, Fy, Inf_pos, Inf_thick;
+extern double CFLmax, Timemax, Timestep, Out_inter, Out_mid;
+extern double PI;
+extern double M, Dratio, Vnorm;
+extern int Wavpert, Subharm, Nmvp;
+extern double Ampli, Lamb, Mum, Lambda;
+extern double RK1[3];   /* global_variables.h:17 */
+extern double RK2[3];   /* global_variables.h:18 */
+extern int Rank, Size;  /* always 0 and 1: one GPU, no MPI */
+
+/* =========================================== allocate_3d_calloc.h:7-204 */
+
+double *alloc1d_dble(int n);
+int **alloc2d_int(int nrows, int ncols);
+double **alloc2d_dble(int nrows, int ncols);
+/* zero-filled host block + device mirror (allocate_3d_calloc.h:46-64) */
+double ***alloc3d_dble(int nrows, int ncols, int nz);
+Tds *alloc_tds(int nrows, int ncols);
+Tri_abc ***alloc3d_tri_abc(int nrows, int ncols, int nz);
+Velo *alloc_velo(int nrows, int ncols, int nz);
+Conc *alloc_conc(int nrows, int ncols, int nz);
+Visc *alloc_visc(int nrows, int ncols, int nz);
+void free3d_dble(int nrows, int ncols, int nz, double ***array3);
+void free2d_dble(double **array2);
+void free_tds(Tds *arr, int nrows, int ncols);
+
+/* ============================== entry points (global_function_decl.h) */
+
+void input(void);                                                      /* decl:2  pllcart3d.c:271 */
+void grid_setup(Grid *);                                               /* decl:3  pllcart3d.c:353 */
+void simulation_setup(Grid *, Velo *, Velo *, Velo *, Conc *);         /* decl:4  pllcart3d.c:394 */
+void mesh_generation(Grid *);                                          /* decl:5  pllcart3d.c:436 */
+void initial_condition(Grid *, Conc *, Velo *);                        /* decl:6  pllcart3d.c:493 */
+void concentration(double, double, double, double, Grid *, double ***, double ***, double ***,
+                   Conc *);                                            /* decl:7  pllcart3d.c:555 */
+void calculate_viscosity(Grid *, double ***, Visc *);                  /* decl:9  pllcart3d.c:843 */
+void xmom(double, double, double, double, Grid *, Visc *, Visc *, double ***, double ***, double ***,
+          Velo *);                                                     /* decl:11 pllcart3d.c:952 */
+void ymom(double, double, double, double, Grid *, Visc *, Visc *, double ***, double ***, double ***,
+          Velo *);                                                     /* decl:12 pllcart3d.c:1110 */
+void zmom(double, double, double, double, Grid *, Visc *, Visc *, double ***, double ***,
+          Velo *);                                                     /* decl:13 pllcart3d.c:1263 */
+void A_conc(Grid *, Conc *);                                           /* decl:14 pllcart3d.c:1411 */
+void A_umom(Grid *, Visc *, Velo *);                                   /* decl:15 pllcart3d.c:1443 */
+void A_vmom(Grid *, Visc *, Velo *);                                   /* decl:16 pllcart3d.c:1475 */
+void A_wmom(Grid *, Visc *, Velo *);                                   /* decl:17 pllcart3d.c:1507 */
+void divergence(Grid *, double ***, double ***, double ***, double ***); /* decl:18 pllcart3d.c:1983 */
+void phisolver_dct(double, double, double, Grid *, double ***, double ***); /* decl:19 pllcart3d.c:2002 */
+void velo_correction(double, double, double, Grid *, double ***, Velo *, Velo *,
+                     Velo *);                                          /* decl:20 pllcart3d.c:2133 */
+void update_veloghost_points(Grid *, Velo *, Velo *, Velo *);          /* decl:21 pllcart3d.c:2810 */
+void update_ghost_points(Grid *, double ***phi, Velo *, Velo *, Velo *); /* decl:22 pllcart3d.c:2887 */
+void update_velo_boundary(Grid *grid, Velo *u, Velo *v, Velo *w);      /* decl:23 pllcart3d.c:2701 */
+void update_phi_ghosts(Grid *grid, double ***phi);                     /* decl:25 pllcart3d.c:2758 */
+void pressure_gradient(double, double, double, Grid *, double ***, Velo *, Velo *,
+                       Velo *);                                        /* decl:26 pllcart3d.c:2964 */
+double find_cfl(Grid *, double ***, double ***, double ***);           /* decl:28 pllcart3d.c:3095 */
+double bulk_velo(Grid *, double ***);                                  /* decl:36 pllcart3d.c:3381 */
+void update_old_conc(Conc *);                                          /* decl:37 pllcart3d.c:3402 */
+void update_old_velo(Velo *, Velo *, Velo *);                          /* decl:38 pllcart3d.c:3417 */
+void update_old_visc(Visc *, Visc *);                                  /* decl:39 pllcart3d.c:3436 */
+double calculate_weno(double *, int, int);                             /* decl:40 pllcart3d.c:3455 */
+int tridiag_1rhs(int Nmax, double *a, double *b, double *c, double *rhs); /* decl:42 pllcart3d.c:3511 */
+
+/* pll_tridiag.c:152-158: N systems of n rows, solved in place; on return
+   rhs holds the solution and bcof/ccof the eliminated factors */
+void pll_tridiag(Tds *arr, int N, int n);
+
+/* fftw_3d.h:3-5: 2D REDFT10 / REDFT01 over the first two indices of
+   [ndim1][ndim2][num_transforms] arrays; idct divides by 4*ndim1*ndim2 */
+void plan_dct_3d(int ndim1, int ndim2, int num_transforms);
+void fdct_3d(int ndim1, int ndim2, int num_transforms, double ***datain, double ***dataout);
+void idct_3d(int ndim1, int ndim2, int num_transforms, double ***datain, double ***dataout);
+
+/* ===================================== drop-in residency control (new) */
+
+const char *pllc_last_error(void);
+int pllc_to_device(double ***a);    /* host block -> device mirror */
+int pllc_to_host(double ***a);      /* device mirror -> host block */
+int pllc_conc_to_host(Conc *c);
+int pllc_velo_to_host(Velo *u);
+int pllc_conc_to_device(Conc *c);
+int pllc_velo_to_device(Velo *u);
+int pllc_device_synchronize(void);
+
+/* ================================================ handle API (new) ===== */
+
+/* physical set-up of one run; same layout as the oracle's parameters */
+typedef struct pllc_params {
+    int nx, ny, nz;                /* interior cells (param.input:2-6) */
+    double X1, X2, Y1, Y2, Z1, Z2; /* domain (param.input:22-32) */
+    double Re, Pe, Vratio, Fx, Fy; /* physics (param.input:38-46) */
+    double Inf_pos, Inf_thick;     /* front (param.input:48-50) */
+    double CFLmax, Timestep;       /* (param.input:52,36) */
+} pllc_params;
+
+/* field ids of pllc_sim_upload / pllc_sim_download */
+enum {
+    PLLC_C_OLD = 0, PLLC_C_NEW, PLLC_C_N, PLLC_C_NM1,
+    PLLC_U_OLD, PLLC_U_NEW, PLLC_U_PG, PLLC_U_N, PLLC_U_NM1,
+    PLLC_V_OLD, PLLC_V_NEW, PLLC_V_PG, PLLC_V_N, PLLC_V_NM1,
+    PLLC_W_OLD, PLLC_W_NEW, PLLC_W_PG, PLLC_W_N, PLLC_W_NM1,
+    PLLC_MUO_CELL, PLLC_MUO_XY, PLLC_MUO_XZ, PLLC_MUO_YZ,
+    PLLC_MUN_CELL, PLLC_MUN_XY, PLLC_MUN_XZ, PLLC_MUN_YZ,
+    PLLC_PHI, PLLC_DIV,
+    PLLC_NSTATE
+};
+
+/* operator ids of pllc_sim_op: one call of the substage body
+   (pllcart3d.c:169-194) on the simulation's device state */
+enum {
+    PLLC_OP_CONCENTRATION = 0, PLLC_OP_VISCOSITY, PLLC_OP_XMOM, PLLC_OP_YMOM,
+    PLLC_OP_ZMOM, PLLC_OP_VELO_BOUNDARY, PLLC_OP_DIVERGENCE, PLLC_OP_PHISOLVER,
+    PLLC_OP_PHI_GHOSTS, PLLC_OP_VELO_CORRECTION, PLLC_OP_VELOGHOST,
+    PLLC_OP_PRESSURE_GRADIENT, PLLC_OP_UPDATE_OLD, PLLC_OP_BULK_VELO,
+    PLLC_OP_FIND_CFL, PLLC_NOPS
+};
+
+/* seeded initial-condition recipes of pllc_ic_fill */
+enum { PLLC_IC_ERF = 0, PLLC_IC_NOISE = 1, PLLC_IC_MULTIMODE = 2 };
+
+typedef struct pllc_sim pllc_sim;
+
+pllc_sim *pllc_sim_create(const pllc_params *p);
+void pllc_sim_destroy(pllc_sim *s);
+/* dims[0..2] = nx_max, ny_max, nz_max; returns points per field */
+size_t pllc_sim_dims(const pllc_sim *s, int *dims);
+/* set-up of pllcart3d.c:75-154 (Initial = 1, Nmvp = 0); c_full (nullable)
+   replaces the erf profile of initial_condition on every cell */
+int pllc_sim_init(pllc_sim *s, const double *c_full);
+int pllc_sim_upload(pllc_sim *s, int field, const double *host);
+int pllc_sim_download(pllc_sim *s, int field, double *host);
+/* one operator; returns bulk velocity / CFL dt for those ops, else 0;
+   NaN on failure */
+double pllc_sim_op(pllc_sim *s, int op, double dt, int rk, double vbulk);
+/* nsteps whole time steps (CFL + 3 substages) with device-resident dt,
+   vbulk and tval; tval/dt (nullable) receive the values after the last step */
+int pllc_sim_run(pllc_sim *s, int nsteps, double *tval, double *dt);
+double pllc_sim_timestep(const pllc_sim *s);
+int pllc_sim_synchronize(pllc_sim *s);
+/* Fill c_full (nx_max*ny_max*nz_max doubles) with an initial
+   concentration: PLLC_IC_ERF is initial_condition's profile
+   (pllcart3d.c:501-508); NOISE adds seeded uniform noise of amplitude amp
+   on the front; MULTIMODE displaces the front by seeded y/z modes. */
+int pllc_ic_fill(const pllc_params *p, int recipe, unsigned long long seed, double amp, double *c_full);
+
+/* timing on the simulation's stream with HIP events: warmup untimed steps,
+   then nsteps timed; *ms receives the elapsed milliseconds */
+int pllc_sim_time_steps(pllc_sim *s, int warmup, int nsteps, double *ms);
+/* per-launch HIP-event timing of one kernel class during nsteps fused
+   steps; *avg_ms = mean duration of one launch, *launches = count */
+int pllc_sim_time_kernel(pllc_sim *s, const char *name, int nsteps, double *avg_ms, int *launches);
+/* names of the kernel classes known to pllc_sim_time_kernel, ';'-separated */
+const char *pllc_kernel_names(void);
+
+#ifdef __cplusplus
+}
+#endif
+#undef PLLC_NEW
+#endif

@@ -1,0 +1,105 @@
+"""pllcart3d_amd -- the pllcart3d time step (RK3-CN, HJ-WENO5 advection,
+variable-viscosity ADI/CN momentum, DCT Poisson projection) on one MI355X.
+
+The product is the C-ABI library libpllcart3d_amd.so (include/pllcart3d.h)
+and the C driver run3_amd; this package is the Python handle over the same
+library for tests and benchmarks.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (FIELDS, FIELD_ID, IC_ERF, IC_MULTIMODE, IC_NOISE, OP_ID, OPS, Params,
+                   last_error, load)
+from .configs import CONFIGS, make_params
+
+__all__ = ["Sim", "Params", "make_params", "CONFIGS", "ic_fill", "FIELDS", "FIELD_ID", "OPS",
+           "OP_ID", "IC_ERF", "IC_NOISE", "IC_MULTIMODE", "load", "last_error"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ic_fill(params, recipe=IC_ERF, seed=0, amp=0.0):
+    """Initial concentration on every cell (nx+6, ny+2, nz+2), host only."""
+    L = load()
+    out = np.zeros((params.nx + 6, params.ny + 2, params.nz + 2))
+    if L.pllc_ic_fill(C.byref(params), recipe, seed, amp, _dp(out)) != 0:
+        raise RuntimeError(last_error())
+    return out
+
+
+class Sim:
+    """One simulation owned by the library (device state on the GPU)."""
+
+    def __init__(self, params):
+        self.L = load()
+        self.params = params
+        self.h = self.L.pllc_sim_create(C.byref(params))
+        if not self.h:
+            raise RuntimeError("pllc_sim_create: " + last_error())
+        d = (C.c_int * 3)()
+        self.n = self.L.pllc_sim_dims(self.h, d)
+        self.shape = (d[0], d[1], d[2])
+
+    def close(self):
+        if self.h:
+            self.L.pllc_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(what + ": " + last_error())
+
+    def init(self, c_full=None):
+        p = None
+        if c_full is not None:
+            c_full = np.ascontiguousarray(c_full, dtype=np.float64)
+            assert c_full.shape == self.shape
+            p = _dp(c_full)
+        self._chk(self.L.pllc_sim_init(self.h, p), "pllc_sim_init")
+
+    def upload(self, field, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        assert arr.shape == self.shape
+        self._chk(self.L.pllc_sim_upload(self.h, FIELD_ID[field], _dp(arr)), "pllc_sim_upload")
+
+    def download(self, field):
+        out = np.empty(self.shape)
+        self._chk(self.L.pllc_sim_download(self.h, FIELD_ID[field], _dp(out)), "pllc_sim_download")
+        return out
+
+    def op(self, name, dt=0.0, rk=0, vbulk=0.0):
+        v = self.L.pllc_sim_op(self.h, OP_ID[name], dt, rk, vbulk)
+        if v != v:
+            raise RuntimeError("pllc_sim_op(%s): %s" % (name, last_error()))
+        return v
+
+    def run(self, nsteps):
+        t = C.c_double(0.0)
+        dt = C.c_double(0.0)
+        self._chk(self.L.pllc_sim_run(self.h, nsteps, C.byref(t), C.byref(dt)), "pllc_sim_run")
+        return t.value, dt.value
+
+    @property
+    def timestep(self):
+        return self.L.pllc_sim_timestep(self.h)
+
+    def time_steps(self, warmup, nsteps):
+        ms = C.c_double(0.0)
+        self._chk(self.L.pllc_sim_time_steps(self.h, warmup, nsteps, C.byref(ms)), "pllc_sim_time_steps")
+        return ms.value
+
+    def time_kernel(self, name, nsteps):
+        avg = C.c_double(0.0)
+        n = C.c_int(0)
+        self._chk(self.L.pllc_sim_time_kernel(self.h, name.encode(), nsteps, C.byref(avg), C.byref(n)),
+                  "pllc_sim_time_kernel")
+        return avg.value, n.value

@@ -1,0 +1,201 @@
+// pllc_dev.h -- device-side definitions shared by the pllcart3d HIP kernels.
+//
+// Index space: every 3D field is one contiguous [i][j][k] block (k fastest)
+// of nx_max*ny_max*nz_max values with ghost widths (3,1,1), the same linear
+// index as the reference's double*** arrays (allocate_3d_calloc.h:46-64).
+// Interior cells are i in [3, nx+2], j in [1, ny], k in [1, nz]
+// (grid_setup, pllcart3d.c:353-370).
+//
+// The plain FP64 path is compiled with -ffp-contract=off and every
+// expression keeps the reference's operand order, so that the kernels
+// reproduce the reference arithmetic operation by operation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pllc {
+
+struct Dims {
+    int nx, ny, nz;   // interior cells
+    int NX, NY, NZ;   // with ghosts
+    long sx, sy;      // strides of i and j (k stride is 1)
+    double dx, dy, dz;
+};
+
+// a field's unknown range (simulation_setup, pllcart3d.c:405-431)
+struct Rng { int is, ie, js, je, ks, ke; };
+
+struct Phys {
+    double Re, Pe, M, Vnorm, Fx, Fy, CFLmax, Timestep;
+    double Y1, Y2, Z1, Z2;
+};
+
+// field kinds (advected quantity of an H operator / ADI update)
+enum { FC = 0, FU = 1, FV = 2, FW = 3 };
+
+__host__ __device__ inline Rng field_rng(const Dims& d, int f)
+{
+    Rng r{3, d.nx + 2, 1, d.ny, 1, d.nz};
+    if (f == FU) r.is += 1;   // inlet face is not an unknown (405)
+    if (f == FV) r.js += 1;   // bottom wall face (414)
+    if (f == FW) r.ks += 1;   // side face (423)
+    return r;
+}
+
+__host__ __device__ inline long IX(const Dims& d, int i, int j, int k)
+{
+    return (long)i * d.sx + (long)j * d.sy + k;
+}
+
+// Device scalars of a step (written by kernels or the host):
+//   scal[S_DT] dt of the current step, scal[S_CFL] max CFL sum,
+//   scal[S_RK + 4*rk + 0] vbulk of substage rk, [+1] inlet flux,
+//   [+2] outlet flux, [+3] deficit.
+enum { S_DT = 0, S_CFL = 1, S_RK = 4, S_SIZE = 16 };
+
+template <typename T> struct Tr;
+template <> struct Tr<double> { static constexpr double tiny = 1.0e-99; };
+template <> struct Tr<float> { static constexpr float tiny = 1.0e-30f; };
+
+// ---------------------------------------------------------------- WENO
+
+// calculate_weno (pllcart3d.c:3455-3507) on five divided differences.
+template <typename T>
+__device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
+{
+    const T con1 = T(1.0) / T(6.0), con2 = T(13.0) / T(12.0), con3 = T(1.0) / T(4.0);
+    T maxd = T(0.0), sq;
+    sq = v0 * v0; maxd = (maxd > sq) ? maxd : sq;
+    sq = v1 * v1; maxd = (maxd > sq) ? maxd : sq;
+    sq = v2 * v2; maxd = (maxd > sq) ? maxd : sq;
+    sq = v3 * v3; maxd = (maxd > sq) ? maxd : sq;
+    sq = v4 * v4; maxd = (maxd > sq) ? maxd : sq;
+    T c1 = con1 * (T(2.0) * v0 - T(7.0) * v1 + T(11.0) * v2);
+    T c2 = con1 * (T(-1.0) * v1 + T(5.0) * v2 + T(2.0) * v3);
+    T c3 = con1 * (T(2.0) * v2 + T(5.0) * v3 - T(1.0) * v4);
+    T t1 = v0 - T(2.0) * v1 + T(1.0) * v2;
+    T t2 = v0 - T(4.0) * v1 + T(3.0) * v2;
+    T s1 = (con2 * t1 * t1) + (con3 * t2 * t2);
+    t1 = v1 - T(2.0) * v2 + v3;
+    t2 = v1 - v3;
+    T s2 = (con2 * t1 * t1) + (con3 * t2 * t2);
+    t1 = T(1.0) * v2 - T(2.0) * v3 + v4;
+    t2 = T(3.0) * v2 - T(4.0) * v3 + v4;
+    T s3 = (con2 * t1 * t1) + (con3 * t2 * t2);
+    T epi = T(1.0e-6) * maxd + Tr<T>::tiny;
+    T a1 = T(0.1) / ((s1 + epi) * (s1 + epi));
+    T a2 = T(0.6) / ((s2 + epi) * (s2 + epi));
+    T a3 = T(0.3) / ((s3 + epi) * (s3 + epi));
+    T at = a1 + a2 + a3;
+    return (a1 / at) * c1 + (a2 / at) * c2 + (a3 / at) * c3;
+}
+
+// first divided difference D1[m] of a line p[m*st] (e.g. pllcart3d.c:1554)
+template <typename T>
+__device__ __forceinline__ T dd1(const T* __restrict__ p, long st, int m, T h)
+{
+    return (p[(long)(m + 1) * st] - p[(long)m * st]) / h;
+}
+
+// D2[m] over the range [ms, me] a pass writes (1557-1559); an index one
+// outside reads the scratch value an earlier pass left: lo at ms-1, hi at
+// me+1 (computed once per operator call by k_stale).
+template <typename T>
+__device__ __forceinline__ T dd2(const T* __restrict__ p, long st, int m, int ms, int me, T h, T lo, T hi)
+{
+    if (m < ms) return lo;
+    if (m > me) return hi;
+    return (dd1(p, st, m, h) - dd1(p, st, m - 1, h)) / (T(2.0) * h);
+}
+
+// Upwind derivative at point m of a line: ENO2 with the minabs second
+// difference within two points of either end of the range, HJ-WENO5
+// elsewhere (pllcart3d.c:1566-1574 and the matching blocks of every pass).
+template <typename T>
+__device__ __forceinline__ T upwind(const T* __restrict__ p, long st, int m, int ms, int me, int sign,
+                                    T h, T lo, T hi)
+{
+    if ((m <= ms + 1) || (m >= me - 1)) {
+        const int kk = (sign == 1) ? m - 1 : m;
+        const T a = dd2(p, st, kk, ms, me, h, lo, hi);
+        const T b = dd2(p, st, kk + 1, ms, me, h, lo, hi);
+        const T d2 = (fabs(a) <= fabs(b)) ? a : b;
+        return dd1(p, st, kk, h) + d2 * h * T(sign);
+    }
+    const int kk = m - 3 - (sign == 1 ? 0 : -5);   // (int)(2.5*(sign-1))
+    return weno5(dd1(p, st, kk, h), dd1(p, st, kk + sign, h), dd1(p, st, kk + 2 * sign, h),
+                 dd1(p, st, kk + 3 * sign, h), dd1(p, st, kk + 4 * sign, h));
+}
+
+// ------------------------------------------------ diffusion coefficients
+//
+// The reference stores Tri_abc arrays filled by A_conc / A_umom / A_vmom /
+// A_wmom (pllcart3d.c:1411-1536).  Here every coefficient is recomputed from
+// the viscosity fields with the same expression, so A(mu) never occupies
+// HBM.  mu = {cell, xy, xz, yz}.
+template <typename T>
+struct Visc4 { const T* cell; const T* xy; const T* xz; const T* yz; };
+
+template <typename T, int F, int D>
+__device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, int i, int j, int k,
+                                      T& a, T& b, T& c)
+{
+    const T Re = T(ph.Re);
+    if (F == FC) {
+        const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
+        const T del = T(1.0) / (T(ph.Pe) * h * h);
+        a = del; c = del;
+        b = (D == 2) ? (-a - c) : -(a + c);
+        return;
+    }
+    if (F == FU) {
+        if (D == 0) { const T h = T(d.dx), del = T(2.0) / (Re * h);
+            a = del * mu.cell[IX(d, i - 1, j, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; b = -(a + c); }
+        else if (D == 1) { const T h = T(d.dy), del = T(1.0) / (Re * h);
+            a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i, j + 1, k)] / h; b = -(a + c); }
+        else { const T h = T(d.dz), del = T(1.0) / (Re * h);
+            a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i, j, k + 1)] / h; b = -a - c; }
+        return;
+    }
+    if (F == FV) {
+        if (D == 0) { const T h = T(d.dx), del = T(1.0) / (Re * h);
+            a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i + 1, j, k)] / h; }
+        else if (D == 1) { const T h = T(d.dy), del = T(2.0) / (Re * h);
+            a = del * mu.cell[IX(d, i, j - 1, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
+        else { const T h = T(d.dz), del = T(1.0) / (Re * h);
+            a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j, k + 1)] / h; }
+        b = -a - c;
+        return;
+    }
+    // FW
+    if (D == 0) { const T h = T(d.dx), del = T(1.0) / (Re * h);
+        a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i + 1, j, k)] / h; }
+    else if (D == 1) { const T h = T(d.dy), del = T(1.0) / (Re * h);
+        a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j + 1, k)] / h; }
+    else { const T h = T(d.dz), del = T(2.0) / (Re * h);
+        a = del * mu.cell[IX(d, i, j, k - 1)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
+    b = -a - c;
+}
+
+// ADI boundary rows (one rank): NEU b += off, DIR b -= off, RHS0 r -= off*0
+// (pllcart3d.c 610-611, 640-641, 674-676, 1010-1011, 1039-1040, 1075,
+// 1167-1168, 1196-1197, 1231, 1317-1318, 1344-1345, 1378)
+enum { BC_NEU = 0, BC_DIR = 1, BC_RHS0 = 2 };
+__host__ __device__ constexpr int bc_kind(int f, int side)
+{
+    // side: 0 y-lo, 1 y-hi, 2 z-lo, 3 z-hi, 4 x-lo
+    return f == FC ? (side == 4 ? BC_RHS0 : BC_NEU)
+         : f == FU ? (side <= 1 ? BC_DIR : side <= 3 ? BC_NEU : BC_RHS0)
+         : f == FV ? (side <= 1 ? BC_RHS0 : side <= 3 ? BC_NEU : BC_DIR)
+                   : (side <= 1 ? BC_DIR : side <= 3 ? BC_RHS0 : BC_DIR);
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_bc(int kind, T& b, T off, T& r)
+{
+    if (kind == BC_NEU) b += off;
+    else if (kind == BC_DIR) b -= off;
+    else r -= off * T(0.0);
+}
+
+}  // namespace pllc

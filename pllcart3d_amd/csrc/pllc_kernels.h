@@ -1,0 +1,92 @@
+// pllc_kernels.h -- device state of one simulation and the launchers of the
+// substage operators.  Host-visible, no torch types.
+#pragma once
+#include "pllc_dev.h"
+
+namespace pllc {
+
+// Field ids: the state of the reference's Conc/Velo/Visc structs plus phi
+// and div (structures_3d.h:24-53), then the scratch fields of this build.
+enum FieldId {
+    C_OLD = 0, C_NEW, C_N, C_NM1,
+    U_OLD, U_NEW, U_PG, U_N, U_NM1,
+    V_OLD, V_NEW, V_PG, V_N, V_NM1,
+    W_OLD, W_NEW, W_PG, W_N, W_NM1,
+    MUO_CELL, MUO_XY, MUO_XZ, MUO_YZ,
+    MUN_CELL, MUN_XY, MUN_XZ, MUN_YZ,
+    PHI, DIV,
+    NSTATE,
+    F_SRC = NSTATE,   // ADI source / solution (the reference's `source`)
+    F_CC,             // Thomas factors of the current sweep
+    NFIELDS
+};
+
+// RK3 low-storage coefficients (global_variables.h:17-18)
+static const double kRK1[3] = {(8.0 / 15.0), (5.0 / 12.0), (3.0 / 4.0)};
+static const double kRK2[3] = {(0.0), (-17.0 / 60.0), (-5.0 / 12.0)};
+
+template <typename T>
+struct Ctx {
+    Dims d;
+    Phys ph;
+    hipStream_t st = nullptr;
+    T* f[NFIELDS] = {};
+    T* P = nullptr;      // packed [nx][ny][nz] Poisson block
+    T* Q = nullptr;      // second packed block
+    T* QC = nullptr;     // Thomas factors of the Poisson x-lines
+    T* stale = nullptr;  // 16 stale-D2 values
+    T* lam_y = nullptr;  // Poisson eigenvalues (pllcart3d.c:2050-2051)
+    T* lam_z = nullptr;
+    T* Cfy = nullptr;    // REDFT10 matrices (ny x ny, nz x nz)
+    T* Cfz = nullptr;
+    T* Ciy = nullptr;    // REDFT01 matrices
+    T* Ciz = nullptr;
+    double* scal = nullptr;   // S_SIZE device scalars
+    double* part = nullptr;   // reduction partials
+    double* tval = nullptr;   // device copy of tval
+    long npart = 0;
+    // optional per-launch hook (phase 0 before, 1 after a launch of the
+    // named kernel class), used for HIP-event timing of one kernel class
+    void (*hook)(void* user, const char* name, int phase) = nullptr;
+    void* hook_user = nullptr;
+};
+
+// Operator launchers.  g, r are the RK coefficients (rk_gamma, rk_rho) the
+// reference passes by value; `slot` selects the device scalars of the
+// substage (vbulk, fluxes, deficit).  All launches go to c.st.
+template <typename T>
+struct Ops {
+    static void stale(Ctx<T>& c, const T* qc, const T* qu, const T* qv, const T* qw);
+    static void concentration(Ctx<T>& c, double g, double r, int slot, const T* un, const T* vn, const T* wn);
+    static void viscosity(Ctx<T>& c, const T* cn, T* const mu[4]);
+    static void momentum(Ctx<T>& c, int comp, double g, double r, int slot);
+    static void velo_ghost(Ctx<T>& c);
+    static void divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack);
+    static void poisson(Ctx<T>& c, double g, double r, bool from_div);
+    static void phi_ghost(Ctx<T>& c);
+    static void velo_correction(Ctx<T>& c, double g, double r, int slot);
+    // mask bit 0 u, bit 1 v, bit 2 w; coefficients A(mu) from c.f[MUN_*]
+    static void pressure_gradient(Ctx<T>& c, double g, double r, int mask);
+    static void cfl(Ctx<T>& c);
+    static void bulk(Ctx<T>& c, int slot, double alpha, bool advance_t);
+    static void copy(Ctx<T>& c, const T* a, T* b, long n);
+    static void fill(Ctx<T>& c, T* a, long n, T v);
+    static void set_scalar(Ctx<T>& c, int idx, double v);
+    // dense 2D DCT of a packed block: forward REDFT10 (z then y) or inverse
+    // REDFT01 (y then z), `in` -> `out` through the temporary c.Q
+    static void dct2(Ctx<T>& c, const T* in, T* out, bool inverse);
+};
+
+// Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)
+void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
+                  hipStream_t st);
+void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int n, hipStream_t st);
+// C (M x N) = A (M x K) * B (K x N), batched with strides, FP64 (fdct_3d)
+void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,
+             hipStream_t st);
+void scale(double* a, long n, double div, hipStream_t st);
+
+// ';'-separated list of the kernel class names passed to Ctx::hook
+extern const char* kernel_class_names;
+
+}  // namespace pllc

@@ -1,0 +1,1142 @@
+// pllc_kernels.hip -- the HIP kernels of one pllcart3d RK3-CN substage.
+//
+// Every kernel restates one reference routine (cited per kernel) with the
+// same operand order; with -ffp-contract=off the FP64 instantiation repeats
+// the reference arithmetic operation by operation.  Launch wrappers live in
+// pllc_runtime.hip; all kernels are templated on the field type T so that
+// the same code serves the FP64 plain path and the FP32 mixed path.
+#include "pllc_dev.h"
+#include "pllc_kernels.h"
+
+namespace pllc {
+
+// ------------------------------------------------------------ stale D2
+
+// The ENO boundary rule of Hc/Hu/Hv/Hw_op reads D2 one index outside the
+// range its pass just wrote and sees what an earlier pass of the same call
+// left in the shared scratch array (pllcart3d.c:1546, 1568, 1596, 1626).
+// For every operator f (c,u,v,w) and pass (y,z) this kernel computes the
+// two values such reads return: [f*4 + 0] y-pass at js-1, [+1] y-pass at
+// je+1, [+2] z-pass at ks-1, [+3] z-pass at ke+1.  The x pass runs first on
+// a zeroed scratch, so its outside reads are 0.
+template <typename T>
+__global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ q1, const T* __restrict__ q2,
+                        const T* __restrict__ q3, T* __restrict__ stale)
+{
+    const int t = threadIdx.x;
+    if (t >= 16) return;
+    const int f = t >> 2, w = t & 3;
+    const T* q = f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : q3;
+    const Rng R = field_rng(d, f);
+    const T hx = T(d.dx), hy = T(d.dy);
+    // D2 of the x pass's last line (j = je, k = ke) and of the y pass's last
+    // line (i = ie, k = ke), at index m
+    const T* px = q + IX(d, 0, R.je, R.ke);
+    const T* py = q + IX(d, R.ie, 0, R.ke);
+    T v = T(0.0);
+    if (w < 2) {
+        const int m = w == 0 ? R.js - 1 : R.je + 1;
+        if (m >= R.is && m <= R.ie) v = (dd1(px, d.sx, m, hx) - dd1(px, d.sx, m - 1, hx)) / (T(2.0) * hx);
+    } else {
+        const int m = w == 2 ? R.ks - 1 : R.ke + 1;
+        if (m >= R.js && m <= R.je) v = (dd1(py, d.sy, m, hy) - dd1(py, d.sy, m - 1, hy)) / (T(2.0) * hy);
+        else if (m >= R.is && m <= R.ie) v = (dd1(px, d.sx, m, hx) - dd1(px, d.sx, m - 1, hx)) / (T(2.0) * hx);
+    }
+    stale[t] = v;
+}
+
+// ------------------------------------------------- explicit terms + source
+
+// concentration's explicit part: Hc_op (1539-1644) and the source loop
+// (573-590).  One thread per interior cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restrict__ cold, const T* __restrict__ un,
+                                               const T* __restrict__ vn, const T* __restrict__ wn,
+                                               const T* __restrict__ Nm1, T* __restrict__ N, T* __restrict__ S,
+                                               const T* __restrict__ stale, const double* __restrict__ scal, T g, T r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const Rng R = field_rng(d, FC);
+    const long o = IX(d, i, j, k);
+    const T dt = T(scal[S_DT]);
+
+    const T uval = T(0.5) * (un[o + d.sx] + un[o]);
+    const T cxd = upwind(cold + IX(d, 0, j, k), d.sx, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    const T convx = uval * cxd;
+    const T vval = T(0.5) * (vn[o + d.sy] + vn[o]);
+    const T cyd = upwind(cold + IX(d, i, 0, k), d.sy, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    const T convy = vval * cyd;
+    const T wval = T(0.5) * (wn[o + 1] + wn[o]);
+    const T czd = upwind(cold + IX(d, i, j, 0), 1, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    const T nval = -(wval * czd + convx + convy);
+    N[o] = nval;
+
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    Visc4<T> none{nullptr, nullptr, nullptr, nullptr};
+    coefs<T, FC, 0>(d, ph, none, i, j, k, ax, bx, cx);
+    coefs<T, FC, 1>(d, ph, none, i, j, k, ay, by, cy);
+    coefs<T, FC, 2>(d, ph, none, i, j, k, az, bz, cz);
+    const T conv = (g * nval + r * Nm1[o]);
+    const T axx = ax * cold[o - d.sx] + bx * cold[o] + cx * cold[o + d.sx];
+    const T axy = ay * cold[o - d.sy] + by * cold[o] + cy * cold[o + d.sy];
+    const T axz = az * cold[o - 1] + bz * cold[o] + cz * cold[o + 1];
+    const T diff = (g + r) * (axx + axy + axz);
+    S[o] = dt * (conv + diff);
+}
+
+// Advecting velocity of Hu/Hv/Hw in direction D (1669/1696/1722,
+// 1780/1807/1834, 1892/1919/1946): the component itself along its own
+// direction, 4-point averages of the others.
+template <typename T, int F, int D>
+__device__ __forceinline__ T mom_vel(const Dims& d, const T* __restrict__ u, const T* __restrict__ v,
+                                     const T* __restrict__ w, int i, int j, int k)
+{
+    if (F == FU) {
+        if (D == 0) return u[IX(d, i, j, k)];
+        if (D == 1) return T(0.25) * (v[IX(d, i - 1, j, k)] + v[IX(d, i, j, k)] + v[IX(d, i, j + 1, k)] + v[IX(d, i - 1, j + 1, k)]);
+        return T(0.25) * (w[IX(d, i - 1, j, k)] + w[IX(d, i, j, k)] + w[IX(d, i, j, k + 1)] + w[IX(d, i - 1, j, k + 1)]);
+    }
+    if (F == FV) {
+        if (D == 0) return T(0.25) * (u[IX(d, i, j - 1, k)] + u[IX(d, i + 1, j - 1, k)] + u[IX(d, i, j, k)] + u[IX(d, i + 1, j, k)]);
+        if (D == 1) return v[IX(d, i, j, k)];
+        return T(0.25) * (w[IX(d, i, j - 1, k)] + w[IX(d, i, j - 1, k + 1)] + w[IX(d, i, j, k)] + w[IX(d, i, j, k + 1)]);
+    }
+    if (D == 0) return T(0.25) * (u[IX(d, i, j, k - 1)] + u[IX(d, i + 1, j, k - 1)] + u[IX(d, i, j, k)] + u[IX(d, i + 1, j, k)]);
+    if (D == 1) return T(0.25) * (v[IX(d, i, j, k - 1)] + v[IX(d, i, j, k)] + v[IX(d, i, j + 1, k - 1)] + v[IX(d, i, j + 1, k)]);
+    return w[IX(d, i, j, k)];
+}
+
+// explicit_n of a velocity component: -convec_term + the explicit
+// cross-viscous terms with mu_old, left to right (Hu 1734-1744,
+// Hv 1846-1856, Hw 1958-1970 with the live index of 1962).
+template <typename T, int F>
+__device__ __forceinline__ T mom_explicit(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ u,
+                                          const T* __restrict__ v, const T* __restrict__ w, int i, int j, int k, T ct)
+{
+    const T Re = T(ph.Re), dx = T(d.dx), dy = T(d.dy), dz = T(d.dz);
+    T del, p1, m1, p2, m2;
+    if (F == FU) {
+        del = T(1.0) / (Re * dy * dx);
+        p1 = del * mu.xy[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i - 1, j + 1, k)]);
+        m1 = del * mu.xy[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i - 1, j, k)]);
+        del = T(1.0) / (Re * dz * dx);
+        p2 = del * mu.xz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i - 1, j, k + 1)]);
+        m2 = del * mu.xz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i - 1, j, k)]);
+    } else if (F == FV) {
+        del = T(1.0) / (Re * dx * dy);
+        p1 = del * mu.xy[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j - 1, k)]);
+        m1 = del * mu.xy[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j - 1, k)]);
+        del = T(1.0) / (Re * dz * dy);
+        p2 = del * mu.yz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i, j - 1, k + 1)]);
+        m2 = del * mu.yz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i, j - 1, k)]);
+    } else {
+        del = T(1.0) / (Re * dx * dz);
+        p1 = del * mu.xz[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j, k - 1)]);
+        m1 = del * mu.xz[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j, k - 1)]);
+        del = T(1.0) / (Re * dy * dz);
+        p2 = del * mu.yz[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i, j + 1, k - 1)]);
+        m2 = del * mu.yz[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i, j, k - 1)]);
+    }
+    return -ct + p1 - m1 + p2 - m2;
+}
+
+// xmom/ymom/zmom explicit part: H?_op with mu_old, then the source loop
+// with A(mu_old) and the body force (970-989, 1127-1146, 1280-1298).
+// u, v, w are the old velocities; q is the component's own old field.
+template <typename T, int F>
+__global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
+                                                 const T* __restrict__ w, Visc4<T> mu, const T* __restrict__ cn,
+                                                 const T* __restrict__ pg, const T* __restrict__ Nm1, T* __restrict__ N,
+                                                 T* __restrict__ S, const T* __restrict__ stale,
+                                                 const double* __restrict__ scal, T g, T r)
+{
+    const Rng R = field_rng(d, F);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + R.ks;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + R.js;
+    const int i = blockIdx.z + R.is;
+    if (k > R.ke || j > R.je) return;
+    const T* __restrict__ q = F == FU ? u : F == FV ? v : w;
+    const long o = IX(d, i, j, k);
+    const T dt = T(scal[S_DT]);
+    const T alpha = g + r;
+
+    const T vx = mom_vel<T, F, 0>(d, u, v, w, i, j, k);
+    const T dxd = upwind(q + IX(d, 0, j, k), d.sx, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    const T convx = vx * dxd;
+    const T vy = mom_vel<T, F, 1>(d, u, v, w, i, j, k);
+    const T dyd = upwind(q + IX(d, i, 0, k), d.sy, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    const T convy = vy * dyd;
+    const T vz = mom_vel<T, F, 2>(d, u, v, w, i, j, k);
+    const T dzd = upwind(q + IX(d, i, j, 0), 1, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    const T ct = vz * dzd + convx + convy;
+    const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, i, j, k, ct);
+    N[o] = nval;
+
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    coefs<T, F, 0>(d, ph, mu, i, j, k, ax, bx, cx);
+    coefs<T, F, 1>(d, ph, mu, i, j, k, ay, by, cy);
+    coefs<T, F, 2>(d, ph, mu, i, j, k, az, bz, cz);
+    const T conv = (g * nval + r * Nm1[o]);
+    const T axx = ax * q[o - d.sx] + bx * q[o] + cx * q[o + d.sx];
+    const T axy = ay * q[o - d.sy] + by * q[o] + cy * q[o + d.sy];
+    const T axz = az * q[o - 1] + bz * q[o] + cz * q[o + 1];
+    const T visc = alpha * (axx + axy + axz);
+    const T Re = T(ph.Re);
+    T bodyforce;
+    if (F == FU) {
+        const T cval = T(0.5) * (cn[o] + cn[o - d.sx]);
+        bodyforce = alpha * (-(T(ph.Fx) / Re) * cval - (pg[o] / Re));
+    } else if (F == FV) {
+        const T cval = T(0.5) * (cn[o] + cn[o - d.sy]);
+        bodyforce = alpha * (-(T(ph.Fy) / Re) * cval - (pg[o] / Re));
+    } else {
+        bodyforce = alpha * (-(pg[o] / Re));
+    }
+    S[o] = dt * (conv + visc + bodyforce);
+}
+
+// --------------------------------------------------------- line solvers
+
+// y-lines of concentration / xmom / ymom / zmom: tridiag_1rhs
+// (3511-3536) on (I - fac A_y) with the field's wall rows, in place in S;
+// the Thomas factors cc go to the scratch field CC.  One thread per (i,k)
+// line, consecutive threads on consecutive k (coalesced).
+template <typename T, int F>
+__global__ __launch_bounds__(256) void k_sweep_y(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
+                                                 const double* __restrict__ scal, T g, T r)
+{
+    const Rng R = field_rng(d, F);
+    const int nk = R.ke - R.ks + 1, n = R.je - R.js + 1;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (long)nk * (R.ie - R.is + 1)) return;
+    const int k = R.ks + (int)(l % nk), i = R.is + (int)(l / nk);
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+    T pc = T(0.0), pr = T(0.0);
+    for (int x = 0; x < n; x++) {
+        const int j = R.js + x;
+        const long o = IX(d, i, j, k);
+        T A, B, Cc;
+        coefs<T, F, 1>(d, ph, mu, i, j, k, A, B, Cc);
+        const T a = -fac * A;
+        T b = T(1.0) - fac * B;
+        const T c = -fac * Cc;
+        T rhs = S[o];
+        if (x == 0) apply_bc(bc_kind(F, 0), b, a, rhs);
+        if (x == n - 1) apply_bc(bc_kind(F, 1), b, c, rhs);
+        T bb, cc, rr;
+        if (x == 0) { bb = b; cc = c / bb; rr = rhs / bb; }
+        else { bb = b - pc * a; cc = c / bb; rr = (rhs - pr * a) / bb; }
+        CC[o] = cc; S[o] = rr;
+        pc = cc; pr = rr;
+    }
+    for (int x = n - 2; x >= 0; x--) {
+        const long o = IX(d, i, R.js + x, k);
+        const T rr = S[o] - pr * CC[o];
+        S[o] = rr;
+        pr = rr;
+    }
+}
+
+// z-lines: as k_sweep_y along k.  One thread per (i,j) line.
+template <typename T, int F>
+__global__ __launch_bounds__(256) void k_sweep_z(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
+                                                 const double* __restrict__ scal, T g, T r)
+{
+    const Rng R = field_rng(d, F);
+    const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (long)nj * (R.ie - R.is + 1)) return;
+    const int j = R.js + (int)(l % nj), i = R.is + (int)(l / nj);
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+    T pc = T(0.0), pr = T(0.0);
+    for (int x = 0; x < n; x++) {
+        const int k = R.ks + x;
+        const long o = IX(d, i, j, k);
+        T A, B, Cc;
+        coefs<T, F, 2>(d, ph, mu, i, j, k, A, B, Cc);
+        const T a = -fac * A;
+        T b = T(1.0) - fac * B;
+        const T c = -fac * Cc;
+        T rhs = S[o];
+        if (x == 0) apply_bc(bc_kind(F, 2), b, a, rhs);
+        if (x == n - 1) apply_bc(bc_kind(F, 3), b, c, rhs);
+        T bb, cc, rr;
+        if (x == 0) { bb = b; cc = c / bb; rr = rhs / bb; }
+        else { bb = b - pc * a; cc = c / bb; rr = (rhs - pr * a) / bb; }
+        CC[o] = cc; S[o] = rr;
+        pc = cc; pr = rr;
+    }
+    for (int x = n - 2; x >= 0; x--) {
+        const long o = IX(d, i, j, R.ks + x);
+        const T rr = S[o] - pr * CC[o];
+        S[o] = rr;
+        pr = rr;
+    }
+}
+
+// x-lines: pll_tridiag at one rank (pll_tridiag.c:21-47, 96-110) with the
+// inlet row and the explicit convective outflow face (concentration
+// 657-698, xmom 1056-1102, ymom 1213-1255, zmom 1361-1403), then
+// q_new = q_old + delta.  For c it also fills the y then z Neumann ghosts of
+// c_new (700-712).  One thread per (j,k) line.
+template <typename T, int F>
+__global__ __launch_bounds__(256) void k_sweep_x(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
+                                                 const T* __restrict__ qold, T* __restrict__ qnew, T* __restrict__ N,
+                                                 const T* __restrict__ Nm1, const double* __restrict__ scal, int rk,
+                                                 T g, T r)
+{
+    const Rng R = field_rng(d, F);
+    const int nk = R.ke - R.ks + 1, n = R.ie - R.is + 1;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (long)nk * (R.je - R.js + 1)) return;
+    const int k = R.ks + (int)(l % nk), j = R.js + (int)(l / nk);
+    const T dt = T(scal[S_DT]);
+    const T vbulk = T(scal[S_RK + 4 * rk]);
+    const T fac = T(0.5) * dt * (g + r);
+
+    // explicit outflow face iend+1
+    const long ob = IX(d, R.ie + 1, j, k);
+    const T nb = -vbulk * (qold[ob] - qold[ob - d.sx]) / T(d.dx);
+    N[ob] = nb;
+    const T bcor = dt * (g * nb + r * Nm1[ob]);
+    qnew[ob] = qold[ob] + bcor;
+
+    const T zero = T(0.0);
+    T pc = T(0.0), pr = T(0.0);
+    for (int x = 0; x < n; x++) {
+        const int i = R.is + x;
+        const long o = IX(d, i, j, k);
+        T A, B, Cc;
+        coefs<T, F, 0>(d, ph, mu, i, j, k, A, B, Cc);
+        const T a = -fac * A;
+        T b = T(1.0) - fac * B;
+        T c = -fac * Cc;
+        T rhs = S[o];
+        if (x == 0) apply_bc(bc_kind(F, 4), b, a, rhs);
+        if (x == n - 1) rhs -= c * bcor;
+        if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
+        else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
+        CC[o] = c; S[o] = rhs;
+        pc = c; pr = rhs;
+    }
+    pr = pr - pc * zero;
+    {
+        const long o = IX(d, R.ie, j, k);
+        qnew[o] = qold[o] + pr;
+    }
+    for (int x = n - 2; x >= 0; x--) {
+        const long o = IX(d, R.is + x, j, k);
+        const T rr = S[o] - CC[o] * pr;
+        qnew[o] = qold[o] + rr;
+        pr = rr;
+    }
+    if (F == FC) {
+        // ghost copies of this line's values: y ghosts (701-704), then z
+        // ghosts incl. the y-ghost rows (707-712)
+        for (int i = R.is; i <= R.ie; i++) {
+            const T val = qnew[IX(d, i, j, k)];
+            if (j == R.js) qnew[IX(d, i, j - 1, k)] = val;
+            if (j == R.je) qnew[IX(d, i, j + 1, k)] = val;
+            if (k == R.ks) {
+                qnew[IX(d, i, j, k - 1)] = val;
+                if (j == R.js) qnew[IX(d, i, j - 1, k - 1)] = val;
+                if (j == R.je) qnew[IX(d, i, j + 1, k - 1)] = val;
+            }
+            if (k == R.ke) {
+                qnew[IX(d, i, j, k + 1)] = val;
+                if (j == R.js) qnew[IX(d, i, j - 1, k + 1)] = val;
+                if (j == R.je) qnew[IX(d, i, j + 1, k + 1)] = val;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ viscosity
+
+// calculate_viscosity (843-886): cell centres on a one-cell ring, and
+// edges from the 4-point mean of c (mean first, then exp).
+template <typename T>
+__global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restrict__ cn, T* __restrict__ cell,
+                                              T* __restrict__ xy, T* __restrict__ xz, T* __restrict__ yz)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;       // 0 .. nz+1
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;       // 0 .. ny+1
+    const int i = blockIdx.z + 2;                              // 2 .. nx+3
+    if (k > d.nz + 1 || j > d.ny + 1) return;
+    const T M = T(ph.M), Vn = T(ph.Vnorm);
+    const long o = IX(d, i, j, k);
+    cell[o] = exp(M * (cn[o] - Vn));
+    const bool ix = i >= 3, iy = j >= 1, iz = k >= 1;
+    const bool jx = j <= d.ny, kx = k <= d.nz, ii = i <= d.nx + 2;
+    if (ix && iy && kx && iz) {
+        const T cavg = T(0.25) * (cn[o - d.sx - d.sy] + cn[o - d.sy] + cn[o - d.sx] + cn[o]);
+        xy[o] = exp(M * (cavg - Vn));
+    }
+    if (ix && iy && jx && iz) {
+        const T cavg = T(0.25) * (cn[o - d.sx - 1] + cn[o - 1] + cn[o - d.sx] + cn[o]);
+        xz[o] = exp(M * (cavg - Vn));
+    }
+    if (ix && ii && iy && iz) {
+        const T cavg = T(0.25) * (cn[o - d.sy - 1] + cn[o - 1] + cn[o - d.sy] + cn[o]);
+        yz[o] = exp(M * (cavg - Vn));
+    }
+}
+
+// ----------------------------------------------------------- ghost rules
+
+// update_velo_boundary (2701-2755) == update_veloghost_points (2810-2885)
+// at one rank.  Three disjoint sets: z ghosts of u,v; y ghosts of u,w;
+// inlet ghosts of v,w.
+template <typename T>
+__global__ void k_velo_ghost(Dims d, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w)
+{
+    const long nA = (long)d.nx * d.ny, nB = (long)d.nx * d.nz, nC = (long)d.ny * d.nz;
+    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nA) {
+        const int i = 3 + (int)(l / d.ny), j = 1 + (int)(l % d.ny);
+        const long lo = IX(d, i, j, 0), hi = IX(d, i, j, d.nz + 1);
+        u[lo] = u[lo + 1]; v[lo] = v[lo + 1];
+        u[hi] = u[hi - 1]; v[hi] = v[hi - 1];
+        return;
+    }
+    l -= nA;
+    if (l < nB) {
+        const int i = 3 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const long lo = IX(d, i, 0, k), hi = IX(d, i, d.ny + 1, k);
+        u[lo] = -u[lo + d.sy]; w[lo] = -w[lo + d.sy];
+        u[hi] = -u[hi - d.sy]; w[hi] = -w[hi - d.sy];
+        return;
+    }
+    l -= nB;
+    if (l < nC) {
+        const int j = 1 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const long lo = IX(d, 2, j, k);
+        v[lo] = -v[lo + d.sx]; w[lo] = -w[lo + d.sx];
+    }
+}
+
+// update_phi_ghosts (2758-2807): Neumann on every face.
+template <typename T>
+__global__ void k_phi_ghost(Dims d, T* __restrict__ phi)
+{
+    const long nA = (long)d.nx * d.ny, nB = (long)d.nx * d.nz, nC = (long)d.ny * d.nz;
+    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nA) {
+        const int i = 3 + (int)(l / d.ny), j = 1 + (int)(l % d.ny);
+        const long lo = IX(d, i, j, 0), hi = IX(d, i, j, d.nz + 1);
+        phi[lo] = phi[lo + 1]; phi[hi] = phi[hi - 1];
+        return;
+    }
+    l -= nA;
+    if (l < nB) {
+        const int i = 3 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const long lo = IX(d, i, 0, k), hi = IX(d, i, d.ny + 1, k);
+        phi[lo] = phi[lo + d.sy]; phi[hi] = phi[hi - d.sy];
+        return;
+    }
+    l -= nB;
+    if (l < nC) {
+        const int j = 1 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const long lo = IX(d, 2, j, k), hi = IX(d, d.nx + 3, j, k);
+        phi[lo] = phi[lo + d.sx]; phi[hi] = phi[hi - d.sx];
+    }
+}
+
+// ------------------------------------------------------------ projection
+
+// divergence (1983-1999).  Writes div (if non-null) and/or the scaled
+// Poisson source Re/(dt*alpha)*div packed as [x][y][z] (2018-2024).
+template <typename T>
+__global__ __launch_bounds__(256) void k_div(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
+                                             const T* __restrict__ w, T* __restrict__ div, T* __restrict__ P,
+                                             const double* __restrict__ scal, T g, T r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const long o = IX(d, i, j, k);
+    const T dvx = (u[o + d.sx] - u[o]) / T(d.dx);
+    const T dvy = (v[o + d.sy] - v[o]) / T(d.dy);
+    const T dvz = (w[o + 1] - w[o]) / T(d.dz);
+    const T dv = (dvx + dvy + dvz);
+    if (div) div[o] = dv;
+    if (P) {
+        const T fac = T(ph.Re) / (T(scal[S_DT]) * (g + r));
+        P[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)] = fac * dv;
+    }
+}
+
+// phisolver_dct packing of an existing div field (2016-2024)
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_div(Dims d, Phys ph, const T* __restrict__ div, T* __restrict__ P,
+                                                  const double* __restrict__ scal, T g, T r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const T fac = T(ph.Re) / (T(scal[S_DT]) * (g + r));
+    P[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)] = fac * div[IX(d, i, j, k)];
+}
+
+// Per-mode x-line solve of the Poisson problem in (y,z) DCT space
+// (2037-2091): a = c = 1/dx^2, b = -2/dx^2 + lam_y + lam_z, Neumann at the
+// inlet; at the outlet Neumann except mode (0,0), whose ghost phi is the
+// constant 0.  One thread per mode, coalesced over kz; in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pois_x(Dims d, const T* __restrict__ lam_y, const T* __restrict__ lam_z,
+                                                T* __restrict__ P, T* __restrict__ CC)
+{
+    const long nm = (long)d.ny * d.nz;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nm) return;
+    const int jy = (int)(l / d.nz), kz = (int)(l % d.nz), n = d.nx;
+    const T dx2i = T(1.0) / (T(d.dx) * T(d.dx));
+    const T phiconst = T(0.0), zero = T(0.0);
+    const T bmid = T(-2.0) * dx2i + lam_y[jy] + lam_z[kz];
+    T pc = T(0.0), pr = T(0.0);
+    for (int x = 0; x < n; x++) {
+        const long o = (long)x * nm + l;
+        const T a = dx2i;
+        T b = bmid, c = dx2i, rhs = P[o];
+        if (x == 0) b += a;
+        if (x == n - 1) {
+            if (jy == 0 && kz == 0) rhs -= c * phiconst;
+            else b += c;
+        }
+        if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
+        else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
+        CC[o] = c; P[o] = rhs;
+        pc = c; pr = rhs;
+    }
+    pr = pr - pc * zero;
+    P[(long)(n - 1) * nm + l] = pr;
+    for (int x = n - 2; x >= 0; x--) {
+        const long o = (long)x * nm + l;
+        const T rr = P[o] - CC[o] * pr;
+        P[o] = rr;
+        pr = rr;
+    }
+}
+
+// idct normalisation and copy back into phi (fftw_3d.h:101,
+// pllcart3d.c:2106-2112)
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi_scatter(Dims d, const T* __restrict__ H, T* __restrict__ phi)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const T norm = T(4 * d.ny * d.nz);
+    phi[IX(d, i, j, k)] = H[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)] / norm;
+}
+
+// velo_correction (2144-2169): u -= dt*alpha/Re * grad(phi) on each
+// component's unknowns (so neither the inlet nor the outlet face).
+template <typename T>
+__global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __restrict__ phi, T* __restrict__ u,
+                                                   T* __restrict__ v, T* __restrict__ w,
+                                                   const double* __restrict__ scal, T g, T r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const long o = IX(d, i, j, k);
+    const T fac = T(scal[S_DT]) * (g + r) / T(ph.Re);
+    if (j >= 2) {
+        const T dp = (phi[o] - phi[o - d.sy]) / T(d.dy);
+        v[o] = v[o] - fac * dp;
+    }
+    if (k >= 2) {
+        const T dp = (phi[o] - phi[o - 1]) / T(d.dz);
+        w[o] = w[o] - fac * dp;
+    }
+    if (i >= 4) {
+        const T dp = (phi[o] - phi[o - d.sx]) / T(d.dx);
+        u[o] = u[o] - fac * dp;
+    }
+}
+
+// Inflow/outflow trapezoids over the interior nodes of the inlet face and
+// the outlet face (2171-2210) in the reference's summation order, FP64,
+// then the uniform deficit added to the outlet face (2216-2223).
+// One block; ny <= 1024 handled with a strided loop over j.
+template <typename T>
+__global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __restrict__ u, double* __restrict__ scal,
+                                                      int rk)
+{
+    __shared__ double qk_in[1056], qk_out[1056];
+    __shared__ double s_def;
+    const double hy = d.dy, hz = d.dz;
+    const int iin = 3, iout = d.nx + 3;
+    for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) {
+        double qi = 0.0, qo = 0.0;
+        for (int k = 2; k <= d.nz; k++) {
+            qi += 0.5 * hz * ((double)u[IX(d, iin, j, k)] + (double)u[IX(d, iin, j, k - 1)]);
+            qo += 0.5 * hz * ((double)u[IX(d, iout, j, k)] + (double)u[IX(d, iout, j, k - 1)]);
+        }
+        qk_in[j] = qi;
+        qk_out[j] = qo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double qin = 0.0, qout = 0.0;
+        for (int j = 2; j <= d.ny; j++) {
+            qin += 0.5 * hy * (qk_in[j - 1] + qk_in[j]);
+            qout += 0.5 * hy * (qk_out[j - 1] + qk_out[j]);
+        }
+        const double deficit = (qin - qout) / ((ph.Y2 - ph.Y1) * (ph.Z2 - ph.Z1));
+        scal[S_RK + 4 * rk + 1] = qin;
+        scal[S_RK + 4 * rk + 2] = qout;
+        scal[S_RK + 4 * rk + 3] = deficit;
+        s_def = deficit;
+    }
+    __syncthreads();
+    const T def = T(s_def);
+    const long np = (long)d.ny * d.nz;
+    for (long l = threadIdx.x; l < np; l += blockDim.x) {
+        const long o = IX(d, iout, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz));
+        u[o] = u[o] + def;
+    }
+}
+
+// pressure_gradient (2964-3044): pg += grad(phi) - fac A(mu_new) grad(phi)
+// for u, v and w at their unknowns; the gradients at the 7 stencil points
+// are recomputed from phi instead of stored.
+template <typename T, int F>
+__device__ __forceinline__ T pgrad(const Dims& d, const T* __restrict__ phi, long o)
+{
+    if (F == FU) return (phi[o] - phi[o - d.sx]) / T(d.dx);
+    if (F == FV) return (phi[o] - phi[o - d.sy]) / T(d.dy);
+    return (phi[o] - phi[o - 1]) / T(d.dz);
+}
+
+template <typename T, int F>
+__device__ __forceinline__ void pg_update(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ phi,
+                                          T* __restrict__ pg, int i, int j, int k, T fac)
+{
+    const long o = IX(d, i, j, k);
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    coefs<T, F, 0>(d, ph, mu, i, j, k, ax, bx, cx);
+    coefs<T, F, 1>(d, ph, mu, i, j, k, ay, by, cy);
+    coefs<T, F, 2>(d, ph, mu, i, j, k, az, bz, cz);
+    const T g0 = pgrad<T, F>(d, phi, o);
+    const T pxx = (ax) * pgrad<T, F>(d, phi, o - d.sx) + (bx) * g0 + (cx) * pgrad<T, F>(d, phi, o + d.sx);
+    const T pyy = (ay) * pgrad<T, F>(d, phi, o - d.sy) + (by) * g0 + (cy) * pgrad<T, F>(d, phi, o + d.sy);
+    const T pzz = (az) * pgrad<T, F>(d, phi, o - 1) + (bz) * g0 + (cz) * pgrad<T, F>(d, phi, o + 1);
+    pg[o] = pg[o] + g0 - fac * (pyy + pxx + pzz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
+                                            T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
+                                            const double* __restrict__ scal, T g, T r, int mask)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+    if ((mask & 2) && j >= 2) pg_update<T, FV>(d, ph, mu, phi, pgv, i, j, k, fac);
+    if ((mask & 1) && i >= 4) pg_update<T, FU>(d, ph, mu, phi, pgu, i, j, k, fac);
+    if ((mask & 4) && k >= 2) pg_update<T, FW>(d, ph, mu, phi, pgw, i, j, k, fac);
+}
+
+// ------------------------------------------------------------ reductions
+
+// find_cfl (3095-3118), pass 1: per-block max of
+// |u|/dx + |v|/dy + |w|/dz at cell centres.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict__ u, const T* __restrict__ v,
+                                                     const T* __restrict__ w, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (k <= d.nz && j <= d.ny) {
+        const long o = IX(d, i, j, k);
+        const double uv = 0.5 * ((double)u[o] + (double)u[o + d.sx]);
+        const double vv = 0.5 * ((double)v[o] + (double)v[o + d.sy]);
+        const double wv = 0.5 * ((double)w[o] + (double)w[o + 1]);
+        m = (fabs(uv) / d.dx) + (fabs(vv) / d.dy) + (fabs(wv) / d.dz);
+    }
+    red[t] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] = red[t] > red[t + s] ? red[t] : red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// pass 2: max over the partials; dt = min(CFLmax/max, Timestep)
+// (pllcart3d.c:159-160, 3115)
+__global__ __launch_bounds__(256) void k_cfl_final(const double* __restrict__ part, long np, double CFLmax,
+                                                   double Timestep, double* __restrict__ scal)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    double m = 0.0;
+    for (long l = t; l < np; l += blockDim.x) m = m > part[l] ? m : part[l];
+    red[t] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] = red[t] > red[t + s] ? red[t] : red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double cfldt = CFLmax / red[0];
+        scal[S_CFL] = red[0];
+        scal[S_DT] = (cfldt < Timestep) ? cfldt : Timestep;
+    }
+}
+
+// bulk_velo (3381-3398): the signed u of largest |u| on the plane iend,
+// the first hit in (j,k) order winning ties; one block.  Also advances the
+// device copy of tval by (gamma+rho)*dt (pllcart3d.c:166).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, double* __restrict__ scal, int rk,
+                                              double alpha, double* __restrict__ tval)
+{
+    __shared__ double sv[256];
+    __shared__ long si[256];
+    const int t = threadIdx.x;
+    const long np = (long)d.ny * d.nz;
+    double best = 0.0;
+    long bi = -1;
+    for (long l = t; l < np; l += blockDim.x) {
+        const double x = (double)u[IX(d, d.nx + 2, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz))];
+        if (fabs(x) > fabs(best)) { best = x; bi = l; }
+    }
+    sv[t] = best; si[t] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            const double a = sv[t], b = sv[t + s];
+            const long ia = si[t], ib = si[t + s];
+            bool take_b;
+            if (ib < 0) take_b = false;
+            else if (ia < 0) take_b = true;
+            else if (fabs(b) > fabs(a)) take_b = true;
+            else if (fabs(b) < fabs(a)) take_b = false;
+            else take_b = ib < ia;
+            if (take_b) { sv[t] = b; si[t] = ib; }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        scal[S_RK + 4 * rk] = sv[0];
+        if (tval) tval[0] = tval[0] + alpha * scal[S_DT];
+    }
+}
+
+// ---------------------------------------------------------------- DCT GEMM
+
+// FP64/FP32 LDS-tiled GEMM for the dense DCT contractions (fftw_3d.h
+// REDFT10 / REDFT01 as matrices):
+//   TRB = true : C[b] (M x N) = A[b] (M x K) * B^T, B stored N x K
+//   TRB = false: C[b] (M x N) = A[b] (M x K) * B[b], B stored K x N
+// 64x64 tile per block of 256 threads, 4x4 per thread, K chunks of 16.
+template <typename T, bool TRB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
+                                              const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
+{
+    __shared__ T As[16][64 + 1];
+    __shared__ T Bs[16][64 + 1];
+    const int b = blockIdx.z;
+    A += (long)b * sA; B += (long)b * sB; C += (long)b * sC;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    T acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[p][q] = T(0);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            const int mm = e >> 4, kk = e & 15;
+            const int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
+            const int nn = TRB ? (e >> 4) : (e & 63);
+            const int kb = TRB ? (e & 15) : (e >> 6);
+            const int gn = n0 + nn, gkb = k0 + kb;
+            T bv = T(0);
+            if (gn < N && gkb < K) bv = TRB ? B[(long)gn * K + gkb] : B[(long)gkb * N + gn];
+            Bs[kb][nn] = bv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            T av[4], bv[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) av[p] = As[kk][ty + 16 * p];
+#pragma unroll
+            for (int q = 0; q < 4; q++) bv[q] = Bs[kk][tx + 16 * q];
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[p][q] = fma(av[p], bv[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int gm = m0 + ty + 16 * p;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int gn = n0 + tx + 16 * q;
+            if (gn < N) C[(long)gm * N + gn] = acc[p][q];
+        }
+    }
+}
+
+// ------------------------------------------------------------ utilities
+
+// stream-ordered store of one device scalar
+__global__ void k_set_scalar(double* __restrict__ p, int i, double v)
+{
+    if (threadIdx.x == 0) p[i] = v;
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, long n)
+{
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) b[l] = a[l];
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, long n, T v)
+{
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) a[l] = v;
+}
+
+// batched single-line Thomas for the Tds entry points: lines stored
+// line-major [line][row]; mode 0 = tridiag_1rhs (b kept, solution in r),
+// mode 1 = pll_tridiag (b, c overwritten as in pll_tridiag.c:21-47).
+__global__ void k_thomas_lines(int nsys, int n, const double* __restrict__ a, double* __restrict__ b,
+                               double* __restrict__ c, double* __restrict__ r, double* __restrict__ cc, int mode)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nsys) return;
+    const long o = (long)l * n;
+    const double zero = 0.0;
+    if (mode == 0) {
+        double bb = b[o];
+        cc[o] = c[o] / bb;
+        r[o] = r[o] / bb;
+        for (int i = 1; i < n; i++) {
+            bb = b[o + i] - cc[o + i - 1] * a[o + i];
+            cc[o + i] = c[o + i] / bb;
+            r[o + i] = (r[o + i] - r[o + i - 1] * a[o + i]) / bb;
+        }
+        for (int i = n - 2; i >= 0; i--) r[o + i] = r[o + i] - r[o + i + 1] * cc[o + i];
+    } else {
+        b[o] = b[o] - zero * a[o];
+        c[o] = c[o] / b[o];
+        r[o] = (r[o] - zero * a[o]) / b[o];
+        for (int i = 1; i < n; i++) {
+            b[o + i] = (b[o + i] - c[o + i - 1] * a[o + i]);
+            c[o + i] = c[o + i] / b[o + i];
+            r[o + i] = (r[o + i] - r[o + i - 1] * a[o + i]) / b[o + i];
+        }
+        r[o + n - 1] = r[o + n - 1] - c[o + n - 1] * zero;
+        for (int i = n - 2; i >= 0; i--) r[o + i] = r[o + i] - c[o + i] * r[o + i + 1];
+    }
+}
+
+// calculate_weno for a batch of (kk, sign) positions on one D1 array
+__global__ void k_weno_batch(const double* __restrict__ d1, const int* __restrict__ kk, const int* __restrict__ sg,
+                             double* __restrict__ out, int n)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int s = sg[l], b = kk[l];
+    out[l] = weno5(d1[b], d1[b + s], d1[b + 2 * s], d1[b + 3 * s], d1[b + 4 * s]);
+}
+
+// ------------------------------------------------------ instantiations
+
+#define PLLC_INST(T)                                                                                           \
+    template __global__ void k_stale<T>(Dims, const T*, const T*, const T*, const T*, T*);                      \
+    template __global__ void k_rhs_c<T>(Dims, Phys, const T*, const T*, const T*, const T*, const T*, T*, T*,  \
+                                        const T*, const double*, T, T);                                         \
+    template __global__ void k_rhs_mom<T, FU>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
+                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
+    template __global__ void k_rhs_mom<T, FV>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
+                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
+    template __global__ void k_rhs_mom<T, FW>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
+                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
+    template __global__ void k_sweep_y<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_y<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_y<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_y<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_z<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_z<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_z<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_z<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
+    template __global__ void k_sweep_x<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
+                                              const double*, int, T, T);                                        \
+    template __global__ void k_sweep_x<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
+                                              const double*, int, T, T);                                        \
+    template __global__ void k_sweep_x<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
+                                              const double*, int, T, T);                                        \
+    template __global__ void k_sweep_x<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
+                                              const double*, int, T, T);                                        \
+    template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
+    template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
+    template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
+    template __global__ void k_div<T>(Dims, Phys, const T*, const T*, const T*, T*, T*, const double*, T, T);  \
+    template __global__ void k_pack_div<T>(Dims, Phys, const T*, T*, const double*, T, T);                     \
+    template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*);                                    \
+    template __global__ void k_phi_scatter<T>(Dims, const T*, T*);                                             \
+    template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
+    template __global__ void k_flux_deficit<T>(Dims, Phys, T*, double*, int);                                  \
+    template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
+    template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
+    template __global__ void k_bulk<T>(Dims, const T*, double*, int, double, double*);                         \
+    template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long);         \
+    template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long);        \
+    template __global__ void k_copy<T>(const T*, T*, long);                                                    \
+    template __global__ void k_fill<T>(T*, long, T);
+
+PLLC_INST(double)
+
+}  // namespace pllc
+
+// ======================================================= launch layer ====
+
+namespace pllc {
+
+const char* kernel_class_names =
+    "stale;rhs_c;rhs_u;rhs_v;rhs_w;sweep_y;sweep_z;sweep_x;visc;velo_ghost;div;dct;pois_x;phi_scatter;"
+    "phi_ghost;velo_corr;flux;pg;cfl;bulk;copy";
+
+// launch `expr` inside the timing hook of kernel class `name`
+#define PLLC_L(c, name, expr)                                \
+    do {                                                     \
+        if ((c).hook) (c).hook((c).hook_user, name, 0);      \
+        expr;                                                \
+        if ((c).hook) (c).hook((c).hook_user, name, 1);      \
+    } while (0)
+
+static inline dim3 cell_grid(const Rng& R, dim3& blk)
+{
+    blk = dim3(64, 4, 1);
+    return dim3((R.ke - R.ks + 1 + 63) / 64, (R.je - R.js + 1 + 3) / 4, R.ie - R.is + 1);
+}
+static inline dim3 line_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
+static inline int bulk_grid(long n) { long b = (n + 255) / 256; return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b)); }
+
+template <typename T>
+static Visc4<T> muo(Ctx<T>& c) { return {c.f[MUO_CELL], c.f[MUO_XY], c.f[MUO_XZ], c.f[MUO_YZ]}; }
+template <typename T>
+static Visc4<T> mun(Ctx<T>& c) { return {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]}; }
+
+template <typename T>
+void Ops<T>::stale(Ctx<T>& c, const T* qc, const T* qu, const T* qv, const T* qw)
+{
+    PLLC_L(c, "stale", (k_stale<T><<<1, 64, 0, c.st>>>(c.d, qc, qu, qv, qw, c.stale)));
+}
+
+template <typename T, int F>
+static void adi(Ctx<T>& c, T g, T r, int slot, T* qold, T* qnew, T* N, T* Nm1, Visc4<T> mu)
+{
+    const Rng R = field_rng(c.d, F);
+    const long ny = R.je - R.js + 1, nz = R.ke - R.ks + 1, nx = R.ie - R.is + 1;
+    PLLC_L(c, "sweep_y", (k_sweep_y<T, F><<<line_grid(nx * nz), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
+                                                                                  c.scal, g, r)));
+    PLLC_L(c, "sweep_z", (k_sweep_z<T, F><<<line_grid(nx * ny), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
+                                                                                  c.scal, g, r)));
+    PLLC_L(c, "sweep_x", (k_sweep_x<T, F><<<line_grid(ny * nz), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
+                                                                                  qold, qnew, N, Nm1, c.scal, slot,
+                                                                                  g, r)));
+}
+
+template <typename T>
+void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un, const T* vn, const T* wn)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "rhs_c", (k_rhs_c<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1], c.f[C_N],
+                                                           c.f[F_SRC], c.stale + 0, c.scal, T(g), T(r))));
+    Visc4<T> none{nullptr, nullptr, nullptr, nullptr};
+    adi<T, FC>(c, T(g), T(r), slot, c.f[C_OLD], c.f[C_NEW], c.f[C_N], c.f[C_NM1], none);
+}
+
+template <typename T>
+void Ops<T>::viscosity(Ctx<T>& c, const T* cn, T* const mu[4])
+{
+    dim3 blk(64, 4, 1), grd((c.d.nz + 2 + 63) / 64, (c.d.ny + 2 + 3) / 4, c.d.nx + 2);
+    PLLC_L(c, "visc", (k_visc<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2], mu[3])));
+}
+
+template <typename T, int F>
+static void mom_impl(Ctx<T>& c, T g, T r, int slot)
+{
+    const int b = F == FU ? U_OLD : F == FV ? V_OLD : W_OLD;
+    const char* nm = F == FU ? "rhs_u" : F == FV ? "rhs_v" : "rhs_w";
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, F), blk);
+    PLLC_L(c, nm, (k_rhs_mom<T, F><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], muo(c),
+                                                           c.f[C_NEW], c.f[b + 2], c.f[b + 4], c.f[b + 3],
+                                                           c.f[F_SRC], c.stale + 4 * F, c.scal, g, r)));
+    adi<T, F>(c, g, r, slot, c.f[b], c.f[b + 1], c.f[b + 3], c.f[b + 4], mun(c));
+}
+
+template <typename T>
+void Ops<T>::momentum(Ctx<T>& c, int comp, double g, double r, int slot)
+{
+    if (comp == 0) mom_impl<T, FU>(c, T(g), T(r), slot);
+    else if (comp == 1) mom_impl<T, FV>(c, T(g), T(r), slot);
+    else mom_impl<T, FW>(c, T(g), T(r), slot);
+}
+
+template <typename T>
+void Ops<T>::velo_ghost(Ctx<T>& c)
+{
+    const long n = (long)c.d.nx * c.d.ny + (long)c.d.nx * c.d.nz + (long)c.d.ny * c.d.nz;
+    PLLC_L(c, "velo_ghost", (k_velo_ghost<T><<<line_grid(n), 256, 0, c.st>>>(c.d, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW])));
+}
+
+template <typename T>
+void Ops<T>::divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "div", (k_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW],
+                                                       write_div ? c.f[DIV] : nullptr, pack ? c.P : nullptr, c.scal,
+                                                       T(g), T(r))));
+}
+
+template <typename T>
+void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse)
+{
+    const int nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
+    const long R = (long)nx * ny, plane = (long)ny * nz;
+    const dim3 gz((nz + 63) / 64, (unsigned)((R + 63) / 64), 1), gy((nz + 63) / 64, (ny + 63) / 64, nx);
+    if (!inverse) {
+        // z: Q[r][kz] = sum_m in[r][m] Cfz[kz][m]; then y: out[x] = Cfy Q[x]
+        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Cfz, 0, c.Q, 0)));
+        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Cfy, 0, c.Q, plane, out, plane)));
+    } else {
+        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Ciy, 0, in, plane, c.Q, plane)));
+        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, c.Q, 0, c.Ciz, 0, out, 0)));
+    }
+}
+
+template <typename T>
+void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    const long nm = (long)c.d.ny * c.d.nz;
+    if (from_div)
+        PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
+    dct2(c, c.P, c.P, false);
+    PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC)));
+    dct2(c, c.P, c.P, true);
+    PLLC_L(c, "phi_scatter", (k_phi_scatter<T><<<grd, blk, 0, c.st>>>(c.d, c.P, c.f[PHI])));
+}
+
+template <typename T>
+void Ops<T>::phi_ghost(Ctx<T>& c)
+{
+    const long n = (long)c.d.nx * c.d.ny + (long)c.d.nx * c.d.nz + (long)c.d.ny * c.d.nz;
+    PLLC_L(c, "phi_ghost", (k_phi_ghost<T><<<line_grid(n), 256, 0, c.st>>>(c.d, c.f[PHI])));
+}
+
+template <typename T>
+void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "velo_corr", (k_velo_corr<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[PHI], c.f[U_NEW], c.f[V_NEW],
+                                                                   c.f[W_NEW], c.scal, T(g), T(r))));
+    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, slot)));
+}
+
+template <typename T>
+void Ops<T>::pressure_gradient(Ctx<T>& c, double g, double r, int mask)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "pg", (k_pg<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_PG], c.f[V_PG], c.f[W_PG],
+                                                     c.scal, T(g), T(r), mask)));
+}
+
+template <typename T>
+void Ops<T>::cfl(Ctx<T>& c)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "cfl", (k_cfl_partial<T><<<grd, blk, 0, c.st>>>(c.d, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.part)));
+    PLLC_L(c, "cfl", (k_cfl_final<<<1, 256, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
+                                                         c.ph.Timestep, c.scal)));
+}
+
+template <typename T>
+void Ops<T>::bulk(Ctx<T>& c, int slot, double alpha, bool advance_t)
+{
+    PLLC_L(c, "bulk", (k_bulk<T><<<1, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
+                                                       advance_t ? c.tval : nullptr)));
+}
+
+template <typename T>
+void Ops<T>::copy(Ctx<T>& c, const T* a, T* b, long n)
+{
+    PLLC_L(c, "copy", (k_copy<T><<<bulk_grid(n), 256, 0, c.st>>>(a, b, n)));
+}
+
+template <typename T>
+void Ops<T>::fill(Ctx<T>& c, T* a, long n, T v)
+{
+    k_fill<T><<<bulk_grid(n), 256, 0, c.st>>>(a, n, v);
+}
+
+template <typename T>
+void Ops<T>::set_scalar(Ctx<T>& c, int idx, double v)
+{
+    k_set_scalar<<<1, 64, 0, c.st>>>(c.scal, idx, v);
+}
+
+template struct Ops<double>;
+
+void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
+                  hipStream_t st)
+{
+    k_thomas_lines<<<(nsys + 255) / 256, 256, 0, st>>>(nsys, n, a, b, c, r, cc, mode);
+}
+
+void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int n, hipStream_t st)
+{
+    k_weno_batch<<<(n + 255) / 256, 256, 0, st>>>(d1, kk, sg, out, n);
+}
+
+void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,
+             hipStream_t st)
+{
+    k_gemm<double, false><<<dim3((N + 63) / 64, (M + 63) / 64, batch), 256, 0, st>>>(M, N, K, A, sA, B, sB, C, sC);
+}
+
+__global__ void k_scale_div(double* __restrict__ a, long n, double div)
+{
+    for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x)
+        a[l] = a[l] / div;
+}
+
+void scale(double* a, long n, double div, hipStream_t st)
+{
+    k_scale_div<<<bulk_grid(n), 256, 0, st>>>(a, n, div);
+}
+
+}  // namespace pllc

@@ -1,0 +1,522 @@
+// pllc_runtime.hip -- host side of libpllcart3d_amd.so: the simulation
+// handle (fused RK3 step, per-operator calls, timing) and the IC recipes.
+// The reference-named drop-in entry points are in pllc_abi.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pllc_kernels.h"
+#include "pllc_host.h"
+#include "pllcart3d.h"
+
+using namespace pllc;
+
+// ------------------------------------------------------------- errors
+
+static thread_local std::string g_err;
+
+void pllc_set_error(const char* where, const char* what)
+{
+    g_err = std::string(where) + ": " + what;
+}
+
+bool pllc_hip_ok(const char* where, hipError_t e)
+{
+    if (e == hipSuccess) return true;
+    pllc_set_error(where, hipGetErrorString(e));
+    return false;
+}
+
+extern "C" const char* pllc_last_error(void) { return g_err.c_str(); }
+
+// --------------------------------------------------------- host tables
+
+// cos(pi*p/(2n)) with p reduced to the first quadrant
+static double cos_q(long p, int n)
+{
+    const double PIv = 4.0 * atan(1.0);
+    const long N4 = 4L * n, N2 = 2L * n;
+    double sg = 1.0;
+    p %= N4;
+    if (p < 0) p += N4;
+    if (p > N2) p = N4 - p;
+    if (p > n) { p = N2 - p; sg = -1.0; }
+    if (p == n) return 0.0;
+    return sg * cos(PIv * (double)p / (double)N2);
+}
+
+// REDFT10 (inverse = false) / REDFT01 (inverse = true) as n x n matrices:
+// y_k = sum_m C[k][m] x_m (FFTW's definitions, used by fftw_3d.h:18-37)
+void pllc_dct_matrix(int n, bool inverse, double* m)
+{
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) {
+            if (!inverse) m[(size_t)k * n + j] = 2.0 * cos_q((long)k * (2 * j + 1), n);
+            else m[(size_t)k * n + j] = j == 0 ? 1.0 : 2.0 * cos_q((long)j * (2 * k + 1), n);
+        }
+}
+
+// Poisson eigenvalues in y or z (pllcart3d.c:2050-2051), same libm cos
+void pllc_eigenvalues(int n, double h, double* lam)
+{
+    const double PIv = 4.0 * atan(1.0);
+    for (int j = 0; j < n; j++) lam[j] = 2.0 * (cos(PIv * (double)j / (double)n) - 1.0) / (h * h);
+}
+
+// -------------------------------------------------------- IC recipes
+
+namespace {
+struct Geo {
+    int nx, ny, nz, NX, NY, NZ;
+    double dx, dy, dz;
+    std::vector<double> Xc, Yc, Zc;
+};
+
+// cell centres of mesh_generation (pllcart3d.c:442-455), one rank
+Geo make_geo(const pllc_params* p)
+{
+    Geo g;
+    g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
+    g.NX = g.nx + 6; g.NY = g.ny + 2; g.NZ = g.nz + 2;
+    g.dx = (p->X2 - p->X1) / (double)p->nx;
+    g.dy = (p->Y2 - p->Y1) / (double)p->ny;
+    g.dz = (p->Z2 - p->Z1) / (double)p->nz;
+    const double xs = p->X1 + (0 - 3) * g.dx + (g.dx / 2.0);
+    const double ys = p->Y1 - 1 * g.dy + (g.dy / 2.0);
+    const double zs = p->Z1 - 1 * g.dz + (g.dz / 2.0);
+    g.Xc.resize(g.NX); g.Yc.resize(g.NY); g.Zc.resize(g.NZ);
+    for (int i = 0; i < g.NX; i++) g.Xc[i] = xs + i * g.dx;
+    for (int j = 0; j < g.NY; j++) g.Yc[j] = ys + j * g.dy;
+    for (int k = 0; k < g.NZ; k++) g.Zc[k] = zs + k * g.dz;
+    return g;
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014)
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double unit() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }   // [0,1)
+};
+}  // namespace
+
+extern "C" int pllc_ic_fill(const pllc_params* p, int recipe, unsigned long long seed, double amp, double* c)
+{
+    if (!p || !c || p->nx < 1 || p->ny < 1 || p->nz < 1) { pllc_set_error("pllc_ic_fill", "bad arguments"); return -1; }
+    const Geo g = make_geo(p);
+    auto at = [&](int i, int j, int k) -> double& { return c[((size_t)i * g.NY + j) * g.NZ + k]; };
+    if (recipe == PLLC_IC_MULTIMODE) {
+        // front displaced by eta(y,z) = amp * S(y,z) / max|S| over interior
+        // cells, S = sum_{m,n=1..16} cos(m pi y'/Ly + phi_mn) cos(n pi z'/Lz + psi_mn)
+        const double PIv = 4.0 * atan(1.0), Ly = p->Y2 - p->Y1, Lz = p->Z2 - p->Z1;
+        SplitMix64 rng{seed};
+        double ph[16][16], ps[16][16];
+        for (int m = 0; m < 16; m++)
+            for (int n = 0; n < 16; n++) { ph[m][n] = 2.0 * PIv * rng.unit(); ps[m][n] = 2.0 * PIv * rng.unit(); }
+        std::vector<double> S((size_t)g.NY * g.NZ);
+        double smax = 0.0;
+        for (int j = 0; j < g.NY; j++)
+            for (int k = 0; k < g.NZ; k++) {
+                double acc = 0.0;
+                for (int m = 0; m < 16; m++)
+                    for (int n = 0; n < 16; n++)
+                        acc += cos((m + 1) * PIv * (g.Yc[j] - p->Y1) / Ly + ph[m][n]) *
+                               cos((n + 1) * PIv * (g.Zc[k] - p->Z1) / Lz + ps[m][n]);
+                S[(size_t)j * g.NZ + k] = acc;
+                if (j >= 1 && j <= g.ny && k >= 1 && k <= g.nz && fabs(acc) > smax) smax = fabs(acc);
+            }
+        for (int i = 0; i < g.NX; i++)
+            for (int j = 0; j < g.NY; j++)
+                for (int k = 0; k < g.NZ; k++) {
+                    const double eta = smax > 0.0 ? amp * S[(size_t)j * g.NZ + k] / smax : 0.0;
+                    at(i, j, k) = 0.5 + 0.5 * erf((g.Xc[i] - p->Inf_pos - eta) / p->Inf_thick);
+                }
+        return 0;
+    }
+    // initial_condition's profile on every cell (pllcart3d.c:501-508)
+    for (int i = 0; i < g.NX; i++) {
+        const double cval = 0.5 + 0.5 * erf((g.Xc[i] - p->Inf_pos) / p->Inf_thick);
+        for (int j = 0; j < g.NY; j++)
+            for (int k = 0; k < g.NZ; k++) at(i, j, k) = cval;
+    }
+    if (recipe == PLLC_IC_NOISE) {
+        // one uniform draw in [-1,1) per interior cell in i-j-k order; added
+        // with amplitude amp where 0.01 < c < 0.99, then clamped to [0,1];
+        // y then z Neumann ghosts refreshed as concentration does (700-712)
+        SplitMix64 rng{seed};
+        for (int i = 3; i <= g.nx + 2; i++)
+            for (int j = 1; j <= g.ny; j++)
+                for (int k = 1; k <= g.nz; k++) {
+                    const double u = 2.0 * rng.unit() - 1.0;
+                    double& v = at(i, j, k);
+                    if (v > 0.01 && v < 0.99) {
+                        v += amp * u;
+                        v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+                    }
+                }
+        for (int i = 3; i <= g.nx + 2; i++) {
+            for (int k = 1; k <= g.nz; k++) { at(i, 0, k) = at(i, 1, k); at(i, g.ny + 1, k) = at(i, g.ny, k); }
+            for (int j = 0; j <= g.ny + 1; j++) { at(i, j, 0) = at(i, j, 1); at(i, j, g.nz + 1) = at(i, j, g.nz); }
+        }
+    } else if (recipe != PLLC_IC_ERF) {
+        pllc_set_error("pllc_ic_fill", "unknown recipe");
+        return -1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------- the sim
+
+struct pllc_sim {
+    pllc_params p{};
+    Ctx<double> c;
+    std::vector<void*> owned;
+    bool new_stale = false;   // old/new pairs swapped by the fused step
+    double Timestep = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // kernel-class timer
+    const char* tname = nullptr;
+    std::vector<hipEvent_t> tev;
+    size_t tn = 0;
+};
+
+static bool dalloc(pllc_sim* s, void** p, size_t bytes, const char* where)
+{
+    if (!pllc_hip_ok(where, hipMalloc(p, bytes ? bytes : 8))) return false;
+    s->owned.push_back(*p);
+    return pllc_hip_ok(where, hipMemset(*p, 0, bytes ? bytes : 8));
+}
+
+static size_t npts(const pllc_sim* s) { return (size_t)s->c.d.NX * s->c.d.NY * s->c.d.NZ; }
+
+void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
+{
+    d.nx = p->nx; d.ny = p->ny; d.nz = p->nz;
+    d.NX = p->nx + 6; d.NY = p->ny + 2; d.NZ = p->nz + 2;
+    d.sy = d.NZ; d.sx = (long)d.NY * d.NZ;
+    d.dx = (p->X2 - p->X1) / (double)p->nx;
+    d.dy = (p->Y2 - p->Y1) / (double)p->ny;
+    d.dz = (p->Z2 - p->Z1) / (double)p->nz;
+    ph.Re = p->Re; ph.Pe = p->Pe; ph.M = p->Vratio; ph.Vnorm = 0.0;
+    ph.Fx = p->Fx; ph.Fy = p->Fy; ph.CFLmax = p->CFLmax; ph.Timestep = p->Timestep;
+    ph.Y1 = p->Y1; ph.Y2 = p->Y2; ph.Z1 = p->Z1; ph.Z2 = p->Z2;
+}
+
+// allocate the per-size tables and scratch of a context
+bool pllc_ctx_tables(Ctx<double>& c, std::vector<void*>& owned, const char* where)
+{
+    const Dims& d = c.d;
+    const size_t npk = (size_t)d.nx * d.ny * d.nz;
+    auto get = [&](void** p, size_t bytes) {
+        if (!pllc_hip_ok(where, hipMalloc(p, bytes ? bytes : 8))) return false;
+        owned.push_back(*p);
+        return pllc_hip_ok(where, hipMemset(*p, 0, bytes ? bytes : 8));
+    };
+    if (!get((void**)&c.P, npk * 8) || !get((void**)&c.Q, npk * 8) || !get((void**)&c.QC, npk * 8)) return false;
+    if (!get((void**)&c.stale, 16 * 8) || !get((void**)&c.scal, S_SIZE * 8) || !get((void**)&c.tval, 8)) return false;
+    c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
+    if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;
+    if (!get((void**)&c.lam_y, d.ny * 8) || !get((void**)&c.lam_z, d.nz * 8)) return false;
+    if (!get((void**)&c.Cfy, (size_t)d.ny * d.ny * 8) || !get((void**)&c.Ciy, (size_t)d.ny * d.ny * 8)) return false;
+    if (!get((void**)&c.Cfz, (size_t)d.nz * d.nz * 8) || !get((void**)&c.Ciz, (size_t)d.nz * d.nz * 8)) return false;
+    std::vector<double> h((size_t)std::max(d.ny, d.nz) * std::max(d.ny, d.nz));
+    pllc_eigenvalues(d.ny, d.dy, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.lam_y, h.data(), d.ny * 8, hipMemcpyHostToDevice))) return false;
+    pllc_eigenvalues(d.nz, d.dz, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.lam_z, h.data(), d.nz * 8, hipMemcpyHostToDevice))) return false;
+    pllc_dct_matrix(d.ny, false, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.Cfy, h.data(), (size_t)d.ny * d.ny * 8, hipMemcpyHostToDevice))) return false;
+    pllc_dct_matrix(d.ny, true, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.Ciy, h.data(), (size_t)d.ny * d.ny * 8, hipMemcpyHostToDevice))) return false;
+    pllc_dct_matrix(d.nz, false, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.Cfz, h.data(), (size_t)d.nz * d.nz * 8, hipMemcpyHostToDevice))) return false;
+    pllc_dct_matrix(d.nz, true, h.data());
+    if (!pllc_hip_ok(where, hipMemcpy(c.Ciz, h.data(), (size_t)d.nz * d.nz * 8, hipMemcpyHostToDevice))) return false;
+    return true;
+}
+
+extern "C" pllc_sim* pllc_sim_create(const pllc_params* p)
+{
+    if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) { pllc_set_error("pllc_sim_create", "bad dimensions"); return nullptr; }
+    if (p->ny > 1024) { pllc_set_error("pllc_sim_create", "ny > 1024 not supported"); return nullptr; }
+    pllc_sim* s = new pllc_sim();
+    s->p = *p;
+    pllc_fill_dims(p, s->c.d, s->c.ph);
+    const size_t n = npts(s);
+    bool ok = pllc_hip_ok("pllc_sim_create", hipStreamCreateWithFlags(&s->c.st, hipStreamNonBlocking));
+    for (int f = 0; ok && f < NFIELDS; f++) ok = dalloc(s, (void**)&s->c.f[f], n * 8, "pllc_sim_create");
+    ok = ok && pllc_ctx_tables(s->c, s->owned, "pllc_sim_create");
+    ok = ok && pllc_hip_ok("pllc_sim_create", hipEventCreate(&s->e0)) &&
+         pllc_hip_ok("pllc_sim_create", hipEventCreate(&s->e1));
+    if (!ok) { pllc_sim_destroy(s); return nullptr; }
+    s->Timestep = p->Timestep;
+    return s;
+}
+
+extern "C" void pllc_sim_destroy(pllc_sim* s)
+{
+    if (!s) return;
+    if (s->c.st) hipStreamSynchronize(s->c.st);
+    for (void* p : s->owned) hipFree(p);
+    for (hipEvent_t e : s->tev) hipEventDestroy(e);
+    if (s->e0) hipEventDestroy(s->e0);
+    if (s->e1) hipEventDestroy(s->e1);
+    if (s->c.st) hipStreamDestroy(s->c.st);
+    delete s;
+}
+
+extern "C" size_t pllc_sim_dims(const pllc_sim* s, int* dims)
+{
+    if (dims) { dims[0] = s->c.d.NX; dims[1] = s->c.d.NY; dims[2] = s->c.d.NZ; }
+    return npts(s);
+}
+
+extern "C" double pllc_sim_timestep(const pllc_sim* s) { return s->Timestep; }
+
+static int finish(pllc_sim* s, const char* where)
+{
+    if (!pllc_hip_ok(where, hipGetLastError())) return -1;
+    if (!pllc_hip_ok(where, hipStreamSynchronize(s->c.st))) return -1;
+    return 0;
+}
+
+// the (new, old) pairs of update_old_conc/velo/visc (3402-3452)
+static const int kPairs[12][2] = {{C_NEW, C_OLD}, {C_N, C_NM1}, {U_NEW, U_OLD}, {U_N, U_NM1}, {V_NEW, V_OLD},
+                                  {V_N, V_NM1},   {W_NEW, W_OLD}, {W_N, W_NM1}, {MUN_CELL, MUO_CELL},
+                                  {MUN_XY, MUO_XY}, {MUN_XZ, MUO_XZ}, {MUN_YZ, MUO_YZ}};
+
+// update_old_*: copy new -> old over every point (3402-3452)
+static void update_old_copy(pllc_sim* s)
+{
+    for (auto& pr : kPairs) Ops<double>::copy(s->c, s->c.f[pr[0]], s->c.f[pr[1]], (long)npts(s));
+}
+
+// after fused steps the pairs are swapped (old holds the latest state);
+// restore the reference's post-update state new == old before per-op use
+static void normalize(pllc_sim* s)
+{
+    if (!s->new_stale) return;
+    for (auto& pr : kPairs) Ops<double>::copy(s->c, s->c.f[pr[1]], s->c.f[pr[0]], (long)npts(s));
+    s->new_stale = false;
+}
+
+extern "C" int pllc_sim_init(pllc_sim* s, const double* c_full)
+{
+    const char* W = "pllc_sim_init";
+    const Dims& d = s->c.d;
+    const size_t n = npts(s);
+    for (int f = 0; f < NFIELDS; f++)
+        if (!pllc_hip_ok(W, hipMemsetAsync(s->c.f[f], 0, n * 8, s->c.st))) return -1;
+    s->new_stale = false;
+    // initial_condition (493-526): c on every cell, Poiseuille inlet face
+    std::vector<double> h(n);
+    if (c_full) memcpy(h.data(), c_full, n * 8);
+    else if (pllc_ic_fill(&s->p, PLLC_IC_ERF, 0, 0.0, h.data())) return -1;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(s->c.f[C_NEW], h.data(), n * 8, hipMemcpyHostToDevice, s->c.st))) return -1;
+    const Geo g = make_geo(&s->p);
+    std::vector<double> plane((size_t)d.NY * d.NZ, 0.0);
+    auto pl = [&](int j, int k) -> double& { return plane[(size_t)j * d.NZ + k]; };
+    for (int j = 1; j <= d.ny; j++) {
+        for (int k = 1; k <= d.nz; k++) pl(j, k) = 1.5 * (1.0 - 4.0 * g.Yc[j] * g.Yc[j]);
+        pl(j, 0) = pl(j, 1);
+        pl(j, d.nz + 1) = pl(j, d.nz);
+    }
+    for (int k = 0; k <= d.nz + 1; k++) { pl(0, k) = -pl(1, k); pl(d.ny + 1, k) = -pl(d.ny, k); }
+    if (!pllc_hip_ok(W, hipMemcpyAsync(s->c.f[U_NEW] + IX(d, 3, 0, 0), plane.data(), plane.size() * 8,
+                                       hipMemcpyHostToDevice, s->c.st)))
+        return -1;
+    Ops<double>::velo_ghost(s->c);   // update_ghost_points (81); phi is 0
+    update_old_copy(s);              // (82-83)
+    double* mu[4] = {s->c.f[MUO_CELL], s->c.f[MUO_XY], s->c.f[MUO_XZ], s->c.f[MUO_YZ]};
+    Ops<double>::viscosity(s->c, s->c.f[C_OLD], mu);   // (142)
+    // Timestep capped by the diffusion limit (150-154)
+    double del_min = (d.dx < d.dy) ? d.dx : d.dy;
+    del_min = (del_min < d.dz) ? del_min : d.dz;
+    const double diff_dt = 0.1 * s->p.Pe * del_min * del_min;
+    s->Timestep = (diff_dt < s->p.Timestep) ? diff_dt : s->p.Timestep;
+    s->c.ph.Timestep = s->Timestep;
+    Ops<double>::set_scalar(s->c, S_DT, 0.0);
+    Ops<double>::fill(s->c, s->c.tval, 1, 0.0);
+    return finish(s, W);
+}
+
+extern "C" int pllc_sim_upload(pllc_sim* s, int field, const double* host)
+{
+    if (field < 0 || field >= NSTATE) { pllc_set_error("pllc_sim_upload", "bad field"); return -1; }
+    normalize(s);
+    if (!pllc_hip_ok("pllc_sim_upload",
+                     hipMemcpyAsync(s->c.f[field], host, npts(s) * 8, hipMemcpyHostToDevice, s->c.st)))
+        return -1;
+    return finish(s, "pllc_sim_upload");
+}
+
+extern "C" int pllc_sim_download(pllc_sim* s, int field, double* host)
+{
+    if (field < 0 || field >= NSTATE) { pllc_set_error("pllc_sim_download", "bad field"); return -1; }
+    normalize(s);
+    if (!pllc_hip_ok("pllc_sim_download",
+                     hipMemcpyAsync(host, s->c.f[field], npts(s) * 8, hipMemcpyDeviceToHost, s->c.st)))
+        return -1;
+    return finish(s, "pllc_sim_download");
+}
+
+static double read_scalar(pllc_sim* s, int idx)
+{
+    double v = NAN;
+    if (!pllc_hip_ok("read_scalar", hipMemcpyAsync(&v, s->c.scal + idx, 8, hipMemcpyDeviceToHost, s->c.st))) return NAN;
+    if (finish(s, "read_scalar")) return NAN;
+    return v;
+}
+
+extern "C" double pllc_sim_op(pllc_sim* s, int op, double dt, int rk, double vbulk)
+{
+    if (rk < 0 || rk > 2 || op < 0 || op >= PLLC_NOPS) { pllc_set_error("pllc_sim_op", "bad op or rk"); return NAN; }
+    normalize(s);
+    Ctx<double>& c = s->c;
+    const double g = kRK1[rk], r = kRK2[rk];
+    Ops<double>::set_scalar(c, S_DT, dt);
+    Ops<double>::set_scalar(c, S_RK + 4 * rk, vbulk);
+    double* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
+    switch (op) {
+    case PLLC_OP_CONCENTRATION:
+        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<double>::concentration(c, g, r, rk, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW]);
+        break;
+    case PLLC_OP_VISCOSITY: Ops<double>::viscosity(c, c.f[C_NEW], mun); break;
+    case PLLC_OP_XMOM:
+    case PLLC_OP_YMOM:
+    case PLLC_OP_ZMOM:
+        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<double>::momentum(c, op - PLLC_OP_XMOM, g, r, rk);
+        break;
+    case PLLC_OP_VELO_BOUNDARY:
+    case PLLC_OP_VELOGHOST: Ops<double>::velo_ghost(c); break;
+    case PLLC_OP_DIVERGENCE: Ops<double>::divergence(c, g, r, true, false); break;
+    case PLLC_OP_PHISOLVER: Ops<double>::poisson(c, g, r, true); break;
+    case PLLC_OP_PHI_GHOSTS: Ops<double>::phi_ghost(c); break;
+    case PLLC_OP_VELO_CORRECTION: Ops<double>::velo_correction(c, g, r, rk); break;
+    case PLLC_OP_PRESSURE_GRADIENT: Ops<double>::pressure_gradient(c, g, r, 7); break;
+    case PLLC_OP_UPDATE_OLD: update_old_copy(s); break;
+    case PLLC_OP_BULK_VELO:
+        Ops<double>::bulk(c, rk, 0.0, false);
+        return read_scalar(s, S_RK + 4 * rk);
+    case PLLC_OP_FIND_CFL: {
+        Ops<double>::cfl(c);
+        const double maxall = read_scalar(s, S_CFL);
+        return c.ph.CFLmax / maxall;
+    }
+    default: break;
+    }
+    return finish(s, "pllc_sim_op") ? NAN : 0.0;
+}
+
+// One time step of pllcart3d.c:158-196 on device-resident scalars.  The
+// new->old copies of update_old_* become pointer swaps: every location of
+// a *new* field that the next substage reads is rewritten before it is read,
+// or is never written and equal in both buffers; the one read of new before
+// its rewrite (concentration's velocities, 171) takes the old buffer, which
+// the reference's copy makes identical.
+static void fused_step(pllc_sim* s)
+{
+    Ctx<double>& c = s->c;
+    Ops<double>::cfl(c);
+    for (int rk = 0; rk < 3; rk++) {
+        const double g = kRK1[rk], r = kRK2[rk];
+        Ops<double>::bulk(c, rk, g + r, true);
+        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<double>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        double* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
+        Ops<double>::viscosity(c, c.f[C_NEW], mun);
+        Ops<double>::momentum(c, 0, g, r, rk);
+        Ops<double>::momentum(c, 1, g, r, rk);
+        Ops<double>::momentum(c, 2, g, r, rk);
+        Ops<double>::velo_ghost(c);
+        Ops<double>::divergence(c, g, r, false, true);
+        Ops<double>::poisson(c, g, r, false);
+        Ops<double>::phi_ghost(c);
+        Ops<double>::velo_correction(c, g, r, rk);
+        Ops<double>::velo_ghost(c);
+        Ops<double>::pressure_gradient(c, g, r, 7);
+        for (auto& pr : kPairs) std::swap(c.f[pr[0]], c.f[pr[1]]);
+        s->new_stale = true;
+    }
+}
+
+extern "C" int pllc_sim_run(pllc_sim* s, int nsteps, double* tval, double* dt)
+{
+    for (int n = 0; n < nsteps; n++) fused_step(s);
+    if (!pllc_hip_ok("pllc_sim_run", hipGetLastError())) return -1;
+    if (tval) {
+        if (!pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(tval, s->c.tval, 8, hipMemcpyDeviceToHost, s->c.st))) return -1;
+    }
+    if (dt) {
+        if (!pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(dt, s->c.scal + S_DT, 8, hipMemcpyDeviceToHost, s->c.st)))
+            return -1;
+    }
+    return finish(s, "pllc_sim_run");
+}
+
+extern "C" int pllc_sim_synchronize(pllc_sim* s) { return finish(s, "pllc_sim_synchronize"); }
+
+extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* ms)
+{
+    for (int n = 0; n < warmup; n++) fused_step(s);
+    if (finish(s, "pllc_sim_time_steps")) return -1;
+    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e0, s->c.st))) return -1;
+    for (int n = 0; n < nsteps; n++) fused_step(s);
+    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e1, s->c.st))) return -1;
+    if (finish(s, "pllc_sim_time_steps")) return -1;
+    float f = 0.f;
+    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventElapsedTime(&f, s->e0, s->e1))) return -1;
+    if (ms) *ms = f;
+    return 0;
+}
+
+static void timer_hook(void* user, const char* name, int phase)
+{
+    pllc_sim* s = (pllc_sim*)user;
+    if (strcmp(name, s->tname) != 0) return;
+    if (s->tn >= s->tev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        s->tev.push_back(e);
+    }
+    (void)hipEventRecord(s->tev[s->tn++], s->c.st);
+    (void)phase;
+}
+
+extern "C" const char* pllc_kernel_names(void) { return kernel_class_names; }
+
+extern "C" int pllc_sim_time_kernel(pllc_sim* s, const char* name, int nsteps, double* avg_ms, int* launches)
+{
+    if (!name) { pllc_set_error("pllc_sim_time_kernel", "no name"); return -1; }
+    if (finish(s, "pllc_sim_time_kernel")) return -1;
+    s->tname = name;
+    s->tn = 0;
+    s->c.hook = timer_hook;
+    s->c.hook_user = s;
+    for (int n = 0; n < nsteps; n++) fused_step(s);
+    s->c.hook = nullptr;
+    if (finish(s, "pllc_sim_time_kernel")) return -1;
+    double tot = 0.0;
+    int cnt = 0;
+    for (size_t e = 0; e + 1 < s->tn; e += 2) {
+        float f = 0.f;
+        if (!pllc_hip_ok("pllc_sim_time_kernel", hipEventElapsedTime(&f, s->tev[e], s->tev[e + 1]))) return -1;
+        tot += f;
+        cnt++;
+    }
+    if (avg_ms) *avg_ms = cnt ? tot / cnt : 0.0;
+    if (launches) *launches = cnt;
+    return 0;
+}
