@@ -18,6 +18,8 @@ enum FieldId {
     NSTATE,
     F_SRC = NSTATE,   // ADI source / solution (the reference's `source`)
     F_CC,             // Thomas factors of the current sweep
+    F_SRC2, F_CC2,    // second and third set for the batched momentum sweeps
+    F_SRC3, F_CC3,
     NFIELDS
 };
 
@@ -60,6 +62,8 @@ struct Ops {
     static void concentration(Ctx<T>& c, double g, double r, int slot, const T* un, const T* vn, const T* wn);
     static void viscosity(Ctx<T>& c, const T* cn, T* const mu[4]);
     static void momentum(Ctx<T>& c, int comp, double g, double r, int slot);
+    // xmom, ymom and zmom together (needs F_SRC2/3, F_CC2/3)
+    static void momentum3(Ctx<T>& c, double g, double r, int slot);
     static void velo_ghost(Ctx<T>& c);
     static void divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack);
     static void poisson(Ctx<T>& c, double g, double r, bool from_div);
@@ -72,8 +76,8 @@ struct Ops {
     static void copy(Ctx<T>& c, const T* a, T* b, long n);
     static void fill(Ctx<T>& c, T* a, long n, T v);
     static void set_scalar(Ctx<T>& c, int idx, double v);
-    // dense 2D DCT of a packed block: forward REDFT10 (z then y) or inverse
-    // REDFT01 (y then z), `in` -> `out` through the temporary c.Q
+    // dense 2D DCT of a packed block: REDFT10 (forward) or REDFT01
+    // (inverse), z then y, `in` -> `out` through the temporary c.Q
     static void dct2(Ctx<T>& c, const T* in, T* out, bool inverse);
 };
 

@@ -199,156 +199,188 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
 }
 
 // --------------------------------------------------------- line solvers
+//
+// The implicit sweeps of concentration (592-698) and xmom/ymom/zmom
+// (992-1102, 1149-1255, 1301-1403): tridiag_1rhs (3511-3536) on y- and
+// z-lines, pll_tridiag at one rank (pll_tridiag.c:21-47, 96-110) on x-lines,
+// for (I - fac A) with fac = dt*alpha/2 and the field's boundary rows.
+//
+// One thread per line.  A row's upper coefficient c is the same expression
+// as the next row's lower coefficient a (e.g. A_umom 1460-1461: xy[j+1] vs
+// xy[j] of the next row), so each row loads one viscosity value.  Rows are
+// loaded LB at a time, one batch ahead of the recurrence, so memory latency
+// overlaps the serial elimination.  The Thomas factors go to the scratch
+// field CC; the solution overwrites S in place.
 
-// y-lines of concentration / xmom / ymom / zmom: tridiag_1rhs
-// (3511-3536) on (I - fac A_y) with the field's wall rows, in place in S;
-// the Thomas factors cc go to the scratch field CC.  One thread per (i,k)
-// line, consecutive threads on consecutive k (coalesced).
-template <typename T, int F>
-__global__ __launch_bounds__(256) void k_sweep_y(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
-                                                 const double* __restrict__ scal, T g, T r)
-{
-    const Rng R = field_rng(d, F);
-    const int nk = R.ke - R.ks + 1, n = R.je - R.js + 1;
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= (long)nk * (R.ie - R.is + 1)) return;
-    const int k = R.ks + (int)(l % nk), i = R.is + (int)(l / nk);
-    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
-    T pc = T(0.0), pr = T(0.0);
-    for (int x = 0; x < n; x++) {
-        const int j = R.js + x;
-        const long o = IX(d, i, j, k);
-        T A, B, Cc;
-        coefs<T, F, 1>(d, ph, mu, i, j, k, A, B, Cc);
-        const T a = -fac * A;
-        T b = T(1.0) - fac * B;
-        const T c = -fac * Cc;
-        T rhs = S[o];
-        if (x == 0) apply_bc(bc_kind(F, 0), b, a, rhs);
-        if (x == n - 1) apply_bc(bc_kind(F, 1), b, c, rhs);
-        T bb, cc, rr;
-        if (x == 0) { bb = b; cc = c / bb; rr = rhs / bb; }
-        else { bb = b - pc * a; cc = c / bb; rr = (rhs - pr * a) / bb; }
-        CC[o] = cc; S[o] = rr;
-        pc = cc; pr = rr;
-    }
-    for (int x = n - 2; x >= 0; x--) {
-        const long o = IX(d, i, R.js + x, k);
-        const T rr = S[o] - pr * CC[o];
-        S[o] = rr;
-        pr = rr;
-    }
-}
+constexpr int LB = 8;
 
-// z-lines: as k_sweep_y along k.  One thread per (i,j) line.
-template <typename T, int F>
-__global__ __launch_bounds__(256) void k_sweep_z(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
-                                                 const double* __restrict__ scal, T g, T r)
-{
-    const Rng R = field_rng(d, F);
-    const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= (long)nj * (R.ie - R.is + 1)) return;
-    const int j = R.js + (int)(l % nj), i = R.is + (int)(l / nj);
-    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
-    T pc = T(0.0), pr = T(0.0);
-    for (int x = 0; x < n; x++) {
-        const int k = R.ks + x;
-        const long o = IX(d, i, j, k);
-        T A, B, Cc;
-        coefs<T, F, 2>(d, ph, mu, i, j, k, A, B, Cc);
-        const T a = -fac * A;
-        T b = T(1.0) - fac * B;
-        const T c = -fac * Cc;
-        T rhs = S[o];
-        if (x == 0) apply_bc(bc_kind(F, 2), b, a, rhs);
-        if (x == n - 1) apply_bc(bc_kind(F, 3), b, c, rhs);
-        T bb, cc, rr;
-        if (x == 0) { bb = b; cc = c / bb; rr = rhs / bb; }
-        else { bb = b - pc * a; cc = c / bb; rr = (rhs - pr * a) / bb; }
-        CC[o] = cc; S[o] = rr;
-        pc = cc; pr = rr;
-    }
-    for (int x = n - 2; x >= 0; x--) {
-        const long o = IX(d, i, j, R.ks + x);
-        const T rr = S[o] - pr * CC[o];
-        S[o] = rr;
-        pr = rr;
-    }
-}
+// one field's lines in one direction
+template <typename T>
+struct LineSet {
+    T* S;                 // source in, solution out
+    T* CC;                // Thomas factors
+    const T* mu;          // viscosity of the lower coefficient of row 0 + x*stride, or nullptr
+    long mu_off;          // offset of that value from the row's own point
+    T del, h;             // coefficient = del*mu/h (A_*mom 1453-1532)
+    T ca, cb, cc;         // constant coefficients when mu == nullptr (A_conc)
+    int bc0, bc1;         // boundary kinds of the first and last row (x: first only)
+    Rng R;
+    // x-lines only: the update q_new = q_old + delta and the outflow face
+    const T* qold;
+    T* qnew;
+    T* N;
+    const T* Nm1;
+    int ghost_c;          // also fill c_new's y/z ghosts (700-712)
+};
 
-// x-lines: pll_tridiag at one rank (pll_tridiag.c:21-47, 96-110) with the
-// inlet row and the explicit convective outflow face (concentration
-// 657-698, xmom 1056-1102, ymom 1213-1255, zmom 1361-1403), then
-// q_new = q_old + delta.  For c it also fills the y then z Neumann ghosts of
-// c_new (700-712).  One thread per (j,k) line.
-template <typename T, int F>
-__global__ __launch_bounds__(256) void k_sweep_x(Dims d, Phys ph, Visc4<T> mu, T* __restrict__ S, T* __restrict__ CC,
-                                                 const T* __restrict__ qold, T* __restrict__ qnew, T* __restrict__ N,
-                                                 const T* __restrict__ Nm1, const double* __restrict__ scal, int rk,
-                                                 T g, T r)
+template <typename T>
+struct LineSets {
+    LineSet<T> s[3];
+};
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const double* __restrict__ scal, int slot,
+                                               T g, T r)
 {
-    const Rng R = field_rng(d, F);
-    const int nk = R.ke - R.ks + 1, n = R.ie - R.is + 1;
+    const LineSet<T>& L = sets.s[blockIdx.y];
+    const Rng R = L.R;
+    // lines: DIR 0 (x) over (j,k), DIR 1 (y) over (i,k), DIR 2 (z) over (i,j);
+    // the fastest line index runs over k for x and y (coalesced)
+    const int n0 = DIR == 2 ? R.je - R.js + 1 : R.ke - R.ks + 1;
+    const int n1 = DIR == 0 ? R.je - R.js + 1 : R.ie - R.is + 1;
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= (long)nk * (R.je - R.js + 1)) return;
-    const int k = R.ks + (int)(l % nk), j = R.js + (int)(l / nk);
+    if (l >= (long)n0 * n1) return;
+    const int a0 = (int)(l % n0), a1 = (int)(l / n0);
+    int i, j, k, n;
+    long st;
+    if (DIR == 0) { k = R.ks + a0; j = R.js + a1; i = R.is; n = R.ie - R.is + 1; st = d.sx; }
+    else if (DIR == 1) { k = R.ks + a0; i = R.is + a1; j = R.js; n = R.je - R.js + 1; st = d.sy; }
+    else { j = R.js + a0; i = R.is + a1; k = R.ks; n = R.ke - R.ks + 1; st = 1; }
+    const long o0 = IX(d, i, j, k);
+    T* __restrict__ S = L.S + o0;
+    T* __restrict__ CC = L.CC + o0;
+    const bool cst = L.mu == nullptr;
+    const T* __restrict__ mu = cst ? nullptr : L.mu + o0 + L.mu_off;
+    const T del = L.del, h = L.h;
     const T dt = T(scal[S_DT]);
-    const T vbulk = T(scal[S_RK + 4 * rk]);
     const T fac = T(0.5) * dt * (g + r);
-
-    // explicit outflow face iend+1
-    const long ob = IX(d, R.ie + 1, j, k);
-    const T nb = -vbulk * (qold[ob] - qold[ob - d.sx]) / T(d.dx);
-    N[ob] = nb;
-    const T bcor = dt * (g * nb + r * Nm1[ob]);
-    qnew[ob] = qold[ob] + bcor;
-
     const T zero = T(0.0);
+
+    T bcor = T(0.0);
+    if (DIR == 0) {
+        // explicit outflow face iend+1 (677-683 and matching lines)
+        const long ob = o0 + (long)n * st;
+        const T vbulk = T(scal[S_RK + 4 * slot]);
+        const T nb = -vbulk * (L.qold[ob] - L.qold[ob - st]) / T(d.dx);
+        L.N[ob] = nb;
+        bcor = dt * (g * nb + r * L.Nm1[ob]);
+        L.qnew[ob] = L.qold[ob] + bcor;
+    }
+
+    // ---- forward elimination
+    T araw = cst ? L.ca : del * mu[0] / h;
     T pc = T(0.0), pr = T(0.0);
-    for (int x = 0; x < n; x++) {
-        const int i = R.is + x;
-        const long o = IX(d, i, j, k);
-        T A, B, Cc;
-        coefs<T, F, 0>(d, ph, mu, i, j, k, A, B, Cc);
-        const T a = -fac * A;
-        T b = T(1.0) - fac * B;
-        T c = -fac * Cc;
-        T rhs = S[o];
-        if (x == 0) apply_bc(bc_kind(F, 4), b, a, rhs);
-        if (x == n - 1) rhs -= c * bcor;
-        if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
-        else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
-        CC[o] = c; S[o] = rhs;
-        pc = c; pr = rhs;
+    T cs[LB], cm[LB], ns[LB], nm[LB];
+#pragma unroll
+    for (int t = 0; t < LB; t++) {
+        if (t < n) { cs[t] = S[(long)t * st]; if (!cst) cm[t] = mu[(long)(t + 1) * st]; }
     }
-    pr = pr - pc * zero;
-    {
-        const long o = IX(d, R.ie, j, k);
-        qnew[o] = qold[o] + pr;
+    for (int x0 = 0; x0 < n; x0 += LB) {
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+            const int x = x0 + LB + t;
+            if (x < n) { ns[t] = S[(long)x * st]; if (!cst) nm[t] = mu[(long)(x + 1) * st]; }
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+            const int x = x0 + t;
+            if (x < n) {
+                const T craw = cst ? L.cc : del * cm[t] / h;
+                const T braw = cst ? L.cb : -(araw + craw);
+                const T a = -fac * araw;
+                T b = T(1.0) - fac * braw;
+                T c = -fac * craw;
+                T rhs = cs[t];
+                if (x == 0) apply_bc(L.bc0, b, a, rhs);
+                if (DIR == 0) {
+                    if (x == n - 1) rhs -= c * bcor;
+                    if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
+                    else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
+                } else {
+                    if (x == n - 1) apply_bc(L.bc1, b, c, rhs);
+                    T bb;
+                    if (x == 0) { bb = b; c = c / bb; rhs = rhs / bb; }
+                    else { bb = b - pc * a; c = c / bb; rhs = (rhs - pr * a) / bb; }
+                }
+                CC[(long)x * st] = c;
+                S[(long)x * st] = rhs;
+                pc = c; pr = rhs;
+                araw = craw;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) { cs[t] = ns[t]; cm[t] = nm[t]; }
     }
-    for (int x = n - 2; x >= 0; x--) {
-        const long o = IX(d, R.is + x, j, k);
-        const T rr = S[o] - CC[o] * pr;
-        qnew[o] = qold[o] + rr;
-        pr = rr;
+
+    // ---- back substitution (row n-1 is final already)
+    if (DIR == 0) {
+        pr = pr - pc * zero;
+        L.qnew[o0 + (long)(n - 1) * st] = L.qold[o0 + (long)(n - 1) * st] + pr;
     }
-    if (F == FC) {
+    T bs[LB], bc[LB], bq[LB], qs[LB], qc[LB], qq[LB];
+#pragma unroll
+    for (int t = 0; t < LB; t++) {
+        const int x = n - 2 - t;
+        if (x >= 0) {
+            bs[t] = S[(long)x * st]; bc[t] = CC[(long)x * st];
+            if (DIR == 0) bq[t] = L.qold[o0 + (long)x * st];
+        }
+    }
+    for (int x0 = n - 2; x0 >= 0; x0 -= LB) {
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+            const int x = x0 - LB - t;
+            if (x >= 0) {
+                qs[t] = S[(long)x * st]; qc[t] = CC[(long)x * st];
+                if (DIR == 0) qq[t] = L.qold[o0 + (long)x * st];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+            const int x = x0 - t;
+            if (x >= 0) {
+                T rr;
+                if (DIR == 0) {
+                    rr = bs[t] - bc[t] * pr;
+                    L.qnew[o0 + (long)x * st] = bq[t] + rr;
+                } else {
+                    rr = bs[t] - pr * bc[t];
+                    S[(long)x * st] = rr;
+                }
+                pr = rr;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) { bs[t] = qs[t]; bc[t] = qc[t]; bq[t] = qq[t]; }
+    }
+
+    if (DIR == 0 && L.ghost_c) {
         // ghost copies of this line's values: y ghosts (701-704), then z
         // ghosts incl. the y-ghost rows (707-712)
-        for (int i = R.is; i <= R.ie; i++) {
-            const T val = qnew[IX(d, i, j, k)];
-            if (j == R.js) qnew[IX(d, i, j - 1, k)] = val;
-            if (j == R.je) qnew[IX(d, i, j + 1, k)] = val;
+        T* qn = L.qnew;
+        for (int ii = R.is; ii <= R.ie; ii++) {
+            const T val = qn[IX(d, ii, j, k)];
+            if (j == R.js) qn[IX(d, ii, j - 1, k)] = val;
+            if (j == R.je) qn[IX(d, ii, j + 1, k)] = val;
             if (k == R.ks) {
-                qnew[IX(d, i, j, k - 1)] = val;
-                if (j == R.js) qnew[IX(d, i, j - 1, k - 1)] = val;
-                if (j == R.je) qnew[IX(d, i, j + 1, k - 1)] = val;
+                qn[IX(d, ii, j, k - 1)] = val;
+                if (j == R.js) qn[IX(d, ii, j - 1, k - 1)] = val;
+                if (j == R.je) qn[IX(d, ii, j + 1, k - 1)] = val;
             }
             if (k == R.ke) {
-                qnew[IX(d, i, j, k + 1)] = val;
-                if (j == R.js) qnew[IX(d, i, j - 1, k + 1)] = val;
-                if (j == R.je) qnew[IX(d, i, j + 1, k + 1)] = val;
+                qn[IX(d, ii, j, k + 1)] = val;
+                if (j == R.js) qn[IX(d, ii, j - 1, k + 1)] = val;
+                if (j == R.je) qn[IX(d, ii, j + 1, k + 1)] = val;
             }
         }
     }
@@ -746,6 +778,9 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, d
 //   TRB = true : C[b] (M x N) = A[b] (M x K) * B^T, B stored N x K
 //   TRB = false: C[b] (M x N) = A[b] (M x K) * B[b], B stored K x N
 // 64x64 tile per block of 256 threads, 4x4 per thread, K chunks of 16.
+// Every output is accumulated over k in increasing order; for T = double
+// the product and the sum are rounded separately (no FMA), i.e. the direct
+// sums of the FFTW definitions evaluated left to right.
 template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
@@ -784,7 +819,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 #pragma unroll
             for (int p = 0; p < 4; p++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[p][q] = fma(av[p], bv[q], acc[p][q]);
+                for (int q = 0; q < 4; q++) {
+                    if (sizeof(T) == 8) acc[p][q] = acc[p][q] + av[p] * bv[q];
+                    else acc[p][q] = fma(av[p], bv[q], acc[p][q]);
+                }
         }
         __syncthreads();
     }
@@ -876,22 +914,9 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
                                               const T*, const T*, T*, T*, const T*, const double*, T, T);       \
     template __global__ void k_rhs_mom<T, FW>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
                                               const T*, const T*, T*, T*, const T*, const double*, T, T);       \
-    template __global__ void k_sweep_y<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_y<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_y<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_y<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_z<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_z<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_z<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_z<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const double*, T, T);             \
-    template __global__ void k_sweep_x<T, FC>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
-                                              const double*, int, T, T);                                        \
-    template __global__ void k_sweep_x<T, FU>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
-                                              const double*, int, T, T);                                        \
-    template __global__ void k_sweep_x<T, FV>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
-                                              const double*, int, T, T);                                        \
-    template __global__ void k_sweep_x<T, FW>(Dims, Phys, Visc4<T>, T*, T*, const T*, T*, T*, const T*,       \
-                                              const double*, int, T, T);                                        \
+    template __global__ void k_sweep<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                      \
+    template __global__ void k_sweep<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                      \
+    template __global__ void k_sweep<T, 2>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
     template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
@@ -948,18 +973,61 @@ void Ops<T>::stale(Ctx<T>& c, const T* qc, const T* qu, const T* qv, const T* qw
     PLLC_L(c, "stale", (k_stale<T><<<1, 64, 0, c.st>>>(c.d, qc, qu, qv, qw, c.stale)));
 }
 
-template <typename T, int F>
-static void adi(Ctx<T>& c, T g, T r, int slot, T* qold, T* qnew, T* N, T* Nm1, Visc4<T> mu)
+// the LineSet of field F's sweep in direction DIR (coefficient source as in
+// A_conc / A_umom / A_vmom / A_wmom, boundary rows as in bc_kind)
+template <typename T>
+static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T>& mu, const T* qold, T* qnew,
+                           T* N, const T* Nm1)
 {
-    const Rng R = field_rng(c.d, F);
-    const long ny = R.je - R.js + 1, nz = R.ke - R.ks + 1, nx = R.ie - R.is + 1;
-    PLLC_L(c, "sweep_y", (k_sweep_y<T, F><<<line_grid(nx * nz), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
-                                                                                  c.scal, g, r)));
-    PLLC_L(c, "sweep_z", (k_sweep_z<T, F><<<line_grid(nx * ny), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
-                                                                                  c.scal, g, r)));
-    PLLC_L(c, "sweep_x", (k_sweep_x<T, F><<<line_grid(ny * nz), 256, 0, c.st>>>(c.d, c.ph, mu, c.f[F_SRC], c.f[F_CC],
-                                                                                  qold, qnew, N, Nm1, c.scal, slot,
-                                                                                  g, r)));
+    LineSet<T> L{};
+    L.S = S; L.CC = CC; L.R = field_rng(c.d, F);
+    const T h = DIR == 0 ? T(c.d.dx) : DIR == 1 ? T(c.d.dy) : T(c.d.dz);
+    L.h = h;
+    if (F == FC) {
+        const T del = T(1.0) / (T(c.ph.Pe) * h * h);
+        L.mu = nullptr; L.ca = del; L.cc = del;
+        L.cb = (DIR == 2) ? (-del - del) : -(del + del);
+    } else {
+        const bool normal = (F == FU && DIR == 0) || (F == FV && DIR == 1) || (F == FW && DIR == 2);
+        L.del = (normal ? T(2.0) : T(1.0)) / (T(c.ph.Re) * h);
+        const long off[3] = {-c.d.sx, -c.d.sy, -1};
+        if (normal) { L.mu = mu.cell; L.mu_off = off[DIR]; }
+        else if ((F == FU && DIR == 1) || (F == FV && DIR == 0)) L.mu = mu.xy;
+        else if ((F == FU && DIR == 2) || (F == FW && DIR == 0)) L.mu = mu.xz;
+        else L.mu = mu.yz;
+    }
+    if (DIR == 1) { L.bc0 = bc_kind(F, 0); L.bc1 = bc_kind(F, 1); }
+    else if (DIR == 2) { L.bc0 = bc_kind(F, 2); L.bc1 = bc_kind(F, 3); }
+    else { L.bc0 = bc_kind(F, 4); L.bc1 = 0; }
+    L.qold = qold; L.qnew = qnew; L.N = N; L.Nm1 = Nm1;
+    L.ghost_c = (F == FC && DIR == 0) ? 1 : 0;
+    return L;
+}
+
+// y, z then x sweeps of up to three fields at once (one launch per direction)
+template <typename T>
+static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* const* S, T* const* CC,
+                     const Visc4<T>& mu, T* const* qold, T* const* qnew, T* const* N, T* const* Nm1)
+{
+    static const char* nm[3] = {"sweep_x", "sweep_y", "sweep_z"};
+    const int order[3] = {1, 2, 0};
+    for (int t = 0; t < 3; t++) {
+        const int DIR = order[t];
+        LineSets<T> ls{};
+        long lines = 0;
+        for (int q = 0; q < nset; q++) {
+            ls.s[q] = line_set<T>(c, F[q], DIR, S[q], CC[q], mu, qold[q], qnew[q], N[q], Nm1[q]);
+            const Rng& R = ls.s[q].R;
+            const long nl = DIR == 0 ? (long)(R.je - R.js + 1) * (R.ke - R.ks + 1)
+                          : DIR == 1 ? (long)(R.ie - R.is + 1) * (R.ke - R.ks + 1)
+                                     : (long)(R.ie - R.is + 1) * (R.je - R.js + 1);
+            lines = nl > lines ? nl : lines;
+        }
+        const dim3 grd((unsigned)((lines + 255) / 256), nset, 1);
+        if (DIR == 0) PLLC_L(c, nm[0], (k_sweep<T, 0><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+        else if (DIR == 1) PLLC_L(c, nm[1], (k_sweep<T, 1><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+        else PLLC_L(c, nm[2], (k_sweep<T, 2><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+    }
 }
 
 template <typename T>
@@ -969,8 +1037,14 @@ void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un,
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "rhs_c", (k_rhs_c<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1], c.f[C_N],
                                                            c.f[F_SRC], c.stale + 0, c.scal, T(g), T(r))));
-    Visc4<T> none{nullptr, nullptr, nullptr, nullptr};
-    adi<T, FC>(c, T(g), T(r), slot, c.f[C_OLD], c.f[C_NEW], c.f[C_N], c.f[C_NM1], none);
+    const int F[1] = {FC};
+    T* S[1] = {c.f[F_SRC]};
+    T* CC[1] = {c.f[F_CC]};
+    T* qo[1] = {c.f[C_OLD]};
+    T* qn[1] = {c.f[C_NEW]};
+    T* N[1] = {c.f[C_N]};
+    T* Nm[1] = {c.f[C_NM1]};
+    adi_sets<T>(c, T(g), T(r), slot, F, 1, S, CC, Visc4<T>{nullptr, nullptr, nullptr, nullptr}, qo, qn, N, Nm);
 }
 
 template <typename T>
@@ -981,24 +1055,53 @@ void Ops<T>::viscosity(Ctx<T>& c, const T* cn, T* const mu[4])
 }
 
 template <typename T, int F>
-static void mom_impl(Ctx<T>& c, T g, T r, int slot)
+static void rhs_mom(Ctx<T>& c, T g, T r, T* S)
 {
     const int b = F == FU ? U_OLD : F == FV ? V_OLD : W_OLD;
     const char* nm = F == FU ? "rhs_u" : F == FV ? "rhs_v" : "rhs_w";
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, F), blk);
     PLLC_L(c, nm, (k_rhs_mom<T, F><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], muo(c),
-                                                           c.f[C_NEW], c.f[b + 2], c.f[b + 4], c.f[b + 3],
-                                                           c.f[F_SRC], c.stale + 4 * F, c.scal, g, r)));
-    adi<T, F>(c, g, r, slot, c.f[b], c.f[b + 1], c.f[b + 3], c.f[b + 4], mun(c));
+                                                           c.f[C_NEW], c.f[b + 2], c.f[b + 4], c.f[b + 3], S,
+                                                           c.stale + 4 * F, c.scal, g, r)));
+}
+
+// comps: bit 0 u, bit 1 v, bit 2 w.  With several components their sources
+// go to F_SRC, F_SRC2, F_SRC3 and the sweeps of all of them share launches;
+// the components are independent (each reads only old velocities, c_new and
+// the viscosities), so this equals xmom; ymom; zmom (179-181).
+template <typename T>
+static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot)
+{
+    const T g = T(gd), r = T(rd);
+    int F[3], n = 0;
+    T *S[3], *CC[3], *qo[3], *qn[3], *N[3], *Nm[3];
+    const int src[3] = {F_SRC, F_SRC2, F_SRC3}, ccs[3] = {F_CC, F_CC2, F_CC3};
+    for (int q = 0; q < 3; q++) {
+        if (!(comps & (1 << q))) continue;
+        const int f = FU + q, b = q == 0 ? U_OLD : q == 1 ? V_OLD : W_OLD;
+        T* s = c.f[src[n]] ? c.f[src[n]] : c.f[F_SRC];
+        T* cc = c.f[ccs[n]] ? c.f[ccs[n]] : c.f[F_CC];
+        if (f == FU) rhs_mom<T, FU>(c, g, r, s);
+        else if (f == FV) rhs_mom<T, FV>(c, g, r, s);
+        else rhs_mom<T, FW>(c, g, r, s);
+        F[n] = f; S[n] = s; CC[n] = cc;
+        qo[n] = c.f[b]; qn[n] = c.f[b + 1]; N[n] = c.f[b + 3]; Nm[n] = c.f[b + 4];
+        n++;
+    }
+    adi_sets<T>(c, g, r, slot, F, n, S, CC, mun(c), qo, qn, N, Nm);
 }
 
 template <typename T>
 void Ops<T>::momentum(Ctx<T>& c, int comp, double g, double r, int slot)
 {
-    if (comp == 0) mom_impl<T, FU>(c, T(g), T(r), slot);
-    else if (comp == 1) mom_impl<T, FV>(c, T(g), T(r), slot);
-    else mom_impl<T, FW>(c, T(g), T(r), slot);
+    momentum_set<T>(c, 1 << comp, g, r, slot);
+}
+
+template <typename T>
+void Ops<T>::momentum3(Ctx<T>& c, double g, double r, int slot)
+{
+    momentum_set<T>(c, 7, g, r, slot);
 }
 
 template <typename T>
@@ -1029,8 +1132,9 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse)
         PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Cfz, 0, c.Q, 0)));
         PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Cfy, 0, c.Q, plane, out, plane)));
     } else {
-        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Ciy, 0, in, plane, c.Q, plane)));
-        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, c.Q, 0, c.Ciz, 0, out, 0)));
+        // z then y, as the forward transform
+        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Ciz, 0, c.Q, 0)));
+        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Ciy, 0, c.Q, plane, out, plane)));
     }
 }
 

@@ -437,9 +437,7 @@ static void fused_step(pllc_sim* s)
         Ops<double>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         double* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
         Ops<double>::viscosity(c, c.f[C_NEW], mun);
-        Ops<double>::momentum(c, 0, g, r, rk);
-        Ops<double>::momentum(c, 1, g, r, rk);
-        Ops<double>::momentum(c, 2, g, r, rk);
+        Ops<double>::momentum3(c, g, r, rk);
         Ops<double>::velo_ghost(c);
         Ops<double>::divergence(c, g, r, false, true);
         Ops<double>::poisson(c, g, r, false);
