@@ -104,8 +104,8 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
         pllc_set_error(where, "ghost widths other than (3,1,1) are not supported");
         return nullptr;
     }
-    if (g->ny_cell > 1024) {
-        pllc_set_error(where, "ny > 1024 is not supported");
+    if (g->ny_cell > 1024 || g->nz_cell > 1024) {
+        pllc_set_error(where, "ny or nz > 1024 is not supported");
         return nullptr;
     }
     if (d.nx != g->nx_cell || d.ny != g->ny_cell || d.nz != g->nz_cell) {

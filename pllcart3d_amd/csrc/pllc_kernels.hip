@@ -5,6 +5,8 @@
 // the reference arithmetic operation by operation.  Launch wrappers live in
 // pllc_runtime.hip; all kernels are templated on the field type T so that
 // the same code serves the FP64 plain path and the FP32 mixed path.
+#include <algorithm>
+
 #include "pllc_dev.h"
 #include "pllc_kernels.h"
 
@@ -386,6 +388,130 @@ __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const d
     }
 }
 
+// z-lines through LDS.  Rows along k are contiguous in memory, so a
+// thread-per-line sweep would make every load of a wavefront touch 64
+// different cache lines.  Here a block of 64 threads owns 64 z-lines (one i,
+// 64 consecutive j) and moves their rows through LDS in chunks of ZC rows:
+// ZC/2 threads read one line's chunk as 16-byte pieces (whole cache lines),
+// the next chunk is fetched into registers while the current one is solved
+// from LDS.  Arithmetic identical to k_sweep<T, 2>.
+constexpr int ZC = 16;
+constexpr int ZL = 64;
+
+template <typename T>
+__global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
+                                                    T r)
+{
+    __shared__ T sS[ZL][ZC + 1];
+    __shared__ T sA[ZL][ZC + 1];   // viscosity in, Thomas factors out
+    const LineSet<T>& L = sets.s[blockIdx.z];
+    const Rng R = L.R;
+    const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
+    const int i = R.is + blockIdx.y;
+    if (i > R.ie) return;
+    const int j0 = R.js + blockIdx.x * ZL;
+    const int nl = min(ZL, nj - (int)(blockIdx.x * ZL));
+    if (nl <= 0) return;
+    const int t = threadIdx.x;
+    const bool mine = t < nl;
+    const long base = IX(d, i, j0, R.ks);   // row 0 of line 0; line q at base + q*sy
+    const bool cst = L.mu == nullptr;
+    const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
+    const T del = L.del, h = L.h;
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+    // cooperative chunk copy: element e of the chunk = (line e / ZC, row e % ZC)
+    constexpr int PER = ZL * ZC / ZL;   // elements per thread per chunk
+    T rs[PER], rm[PER];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = p * ZL + t, q = e / ZC, kk = e % ZC;
+            const long o = base + (long)q * d.sy + k0 + kk;
+            if (q < nl && k0 + kk < n) {
+                rs[p] = L.S[o];
+                if (!cst) rm[p] = mu[o + 1];
+            }
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = p * ZL + t, q = e / ZC, kk = e % ZC;
+            sS[q][kk] = rs[p];
+            if (!cst) sA[q][kk] = rm[p];
+        }
+    };
+    auto gstore = [&](int k0, bool with_cc) {
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = p * ZL + t, q = e / ZC, kk = e % ZC;
+            const long o = base + (long)q * d.sy + k0 + kk;
+            if (q < nl && k0 + kk < n) {
+                L.S[o] = sS[q][kk];
+                if (with_cc) L.CC[o] = sA[q][kk];
+            }
+        }
+    };
+
+    // ---- forward elimination
+    T araw = cst ? L.ca : (mine ? del * mu[base + (long)t * d.sy] / h : T(0.0));
+    T pc = T(0.0), pr = T(0.0);
+    gload(0);
+    for (int k0 = 0; k0 < n; k0 += ZC) {
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (k0 + ZC < n) gload(k0 + ZC);
+        if (mine) {
+            const int m = min(ZC, n - k0);
+            for (int kk = 0; kk < m; kk++) {
+                const int x = k0 + kk;
+                const T craw = cst ? L.cc : del * sA[t][kk] / h;
+                const T braw = cst ? L.cb : -(araw + craw);
+                const T a = -fac * araw;
+                T b = T(1.0) - fac * braw;
+                T c = -fac * craw;
+                T rhs = sS[t][kk];
+                if (x == 0) apply_bc(L.bc0, b, a, rhs);
+                if (x == n - 1) apply_bc(L.bc1, b, c, rhs);
+                T bb;
+                if (x == 0) { bb = b; c = c / bb; rhs = rhs / bb; }
+                else { bb = b - pc * a; c = c / bb; rhs = (rhs - pr * a) / bb; }
+                sA[t][kk] = c;
+                sS[t][kk] = rhs;
+                pc = c; pr = rhs;
+                araw = craw;
+            }
+        }
+        __syncthreads();
+        gstore(k0, true);
+    }
+    // ---- back substitution, chunks from the end; pr = solution of row n-1
+    const int last0 = ((n - 1) / ZC) * ZC;
+    for (int k0 = last0; k0 >= 0; k0 -= ZC) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < PER; p++) {
+            const int e = p * ZL + t, q = e / ZC, kk = e % ZC;
+            const long o = base + (long)q * d.sy + k0 + kk;
+            if (q < nl && k0 + kk < n) { sS[q][kk] = L.S[o]; sA[q][kk] = L.CC[o]; }
+        }
+        __syncthreads();
+        if (mine) {
+            const int m = min(ZC, n - k0);
+            for (int kk = m - 1; kk >= 0; kk--) {
+                const int x = k0 + kk;
+                if (x == n - 1) continue;
+                const T rr = sS[t][kk] - pr * sA[t][kk];
+                sS[t][kk] = rr;
+                pr = rr;
+            }
+        }
+        __syncthreads();
+        gstore(k0, false);
+    }
+}
+
 // ------------------------------------------------------------ viscosity
 
 // calculate_viscosity (843-886): cell centres on a one-cell ring, and
@@ -596,26 +722,38 @@ __global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __r
 }
 
 // Inflow/outflow trapezoids over the interior nodes of the inlet face and
-// the outlet face (2171-2210) in the reference's summation order, FP64,
-// then the uniform deficit added to the outlet face (2216-2223).
-// One block; ny <= 1024 handled with a strided loop over j.
+// the outlet face (2171-2210), FP64, in the reference's summation order.
+// Pass 1: one block per j row sums over k (the row staged in LDS, the sum
+// sequential); pass 2: one block sums the row integrals over j, forms the
+// deficit (2216) and adds it to the outlet face (2219-2223).
+template <typename T>
+__global__ __launch_bounds__(64) void k_flux_rows(Dims d, const T* __restrict__ u, double* __restrict__ part)
+{
+    __shared__ double ri[1056], ro[1056];
+    const int j = 1 + blockIdx.x;
+    const double hz = d.dz;
+    const long bi = IX(d, 3, j, 0), bo = IX(d, d.nx + 3, j, 0);
+    for (int k = threadIdx.x; k <= d.nz; k += blockDim.x) { ri[k] = (double)u[bi + k]; ro[k] = (double)u[bo + k]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double qi = 0.0, qo = 0.0;
+        for (int k = 2; k <= d.nz; k++) {
+            qi += 0.5 * hz * (ri[k] + ri[k - 1]);
+            qo += 0.5 * hz * (ro[k] + ro[k - 1]);
+        }
+        part[j] = qi;
+        part[1056 + j] = qo;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __restrict__ u, double* __restrict__ scal,
-                                                      int rk)
+                                                      const double* __restrict__ part, int rk)
 {
     __shared__ double qk_in[1056], qk_out[1056];
     __shared__ double s_def;
-    const double hy = d.dy, hz = d.dz;
-    const int iin = 3, iout = d.nx + 3;
-    for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) {
-        double qi = 0.0, qo = 0.0;
-        for (int k = 2; k <= d.nz; k++) {
-            qi += 0.5 * hz * ((double)u[IX(d, iin, j, k)] + (double)u[IX(d, iin, j, k - 1)]);
-            qo += 0.5 * hz * ((double)u[IX(d, iout, j, k)] + (double)u[IX(d, iout, j, k - 1)]);
-        }
-        qk_in[j] = qi;
-        qk_out[j] = qo;
-    }
+    const double hy = d.dy;
+    for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) { qk_in[j] = part[j]; qk_out[j] = part[1056 + j]; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double qin = 0.0, qout = 0.0;
@@ -633,7 +771,7 @@ __global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __rest
     const T def = T(s_def);
     const long np = (long)d.ny * d.nz;
     for (long l = threadIdx.x; l < np; l += blockDim.x) {
-        const long o = IX(d, iout, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz));
+        const long o = IX(d, d.nx + 3, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz));
         u[o] = u[o] + def;
     }
 }
@@ -733,40 +871,62 @@ __global__ __launch_bounds__(256) void k_cfl_final(const double* __restrict__ pa
 }
 
 // bulk_velo (3381-3398): the signed u of largest |u| on the plane iend,
-// the first hit in (j,k) order winning ties; one block.  Also advances the
+// the first hit in (j,k) order winning ties.  Pass 1: per-block candidates
+// (value, scan index); pass 2: one block picks the winner, and advances the
 // device copy of tval by (gamma+rho)*dt (pllcart3d.c:166).
+__device__ __forceinline__ bool bulk_better(double b, long ib, double a, long ia)
+{
+    if (ib < 0) return false;
+    if (ia < 0) return true;
+    if (fabs(b) > fabs(a)) return true;
+    if (fabs(b) < fabs(a)) return false;
+    return ib < ia;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, double* __restrict__ scal, int rk,
-                                              double alpha, double* __restrict__ tval)
+__global__ __launch_bounds__(256) void k_bulk_partial(Dims d, const T* __restrict__ u, double* __restrict__ part)
 {
     __shared__ double sv[256];
     __shared__ long si[256];
     const int t = threadIdx.x;
     const long np = (long)d.ny * d.nz;
-    double best = 0.0;
-    long bi = -1;
-    for (long l = t; l < np; l += blockDim.x) {
+    const long l = (long)blockIdx.x * blockDim.x + t;
+    double v = 0.0;
+    long id = -1;
+    if (l < np) {
         const double x = (double)u[IX(d, d.nx + 2, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz))];
-        if (fabs(x) > fabs(best)) { best = x; bi = l; }
+        if (fabs(x) > 0.0) { v = x; id = l; }
     }
-    sv[t] = best; si[t] = bi;
+    sv[t] = v; si[t] = id;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) {
-            const double a = sv[t], b = sv[t + s];
-            const long ia = si[t], ib = si[t + s];
-            bool take_b;
-            if (ib < 0) take_b = false;
-            else if (ia < 0) take_b = true;
-            else if (fabs(b) > fabs(a)) take_b = true;
-            else if (fabs(b) < fabs(a)) take_b = false;
-            else take_b = ib < ia;
-            if (take_b) { sv[t] = b; si[t] = ib; }
-        }
+        if (t < s && bulk_better(sv[t + s], si[t + s], sv[t], si[t])) { sv[t] = sv[t + s]; si[t] = si[t + s]; }
+        __syncthreads();
+    }
+    if (t == 0) { part[2 * blockIdx.x] = sv[0]; part[2 * blockIdx.x + 1] = (double)si[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ part, int nb, double* __restrict__ scal,
+                                                    int rk, double alpha, double* __restrict__ tval)
+{
+    __shared__ double sv[256];
+    __shared__ long si[256];
+    const int t = threadIdx.x;
+    double v = 0.0;
+    long id = -1;
+    for (int b = t; b < nb; b += blockDim.x) {
+        const double x = part[2 * b];
+        const long ix = (long)part[2 * b + 1];
+        if (bulk_better(x, ix, v, id)) { v = x; id = ix; }
+    }
+    sv[t] = v; si[t] = id;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s && bulk_better(sv[t + s], si[t + s], sv[t], si[t])) { sv[t] = sv[t + s]; si[t] = si[t + s]; }
         __syncthreads();
     }
     if (t == 0) {
-        scal[S_RK + 4 * rk] = sv[0];
+        scal[S_RK + 4 * rk] = si[0] < 0 ? 0.0 : sv[0];
         if (tval) tval[0] = tval[0] + alpha * scal[S_DT];
     }
 }
@@ -917,6 +1077,7 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_sweep<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 2>(Dims, LineSets<T>, const double*, int, T, T);                      \
+    template __global__ void k_sweep_z_lds<T>(Dims, LineSets<T>, const double*, T, T);                         \
     template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
     template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
@@ -925,10 +1086,11 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*);                                    \
     template __global__ void k_phi_scatter<T>(Dims, const T*, T*);                                             \
     template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
-    template __global__ void k_flux_deficit<T>(Dims, Phys, T*, double*, int);                                  \
+    template __global__ void k_flux_rows<T>(Dims, const T*, double*);                                          \
+    template __global__ void k_flux_deficit<T>(Dims, Phys, T*, double*, const double*, int);                   \
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
     template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
-    template __global__ void k_bulk<T>(Dims, const T*, double*, int, double, double*);                         \
+    template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
     template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long);         \
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long);        \
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
@@ -1026,7 +1188,17 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
         const dim3 grd((unsigned)((lines + 255) / 256), nset, 1);
         if (DIR == 0) PLLC_L(c, nm[0], (k_sweep<T, 0><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
         else if (DIR == 1) PLLC_L(c, nm[1], (k_sweep<T, 1><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
-        else PLLC_L(c, nm[2], (k_sweep<T, 2><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+        else {
+            // z-lines: one block per 64 lines of an i-plane
+            int nj = 0, ni = 0;
+            for (int q = 0; q < nset; q++) {
+                const Rng& R = ls.s[q].R;
+                nj = std::max(nj, R.je - R.js + 1);
+                ni = std::max(ni, R.ie - R.is + 1);
+            }
+            const dim3 gz((nj + ZL - 1) / ZL, ni, nset);
+            PLLC_L(c, nm[2], (k_sweep_z_lds<T><<<gz, ZL, 0, c.st>>>(c.d, ls, c.scal, g, r)));
+        }
     }
 }
 
@@ -1166,7 +1338,8 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "velo_corr", (k_velo_corr<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[PHI], c.f[U_NEW], c.f[V_NEW],
                                                                    c.f[W_NEW], c.scal, T(g), T(r))));
-    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, slot)));
+    PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
+    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
 }
 
 template <typename T>
@@ -1191,8 +1364,10 @@ void Ops<T>::cfl(Ctx<T>& c)
 template <typename T>
 void Ops<T>::bulk(Ctx<T>& c, int slot, double alpha, bool advance_t)
 {
-    PLLC_L(c, "bulk", (k_bulk<T><<<1, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
-                                                       advance_t ? c.tval : nullptr)));
+    const int nb = (int)(((long)c.d.ny * c.d.nz + 255) / 256);
+    PLLC_L(c, "bulk", (k_bulk_partial<T><<<nb, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.part)));
+    PLLC_L(c, "bulk", (k_bulk_final<<<1, 256, 0, c.st>>>(c.part, nb, c.scal, slot, alpha,
+                                                          advance_t ? c.tval : nullptr)));
 }
 
 template <typename T>

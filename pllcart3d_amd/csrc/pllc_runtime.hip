@@ -3,6 +3,7 @@
 // The reference-named drop-in entry points are in pllc_abi.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -225,6 +226,7 @@ bool pllc_ctx_tables(Ctx<double>& c, std::vector<void*>& owned, const char* wher
     if (!get((void**)&c.P, npk * 8) || !get((void**)&c.Q, npk * 8) || !get((void**)&c.QC, npk * 8)) return false;
     if (!get((void**)&c.stale, 16 * 8) || !get((void**)&c.scal, S_SIZE * 8) || !get((void**)&c.tval, 8)) return false;
     c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
+    c.npart = std::max(c.npart, (long)(2 * 1056 + 2 * (((long)d.ny * d.nz + 255) / 256) + 64));
     if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;
     if (!get((void**)&c.lam_y, d.ny * 8) || !get((void**)&c.lam_z, d.nz * 8)) return false;
     if (!get((void**)&c.Cfy, (size_t)d.ny * d.ny * 8) || !get((void**)&c.Ciy, (size_t)d.ny * d.ny * 8)) return false;
@@ -248,7 +250,7 @@ bool pllc_ctx_tables(Ctx<double>& c, std::vector<void*>& owned, const char* wher
 extern "C" pllc_sim* pllc_sim_create(const pllc_params* p)
 {
     if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) { pllc_set_error("pllc_sim_create", "bad dimensions"); return nullptr; }
-    if (p->ny > 1024) { pllc_set_error("pllc_sim_create", "ny > 1024 not supported"); return nullptr; }
+    if (p->ny > 1024 || p->nz > 1024) { pllc_set_error("pllc_sim_create", "ny or nz > 1024 not supported"); return nullptr; }
     pllc_sim* s = new pllc_sim();
     s->p = *p;
     pllc_fill_dims(p, s->c.d, s->c.ph);
