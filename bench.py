@@ -64,42 +64,66 @@ def cpu_baseline(prm, c0, cells, threads):
                       (n, prm.nx, prm.ny, prm.nz, el)}
 
 
+def kernel_table(sim, ksteps):
+    import pllcart3d_amd as P
+    names = P.load().pllc_kernel_names().decode().split(";")
+    out = {}
+    for nm in names:
+        avg, n = sim.time_kernel(nm, ksteps)
+        if n:
+            out[nm] = {"avg_ms": avg, "launches_per_step": n / float(ksteps), "ms_per_step": avg * n / ksteps}
+    return out
+
+
+def roofline(kernels, cells, bytes_per_value):
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    t = TOUCHES.get(dom) or SWEEP_TOUCHES.get(dom)
+    if t is None:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": dom}
+    ab = t * cells * bytes_per_value
+    ach = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import pllcart3d_amd as P
     from pllcart3d_amd.configs import CONFIGS, config_params
     cfg = CONFIGS[a.config]
     prm = config_params(a.config)
     cells = prm.nx * prm.ny * prm.nz
     c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
-    sim = P.Sim(prm)
-    sim.init(c0)
-    ms = sim.time_steps(a.warmup, a.steps)
+
+    plain = P.Sim(prm, "plain")
+    plain.init(c0)
+    ms = plain.time_steps(a.warmup, a.steps)
     value = cells * a.steps / (ms * 1e-3)
+
+    mixed = P.Sim(prm, "mixed")
+    mixed.init(c0)
+    ms_m = mixed.time_steps(a.warmup, a.steps)
+    # both runs advanced warmup+steps steps from the same state: the plain
+    # FP64 run is the reference trajectory of the mixed one
+    err = {}
+    for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI"):
+        ref = plain.download(f)
+        err[f] = float(np.linalg.norm(mixed.download(f) - ref) / np.linalg.norm(ref))
 
     roof = None
     kernels = {}
+    kernels_m = {}
     if not a.no_kernels:
-        names = P.load().pllc_kernel_names().decode().split(";")
         ksteps = max(2, min(5, a.steps))
-        for nm in names:
-            avg, n = sim.time_kernel(nm, ksteps)
-            if n:
-                kernels[nm] = {"avg_ms": avg, "launches_per_step": n / float(ksteps), "ms_per_step": avg * n / ksteps}
-        dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-        t = TOUCHES.get(dom) or SWEEP_TOUCHES.get(dom)
-        if t is not None:
-            ab = t * cells * 8.0
-            ach = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
-            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                    "algorithmic_bytes_per_launch": ab, "avg_launch_ms": kernels[dom]["avg_ms"]}
-        else:
-            roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                    "traffic": None, "kernel": dom}
+        kernels = kernel_table(plain, ksteps)
+        kernels_m = kernel_table(mixed, ksteps)
+        roof = roofline(kernels, cells, 8.0)
     step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
     cpu = None
     if not a.no_cpu:
@@ -109,13 +133,17 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "%s: %dx%dx%d box 4x1x1, R=%g Pe=%g, plain FP64 RK3-CN step" %
-                   (a.config, prm.nx, prm.ny, prm.nz, prm.Vratio, prm.Pe), "cells": cells,
-                   "precision": "plain-fp64"},
+        "config": {"workload": "%s: %dx%dx%d box 4x1x1, R=%g Pe=%g, RK3-CN step (plain FP64; mixed in 'mixed')" %
+                   (a.config, prm.nx, prm.ny, prm.nz, prm.Vratio, prm.Pe), "cells": cells},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_hbm_fraction": round(step_ab / (ms / a.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "mixed": {"value": cells * a.steps / (ms_m * 1e-3), "ms_per_step": ms_m / a.steps, "dtype": "f32/f64",
+                  "speedup_vs_plain": ms / ms_m,
+                  "rel_l2_vs_fp64_after_%d_steps" % (a.warmup + a.steps): err,
+                  "roofline": roofline(kernels_m, cells, 4.0) if kernels_m else None},
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
+        "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},
     }
     print(json.dumps(out))
 

@@ -222,7 +222,16 @@ enum { PLLC_IC_ERF = 0, PLLC_IC_NOISE = 1, PLLC_IC_MULTIMODE = 2 };
 
 typedef struct pllc_sim pllc_sim;
 
-pllc_sim *pllc_sim_create(const pllc_params *p);
+/* precision of a simulation's fields and stencil/line-solve arithmetic */
+enum {
+    PLLC_PREC_PLAIN = 0,   /* FP64 throughout, the reference's arithmetic */
+    PLLC_PREC_MIXED = 1    /* FP32 fields, stencils, line solves and DCT; FP64 CFL,
+                              flux and bulk reductions and the (0,0) Poisson pencil */
+};
+
+pllc_sim *pllc_sim_create(const pllc_params *p);   /* = pllc_sim_create2(p, PLLC_PREC_PLAIN) */
+pllc_sim *pllc_sim_create2(const pllc_params *p, int precision);
+int pllc_sim_precision(const pllc_sim *s);
 void pllc_sim_destroy(pllc_sim *s);
 /* dims[0..2] = nx_max, ny_max, nz_max; returns points per field */
 size_t pllc_sim_dims(const pllc_sim *s, int *dims);

@@ -9,7 +9,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (FIELDS, FIELD_ID, IC_ERF, IC_MULTIMODE, IC_NOISE, OP_ID, OPS, Params,
+from ._lib import (FIELDS, FIELD_ID, IC_ERF, IC_MULTIMODE, IC_NOISE, OP_ID, OPS, PRECISION, Params,
                    last_error, load)
 from .configs import CONFIGS, make_params
 
@@ -31,12 +31,18 @@ def ic_fill(params, recipe=IC_ERF, seed=0, amp=0.0):
 
 
 class Sim:
-    """One simulation owned by the library (device state on the GPU)."""
+    """One simulation owned by the library (device state on the GPU).
 
-    def __init__(self, params):
+    precision: "plain" (FP64 throughout) or "mixed" (FP32 fields and
+    stencils, FP64 reductions and (0,0) Poisson pencil).  Fields cross the
+    boundary as float64 either way.
+    """
+
+    def __init__(self, params, precision="plain"):
         self.L = load()
         self.params = params
-        self.h = self.L.pllc_sim_create(C.byref(params))
+        self.precision = precision
+        self.h = self.L.pllc_sim_create2(C.byref(params), PRECISION[precision])
         if not self.h:
             raise RuntimeError("pllc_sim_create: " + last_error())
         d = (C.c_int * 3)()

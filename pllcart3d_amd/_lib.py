@@ -34,6 +34,7 @@ OPS = ["CONCENTRATION", "VISCOSITY", "XMOM", "YMOM", "ZMOM", "VELO_BOUNDARY", "D
 OP_ID = {n: i for i, n in enumerate(OPS)}
 
 IC_ERF, IC_NOISE, IC_MULTIMODE = 0, 1, 2
+PRECISION = {"plain": 0, "mixed": 1}
 
 _lib = None
 
@@ -51,6 +52,9 @@ def load():
     L.pllc_last_error.restype = C.c_char_p
     L.pllc_sim_create.restype = vp
     L.pllc_sim_create.argtypes = [C.POINTER(Params)]
+    L.pllc_sim_create2.restype = vp
+    L.pllc_sim_create2.argtypes = [C.POINTER(Params), C.c_int]
+    L.pllc_sim_precision.argtypes = [vp]
     L.pllc_sim_destroy.argtypes = [vp]
     L.pllc_sim_dims.restype = C.c_size_t
     L.pllc_sim_dims.argtypes = [vp, C.POINTER(C.c_int)]

@@ -15,9 +15,16 @@ LIB = os.path.join(HERE, "libpllcart3d_amd.so")
 DRIVER = os.path.join(HERE, "run3_amd")
 OBJDIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["pllc_kernels.hip", "pllc_runtime.hip", "pllc_abi.hip"]
+# (source, extra flags): the FP64 kernels and the host code never fuse
+# multiply-adds; the FP32 kernels of the mixed path may
+HIP_SOURCES = [
+    ("pllc_kernels_f64.hip", ["-ffp-contract=off"]),
+    ("pllc_kernels_f32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("pllc_runtime.hip", ["-ffp-contract=off"]),
+    ("pllc_abi.hip", ["-ffp-contract=off"]),
+]
 HIPFLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-I" + INC, "-I" + CSRC,
 ]
@@ -36,9 +43,9 @@ def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
-    for src in HIP_SOURCES:
+    for src, extra in HIP_SOURCES:
         obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
-        out = _run([hipcc] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        out = _run([hipcc] + HIPFLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose and out.strip():
             print(out)
         objs.append(obj)

@@ -126,7 +126,7 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
             d.owned.push_back(p);
             d.c.f[f] = (double*)p;
         }
-        if (!pllc_ctx_tables(d.c, d.owned, where)) return nullptr;
+        if (!pllc_ctx_tables<double>(d.c, d.owned, where)) return nullptr;
         d.nx = g->nx_cell; d.ny = g->ny_cell; d.nz = g->nz_cell;
     }
     d.c.d.dx = g->dx; d.c.d.dy = g->dy; d.c.d.dz = g->dz;

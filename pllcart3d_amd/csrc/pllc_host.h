@@ -11,6 +11,7 @@ struct pllc_params;
 void pllc_set_error(const char* where, const char* what);
 bool pllc_hip_ok(const char* where, hipError_t e);
 void pllc_fill_dims(const pllc_params* p, pllc::Dims& d, pllc::Phys& ph);
-bool pllc_ctx_tables(pllc::Ctx<double>& c, std::vector<void*>& owned, const char* where);
+template <typename T>
+bool pllc_ctx_tables(pllc::Ctx<T>& c, std::vector<void*>& owned, const char* where);
 void pllc_dct_matrix(int n, bool inverse, double* m);
 void pllc_eigenvalues(int n, double h, double* lam);

@@ -39,6 +39,7 @@ struct Ctx {
     T* stale = nullptr;  // 16 stale-D2 values
     T* lam_y = nullptr;  // Poisson eigenvalues (pllcart3d.c:2050-2051)
     T* lam_z = nullptr;
+    double* lam00 = nullptr;   // lam_y[0], lam_z[0] in FP64 for the (0,0) pencil
     T* Cfy = nullptr;    // REDFT10 matrices (ny x ny, nz x nz)
     T* Cfz = nullptr;
     T* Ciy = nullptr;    // REDFT01 matrices

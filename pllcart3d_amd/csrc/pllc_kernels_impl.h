@@ -1,10 +1,13 @@
-// pllc_kernels.hip -- the HIP kernels of one pllcart3d RK3-CN substage.
+// pllc_kernels_impl.h -- the HIP kernels of one pllcart3d RK3-CN substage
+// and their launch layer, templated on the field type T.
 //
 // Every kernel restates one reference routine (cited per kernel) with the
-// same operand order; with -ffp-contract=off the FP64 instantiation repeats
-// the reference arithmetic operation by operation.  Launch wrappers live in
-// pllc_runtime.hip; all kernels are templated on the field type T so that
-// the same code serves the FP64 plain path and the FP32 mixed path.
+// same operand order.  This file is compiled twice: pllc_kernels_f64.hip
+// instantiates T = double with -ffp-contract=off (the reference arithmetic
+// operation by operation), pllc_kernels_f32.hip instantiates T = float with
+// contraction and fast division (the mixed path).  PLLC_F64_TU selects the
+// instantiation and the FP64-only helpers of the Tds / fftw entry points.
+#pragma once
 #include <algorithm>
 
 #include "pllc_dev.h"
@@ -214,7 +217,9 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
 // overlaps the serial elimination.  The Thomas factors go to the scratch
 // field CC; the solution overwrites S in place.
 
-constexpr int LB = 8;
+// rows prefetched per batch: x-lines are the fewest (one wave per SIMD),
+// so they get the deepest prefetch
+template <int DIR> struct LinePrefetch { static constexpr int value = DIR == 0 ? 16 : 8; };
 
 // one field's lines in one direction
 template <typename T>
@@ -244,6 +249,7 @@ template <typename T, int DIR>
 __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const double* __restrict__ scal, int slot,
                                                T g, T r)
 {
+    constexpr int LB = LinePrefetch<DIR>::value;
     const LineSet<T>& L = sets.s[blockIdx.y];
     const Rng R = L.R;
     // lines: DIR 0 (x) over (j,k), DIR 1 (y) over (i,k), DIR 2 (z) over (i,j);
@@ -645,40 +651,86 @@ __global__ __launch_bounds__(256) void k_pack_div(Dims d, Phys ph, const T* __re
 // (2037-2091): a = c = 1/dx^2, b = -2/dx^2 + lam_y + lam_z, Neumann at the
 // inlet; at the outlet Neumann except mode (0,0), whose ghost phi is the
 // constant 0.  One thread per mode, coalesced over kz; in place.
+// one mode's x-line in arithmetic type U on fields of type T
+template <typename U, typename T>
+__device__ __forceinline__ void pois_line(const Dims& d, U ly, U lz, int jy, int kz, long l, T* __restrict__ P,
+                                          T* __restrict__ CC)
+{
+    constexpr int PB = 16;   // rows prefetched per batch
+    const long nm = (long)d.ny * d.nz;
+    const int n = d.nx;
+    const U dx2i = U(1.0) / (U(d.dx) * U(d.dx));
+    const U phiconst = U(0.0), zero = U(0.0);
+    const U bmid = U(-2.0) * dx2i + ly + lz;
+    T* __restrict__ p = P + l;
+    T* __restrict__ cq = CC + l;
+    U pc = U(0.0), pr = U(0.0);
+    U cur[PB], nxt[PB];
+#pragma unroll
+    for (int t = 0; t < PB; t++) if (t < n) cur[t] = U(p[(long)t * nm]);
+    for (int x0 = 0; x0 < n; x0 += PB) {
+#pragma unroll
+        for (int t = 0; t < PB; t++) if (x0 + PB + t < n) nxt[t] = U(p[(long)(x0 + PB + t) * nm]);
+#pragma unroll
+        for (int t = 0; t < PB; t++) {
+            const int x = x0 + t;
+            if (x < n) {
+                const U a = dx2i;
+                U b = bmid, c = dx2i, rhs = cur[t];
+                if (x == 0) b += a;
+                if (x == n - 1) {
+                    if (jy == 0 && kz == 0) rhs -= c * phiconst;
+                    else b += c;
+                }
+                if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
+                else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
+                cq[(long)x * nm] = T(c);
+                p[(long)x * nm] = T(rhs);
+                pc = c; pr = rhs;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PB; t++) cur[t] = nxt[t];
+    }
+    pr = pr - pc * zero;
+    p[(long)(n - 1) * nm] = T(pr);
+    U cs[PB], cc[PB], ns[PB], nc[PB];
+#pragma unroll
+    for (int t = 0; t < PB; t++) { const int x = n - 2 - t; if (x >= 0) { cs[t] = U(p[(long)x * nm]); cc[t] = U(cq[(long)x * nm]); } }
+    for (int x0 = n - 2; x0 >= 0; x0 -= PB) {
+#pragma unroll
+        for (int t = 0; t < PB; t++) {
+            const int x = x0 - PB - t;
+            if (x >= 0) { ns[t] = U(p[(long)x * nm]); nc[t] = U(cq[(long)x * nm]); }
+        }
+#pragma unroll
+        for (int t = 0; t < PB; t++) {
+            const int x = x0 - t;
+            if (x >= 0) {
+                const U rr = cs[t] - cc[t] * pr;
+                p[(long)x * nm] = T(rr);
+                pr = rr;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PB; t++) { cs[t] = ns[t]; cc[t] = nc[t]; }
+    }
+}
+
+// The (0,0) pencil (the mean of every x-plane, closed by the ghost phi = 0
+// at the outlet) is solved in FP64 whatever the field type: it is the
+// gauge of the otherwise singular Neumann problem and carries the most
+// ill-conditioned line.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pois_x(Dims d, const T* __restrict__ lam_y, const T* __restrict__ lam_z,
-                                                T* __restrict__ P, T* __restrict__ CC)
+                                                T* __restrict__ P, T* __restrict__ CC, const double* __restrict__ lam00)
 {
     const long nm = (long)d.ny * d.nz;
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nm) return;
-    const int jy = (int)(l / d.nz), kz = (int)(l % d.nz), n = d.nx;
-    const T dx2i = T(1.0) / (T(d.dx) * T(d.dx));
-    const T phiconst = T(0.0), zero = T(0.0);
-    const T bmid = T(-2.0) * dx2i + lam_y[jy] + lam_z[kz];
-    T pc = T(0.0), pr = T(0.0);
-    for (int x = 0; x < n; x++) {
-        const long o = (long)x * nm + l;
-        const T a = dx2i;
-        T b = bmid, c = dx2i, rhs = P[o];
-        if (x == 0) b += a;
-        if (x == n - 1) {
-            if (jy == 0 && kz == 0) rhs -= c * phiconst;
-            else b += c;
-        }
-        if (x == 0) { b = b - zero * a; c = c / b; rhs = (rhs - zero * a) / b; }
-        else { b = (b - pc * a); c = c / b; rhs = (rhs - pr * a) / b; }
-        CC[o] = c; P[o] = rhs;
-        pc = c; pr = rhs;
-    }
-    pr = pr - pc * zero;
-    P[(long)(n - 1) * nm + l] = pr;
-    for (int x = n - 2; x >= 0; x--) {
-        const long o = (long)x * nm + l;
-        const T rr = P[o] - CC[o] * pr;
-        P[o] = rr;
-        pr = rr;
-    }
+    const int jy = (int)(l / d.nz), kz = (int)(l % d.nz);
+    if (l == 0) pois_line<double, T>(d, lam00[0], lam00[1], 0, 0, 0, P, CC);
+    else pois_line<T, T>(d, lam_y[jy], lam_z[kz], jy, kz, l, P, CC);
 }
 
 // idct normalisation and copy back into phi (fftw_3d.h:101,
@@ -850,6 +902,7 @@ __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict
 
 // pass 2: max over the partials; dt = min(CFLmax/max, Timestep)
 // (pllcart3d.c:159-160, 3115)
+template <typename T>
 __global__ __launch_bounds__(256) void k_cfl_final(const double* __restrict__ part, long np, double CFLmax,
                                                    double Timestep, double* __restrict__ scal)
 {
@@ -906,6 +959,7 @@ __global__ __launch_bounds__(256) void k_bulk_partial(Dims d, const T* __restric
     if (t == 0) { part[2 * blockIdx.x] = sv[0]; part[2 * blockIdx.x + 1] = (double)si[0]; }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ part, int nb, double* __restrict__ scal,
                                                     int rk, double alpha, double* __restrict__ tval)
 {
@@ -945,24 +999,28 @@ template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
 {
+    // 64 x 128 output tile per block, 4 x 8 per thread (rows ty+16p,
+    // columns tx+16q), K chunks of 16 staged in LDS
     __shared__ T As[16][64 + 1];
-    __shared__ T Bs[16][64 + 1];
+    __shared__ T Bs[16][128 + 1];
     const int b = blockIdx.z;
     A += (long)b * sA; B += (long)b * sB; C += (long)b * sC;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 128;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    T acc[4][4];
+    T acc[4][8];
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[p][q] = T(0);
+        for (int q = 0; q < 8; q++) acc[p][q] = T(0);
     for (int k0 = 0; k0 < K; k0 += 16) {
         for (int e = threadIdx.x; e < 64 * 16; e += 256) {
             const int mm = e >> 4, kk = e & 15;
             const int gm = m0 + mm, gk = k0 + kk;
             As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
-            const int nn = TRB ? (e >> 4) : (e & 63);
-            const int kb = TRB ? (e & 15) : (e >> 6);
+        }
+        for (int e = threadIdx.x; e < 128 * 16; e += 256) {
+            const int nn = TRB ? (e >> 4) : (e & 127);
+            const int kb = TRB ? (e & 15) : (e >> 7);
             const int gn = n0 + nn, gkb = k0 + kb;
             T bv = T(0);
             if (gn < N && gkb < K) bv = TRB ? B[(long)gn * K + gkb] : B[(long)gkb * N + gn];
@@ -971,15 +1029,15 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
-            T av[4], bv[4];
+            T av[4], bv[8];
 #pragma unroll
             for (int p = 0; p < 4; p++) av[p] = As[kk][ty + 16 * p];
 #pragma unroll
-            for (int q = 0; q < 4; q++) bv[q] = Bs[kk][tx + 16 * q];
+            for (int q = 0; q < 8; q++) bv[q] = Bs[kk][tx + 16 * q];
 #pragma unroll
             for (int p = 0; p < 4; p++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < 8; q++) {
                     if (sizeof(T) == 8) acc[p][q] = acc[p][q] + av[p] * bv[q];
                     else acc[p][q] = fma(av[p], bv[q], acc[p][q]);
                 }
@@ -991,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         const int gm = m0 + ty + 16 * p;
         if (gm >= M) continue;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < 8; q++) {
             const int gn = n0 + tx + 16 * q;
             if (gn < N) C[(long)gm * N + gn] = acc[p][q];
         }
@@ -1001,6 +1059,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 // ------------------------------------------------------------ utilities
 
 // stream-ordered store of one device scalar
+template <typename T>
 __global__ void k_set_scalar(double* __restrict__ p, int i, double v)
 {
     if (threadIdx.x == 0) p[i] = v;
@@ -1018,6 +1077,7 @@ __global__ void k_fill(T* __restrict__ a, long n, T v)
     for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) a[l] = v;
 }
 
+#ifdef PLLC_F64_TU
 // batched single-line Thomas for the Tds entry points: lines stored
 // line-major [line][row]; mode 0 = tridiag_1rhs (b kept, solution in r),
 // mode 1 = pll_tridiag (b, c overwritten as in pll_tridiag.c:21-47).
@@ -1062,6 +1122,8 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     out[l] = weno5(d1[b], d1[b + s], d1[b + 2 * s], d1[b + 3 * s], d1[b + 4 * s]);
 }
 
+#endif  // PLLC_F64_TU
+
 // ------------------------------------------------------ instantiations
 
 #define PLLC_INST(T)                                                                                           \
@@ -1083,7 +1145,7 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
     template __global__ void k_div<T>(Dims, Phys, const T*, const T*, const T*, T*, T*, const double*, T, T);  \
     template __global__ void k_pack_div<T>(Dims, Phys, const T*, T*, const double*, T, T);                     \
-    template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*);                                    \
+    template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*, const double*);                     \
     template __global__ void k_phi_scatter<T>(Dims, const T*, T*);                                             \
     template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
     template __global__ void k_flux_rows<T>(Dims, const T*, double*);                                          \
@@ -1096,7 +1158,11 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
 
+#ifdef PLLC_F64_TU
 PLLC_INST(double)
+#else
+PLLC_INST(float)
+#endif
 
 }  // namespace pllc
 
@@ -1104,9 +1170,11 @@ PLLC_INST(double)
 
 namespace pllc {
 
+#ifdef PLLC_F64_TU
 const char* kernel_class_names =
     "stale;rhs_c;rhs_u;rhs_v;rhs_w;sweep_y;sweep_z;sweep_x;visc;velo_ghost;div;dct;pois_x;phi_scatter;"
     "phi_ghost;velo_corr;flux;pg;cfl;bulk;copy";
+#endif
 
 // launch `expr` inside the timing hook of kernel class `name`
 #define PLLC_L(c, name, expr)                                \
@@ -1298,7 +1366,7 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse)
 {
     const int nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
     const long R = (long)nx * ny, plane = (long)ny * nz;
-    const dim3 gz((nz + 63) / 64, (unsigned)((R + 63) / 64), 1), gy((nz + 63) / 64, (ny + 63) / 64, nx);
+    const dim3 gz((nz + 127) / 128, (unsigned)((R + 63) / 64), 1), gy((nz + 127) / 128, (ny + 63) / 64, nx);
     if (!inverse) {
         // z: Q[r][kz] = sum_m in[r][m] Cfz[kz][m]; then y: out[x] = Cfy Q[x]
         PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Cfz, 0, c.Q, 0)));
@@ -1319,7 +1387,7 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div)
     if (from_div)
         PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
     dct2(c, c.P, c.P, false);
-    PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC)));
+    PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC, c.lam00)));
     dct2(c, c.P, c.P, true);
     PLLC_L(c, "phi_scatter", (k_phi_scatter<T><<<grd, blk, 0, c.st>>>(c.d, c.P, c.f[PHI])));
 }
@@ -1357,7 +1425,7 @@ void Ops<T>::cfl(Ctx<T>& c)
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "cfl", (k_cfl_partial<T><<<grd, blk, 0, c.st>>>(c.d, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.part)));
-    PLLC_L(c, "cfl", (k_cfl_final<<<1, 256, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
+    PLLC_L(c, "cfl", (k_cfl_final<T><<<1, 256, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
                                                          c.ph.Timestep, c.scal)));
 }
 
@@ -1366,7 +1434,7 @@ void Ops<T>::bulk(Ctx<T>& c, int slot, double alpha, bool advance_t)
 {
     const int nb = (int)(((long)c.d.ny * c.d.nz + 255) / 256);
     PLLC_L(c, "bulk", (k_bulk_partial<T><<<nb, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.part)));
-    PLLC_L(c, "bulk", (k_bulk_final<<<1, 256, 0, c.st>>>(c.part, nb, c.scal, slot, alpha,
+    PLLC_L(c, "bulk", (k_bulk_final<T><<<1, 256, 0, c.st>>>(c.part, nb, c.scal, slot, alpha,
                                                           advance_t ? c.tval : nullptr)));
 }
 
@@ -1385,11 +1453,16 @@ void Ops<T>::fill(Ctx<T>& c, T* a, long n, T v)
 template <typename T>
 void Ops<T>::set_scalar(Ctx<T>& c, int idx, double v)
 {
-    k_set_scalar<<<1, 64, 0, c.st>>>(c.scal, idx, v);
+    k_set_scalar<T><<<1, 64, 0, c.st>>>(c.scal, idx, v);
 }
 
+#ifdef PLLC_F64_TU
 template struct Ops<double>;
+#else
+template struct Ops<float>;
+#endif
 
+#ifdef PLLC_F64_TU
 void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
                   hipStream_t st)
 {
@@ -1404,7 +1477,7 @@ void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int
 void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,
              hipStream_t st)
 {
-    k_gemm<double, false><<<dim3((N + 63) / 64, (M + 63) / 64, batch), 256, 0, st>>>(M, N, K, A, sA, B, sB, C, sC);
+    k_gemm<double, false><<<dim3((N + 127) / 128, (M + 63) / 64, batch), 256, 0, st>>>(M, N, K, A, sA, B, sB, C, sC);
 }
 
 __global__ void k_scale_div(double* __restrict__ a, long n, double div)
@@ -1417,5 +1490,7 @@ void scale(double* a, long n, double div, hipStream_t st)
 {
     k_scale_div<<<bulk_grid(n), 256, 0, st>>>(a, n, div);
 }
+
+#endif  // PLLC_F64_TU
 
 }  // namespace pllc

@@ -176,13 +176,66 @@ extern "C" int pllc_ic_fill(const pllc_params* p, int recipe, unsigned long long
     return 0;
 }
 
+// allocate the per-size tables and scratch of a context; tables are
+// computed in FP64 on the host and stored in the field type
+template <typename T>
+bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
+{
+    const Dims& d = c.d;
+    const size_t npk = (size_t)d.nx * d.ny * d.nz;
+    auto get = [&](void** p, size_t bytes) {
+        if (!pllc_hip_ok(where, hipMalloc(p, bytes ? bytes : 8))) return false;
+        owned.push_back(*p);
+        return pllc_hip_ok(where, hipMemset(*p, 0, bytes ? bytes : 8));
+    };
+    const size_t ts = sizeof(T);
+    if (!get((void**)&c.P, npk * ts) || !get((void**)&c.Q, npk * ts) || !get((void**)&c.QC, npk * ts)) return false;
+    if (!get((void**)&c.stale, 16 * ts) || !get((void**)&c.scal, S_SIZE * 8) || !get((void**)&c.tval, 8)) return false;
+    c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
+    c.npart = std::max(c.npart, (long)(2 * 1056 + 2 * (((long)d.ny * d.nz + 255) / 256) + 64));
+    if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;
+    if (!get((void**)&c.lam_y, d.ny * ts) || !get((void**)&c.lam_z, d.nz * ts) || !get((void**)&c.lam00, 2 * 8))
+        return false;
+    if (!get((void**)&c.Cfy, (size_t)d.ny * d.ny * ts) || !get((void**)&c.Ciy, (size_t)d.ny * d.ny * ts)) return false;
+    if (!get((void**)&c.Cfz, (size_t)d.nz * d.nz * ts) || !get((void**)&c.Ciz, (size_t)d.nz * d.nz * ts)) return false;
+    const int mx = std::max(d.ny, d.nz);
+    std::vector<double> h((size_t)mx * mx);
+    std::vector<T> ht((size_t)mx * mx);
+    auto put = [&](T* dst, size_t n) {
+        for (size_t e = 0; e < n; e++) ht[e] = T(h[e]);
+        return pllc_hip_ok(where, hipMemcpy(dst, ht.data(), n * ts, hipMemcpyHostToDevice));
+    };
+    double l00[2];
+    pllc_eigenvalues(d.ny, d.dy, h.data());
+    l00[0] = h[0];
+    if (!put(c.lam_y, d.ny)) return false;
+    pllc_eigenvalues(d.nz, d.dz, h.data());
+    l00[1] = h[0];
+    if (!put(c.lam_z, d.nz)) return false;
+    if (!pllc_hip_ok(where, hipMemcpy(c.lam00, l00, 16, hipMemcpyHostToDevice))) return false;
+    pllc_dct_matrix(d.ny, false, h.data());
+    if (!put(c.Cfy, (size_t)d.ny * d.ny)) return false;
+    pllc_dct_matrix(d.ny, true, h.data());
+    if (!put(c.Ciy, (size_t)d.ny * d.ny)) return false;
+    pllc_dct_matrix(d.nz, false, h.data());
+    if (!put(c.Cfz, (size_t)d.nz * d.nz)) return false;
+    pllc_dct_matrix(d.nz, true, h.data());
+    if (!put(c.Ciz, (size_t)d.nz * d.nz)) return false;
+    return true;
+}
+template bool pllc_ctx_tables<double>(Ctx<double>&, std::vector<void*>&, const char*);
+template bool pllc_ctx_tables<float>(Ctx<float>&, std::vector<void*>&, const char*);
+
 // ------------------------------------------------------------- the sim
 
 struct pllc_sim {
     pllc_params p{};
-    Ctx<double> c;
+    int prec = 0;                 // PLLC_PREC_PLAIN (FP64) or PLLC_PREC_MIXED (FP32 fields)
+    Ctx<double> cd;
+    Ctx<float> cf;
+    hipStream_t st = nullptr;
     std::vector<void*> owned;
-    bool new_stale = false;   // old/new pairs swapped by the fused step
+    bool new_stale = false;       // old/new pairs swapped by the fused step
     double Timestep = 0.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // kernel-class timer
@@ -191,14 +244,11 @@ struct pllc_sim {
     size_t tn = 0;
 };
 
-static bool dalloc(pllc_sim* s, void** p, size_t bytes, const char* where)
-{
-    if (!pllc_hip_ok(where, hipMalloc(p, bytes ? bytes : 8))) return false;
-    s->owned.push_back(*p);
-    return pllc_hip_ok(where, hipMemset(*p, 0, bytes ? bytes : 8));
-}
+template <typename T> static Ctx<T>& ctx(pllc_sim* s);
+template <> Ctx<double>& ctx<double>(pllc_sim* s) { return s->cd; }
+template <> Ctx<float>& ctx<float>(pllc_sim* s) { return s->cf; }
 
-static size_t npts(const pllc_sim* s) { return (size_t)s->c.d.NX * s->c.d.NY * s->c.d.NZ; }
+static size_t npts(const pllc_sim* s) { return (size_t)s->cd.d.NX * s->cd.d.NY * s->cd.d.NZ; }
 
 void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
 {
@@ -213,51 +263,38 @@ void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
     ph.Y1 = p->Y1; ph.Y2 = p->Y2; ph.Z1 = p->Z1; ph.Z2 = p->Z2;
 }
 
-// allocate the per-size tables and scratch of a context
-bool pllc_ctx_tables(Ctx<double>& c, std::vector<void*>& owned, const char* where)
+template <typename T>
+static bool create_t(pllc_sim* s)
 {
-    const Dims& d = c.d;
-    const size_t npk = (size_t)d.nx * d.ny * d.nz;
-    auto get = [&](void** p, size_t bytes) {
-        if (!pllc_hip_ok(where, hipMalloc(p, bytes ? bytes : 8))) return false;
-        owned.push_back(*p);
-        return pllc_hip_ok(where, hipMemset(*p, 0, bytes ? bytes : 8));
-    };
-    if (!get((void**)&c.P, npk * 8) || !get((void**)&c.Q, npk * 8) || !get((void**)&c.QC, npk * 8)) return false;
-    if (!get((void**)&c.stale, 16 * 8) || !get((void**)&c.scal, S_SIZE * 8) || !get((void**)&c.tval, 8)) return false;
-    c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
-    c.npart = std::max(c.npart, (long)(2 * 1056 + 2 * (((long)d.ny * d.nz + 255) / 256) + 64));
-    if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;
-    if (!get((void**)&c.lam_y, d.ny * 8) || !get((void**)&c.lam_z, d.nz * 8)) return false;
-    if (!get((void**)&c.Cfy, (size_t)d.ny * d.ny * 8) || !get((void**)&c.Ciy, (size_t)d.ny * d.ny * 8)) return false;
-    if (!get((void**)&c.Cfz, (size_t)d.nz * d.nz * 8) || !get((void**)&c.Ciz, (size_t)d.nz * d.nz * 8)) return false;
-    std::vector<double> h((size_t)std::max(d.ny, d.nz) * std::max(d.ny, d.nz));
-    pllc_eigenvalues(d.ny, d.dy, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.lam_y, h.data(), d.ny * 8, hipMemcpyHostToDevice))) return false;
-    pllc_eigenvalues(d.nz, d.dz, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.lam_z, h.data(), d.nz * 8, hipMemcpyHostToDevice))) return false;
-    pllc_dct_matrix(d.ny, false, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.Cfy, h.data(), (size_t)d.ny * d.ny * 8, hipMemcpyHostToDevice))) return false;
-    pllc_dct_matrix(d.ny, true, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.Ciy, h.data(), (size_t)d.ny * d.ny * 8, hipMemcpyHostToDevice))) return false;
-    pllc_dct_matrix(d.nz, false, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.Cfz, h.data(), (size_t)d.nz * d.nz * 8, hipMemcpyHostToDevice))) return false;
-    pllc_dct_matrix(d.nz, true, h.data());
-    if (!pllc_hip_ok(where, hipMemcpy(c.Ciz, h.data(), (size_t)d.nz * d.nz * 8, hipMemcpyHostToDevice))) return false;
-    return true;
+    Ctx<T>& c = ctx<T>(s);
+    const char* W = "pllc_sim_create";
+    pllc_fill_dims(&s->p, c.d, c.ph);
+    c.st = s->st;
+    const size_t n = npts(s);
+    for (int f = 0; f < NFIELDS; f++) {
+        void* p = nullptr;
+        if (!pllc_hip_ok(W, hipMalloc(&p, n * sizeof(T)))) return false;
+        s->owned.push_back(p);
+        if (!pllc_hip_ok(W, hipMemset(p, 0, n * sizeof(T)))) return false;
+        c.f[f] = (T*)p;
+    }
+    return pllc_ctx_tables<T>(c, s->owned, W);
 }
 
-extern "C" pllc_sim* pllc_sim_create(const pllc_params* p)
+extern "C" pllc_sim* pllc_sim_create2(const pllc_params* p, int precision)
 {
     if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) { pllc_set_error("pllc_sim_create", "bad dimensions"); return nullptr; }
     if (p->ny > 1024 || p->nz > 1024) { pllc_set_error("pllc_sim_create", "ny or nz > 1024 not supported"); return nullptr; }
+    if (precision != PLLC_PREC_PLAIN && precision != PLLC_PREC_MIXED) {
+        pllc_set_error("pllc_sim_create", "unknown precision");
+        return nullptr;
+    }
     pllc_sim* s = new pllc_sim();
     s->p = *p;
-    pllc_fill_dims(p, s->c.d, s->c.ph);
-    const size_t n = npts(s);
-    bool ok = pllc_hip_ok("pllc_sim_create", hipStreamCreateWithFlags(&s->c.st, hipStreamNonBlocking));
-    for (int f = 0; ok && f < NFIELDS; f++) ok = dalloc(s, (void**)&s->c.f[f], n * 8, "pllc_sim_create");
-    ok = ok && pllc_ctx_tables(s->c, s->owned, "pllc_sim_create");
+    s->prec = precision;
+    pllc_fill_dims(p, s->cd.d, s->cd.ph);   // dims are read from cd in both precisions
+    bool ok = pllc_hip_ok("pllc_sim_create", hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    ok = ok && (precision == PLLC_PREC_PLAIN ? create_t<double>(s) : create_t<float>(s));
     ok = ok && pllc_hip_ok("pllc_sim_create", hipEventCreate(&s->e0)) &&
          pllc_hip_ok("pllc_sim_create", hipEventCreate(&s->e1));
     if (!ok) { pllc_sim_destroy(s); return nullptr; }
@@ -265,30 +302,33 @@ extern "C" pllc_sim* pllc_sim_create(const pllc_params* p)
     return s;
 }
 
+extern "C" pllc_sim* pllc_sim_create(const pllc_params* p) { return pllc_sim_create2(p, PLLC_PREC_PLAIN); }
+
 extern "C" void pllc_sim_destroy(pllc_sim* s)
 {
     if (!s) return;
-    if (s->c.st) hipStreamSynchronize(s->c.st);
-    for (void* p : s->owned) hipFree(p);
-    for (hipEvent_t e : s->tev) hipEventDestroy(e);
-    if (s->e0) hipEventDestroy(s->e0);
-    if (s->e1) hipEventDestroy(s->e1);
-    if (s->c.st) hipStreamDestroy(s->c.st);
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    for (void* p : s->owned) (void)hipFree(p);
+    for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
+    if (s->e0) (void)hipEventDestroy(s->e0);
+    if (s->e1) (void)hipEventDestroy(s->e1);
+    if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
 }
 
 extern "C" size_t pllc_sim_dims(const pllc_sim* s, int* dims)
 {
-    if (dims) { dims[0] = s->c.d.NX; dims[1] = s->c.d.NY; dims[2] = s->c.d.NZ; }
+    if (dims) { dims[0] = s->cd.d.NX; dims[1] = s->cd.d.NY; dims[2] = s->cd.d.NZ; }
     return npts(s);
 }
 
+extern "C" int pllc_sim_precision(const pllc_sim* s) { return s->prec; }
 extern "C" double pllc_sim_timestep(const pllc_sim* s) { return s->Timestep; }
 
 static int finish(pllc_sim* s, const char* where)
 {
     if (!pllc_hip_ok(where, hipGetLastError())) return -1;
-    if (!pllc_hip_ok(where, hipStreamSynchronize(s->c.st))) return -1;
+    if (!pllc_hip_ok(where, hipStreamSynchronize(s->st))) return -1;
     return 0;
 }
 
@@ -298,33 +338,58 @@ static const int kPairs[12][2] = {{C_NEW, C_OLD}, {C_N, C_NM1}, {U_NEW, U_OLD}, 
                                   {MUN_XY, MUO_XY}, {MUN_XZ, MUO_XZ}, {MUN_YZ, MUO_YZ}};
 
 // update_old_*: copy new -> old over every point (3402-3452)
-static void update_old_copy(pllc_sim* s)
+template <typename T>
+static void update_old_copy(pllc_sim* s, Ctx<T>& c)
 {
-    for (auto& pr : kPairs) Ops<double>::copy(s->c, s->c.f[pr[0]], s->c.f[pr[1]], (long)npts(s));
+    for (auto& pr : kPairs) Ops<T>::copy(c, c.f[pr[0]], c.f[pr[1]], (long)npts(s));
 }
 
 // after fused steps the pairs are swapped (old holds the latest state);
 // restore the reference's post-update state new == old before per-op use
-static void normalize(pllc_sim* s)
+template <typename T>
+static void normalize(pllc_sim* s, Ctx<T>& c)
 {
     if (!s->new_stale) return;
-    for (auto& pr : kPairs) Ops<double>::copy(s->c, s->c.f[pr[1]], s->c.f[pr[0]], (long)npts(s));
+    for (auto& pr : kPairs) Ops<T>::copy(c, c.f[pr[1]], c.f[pr[0]], (long)npts(s));
     s->new_stale = false;
 }
 
-extern "C" int pllc_sim_init(pllc_sim* s, const double* c_full)
+template <typename T>
+static bool h2d(pllc_sim* s, T* dst, const double* src, size_t n, const char* W)
+{
+    if (sizeof(T) == 8) return pllc_hip_ok(W, hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice));
+    std::vector<T> t(n);
+    for (size_t e = 0; e < n; e++) t[e] = T(src[e]);
+    (void)s;
+    return pllc_hip_ok(W, hipMemcpy(dst, t.data(), n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+static bool d2h(pllc_sim* s, double* dst, const T* src, size_t n, const char* W)
+{
+    if (!pllc_hip_ok(W, hipStreamSynchronize(s->st))) return false;
+    if (sizeof(T) == 8) return pllc_hip_ok(W, hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
+    std::vector<T> t(n);
+    if (!pllc_hip_ok(W, hipMemcpy(t.data(), src, n * sizeof(T), hipMemcpyDeviceToHost))) return false;
+    for (size_t e = 0; e < n; e++) dst[e] = double(t[e]);
+    return true;
+}
+
+template <typename T>
+static int init_t(pllc_sim* s, Ctx<T>& c, const double* c_full)
 {
     const char* W = "pllc_sim_init";
-    const Dims& d = s->c.d;
+    const Dims& d = c.d;
     const size_t n = npts(s);
     for (int f = 0; f < NFIELDS; f++)
-        if (!pllc_hip_ok(W, hipMemsetAsync(s->c.f[f], 0, n * 8, s->c.st))) return -1;
+        if (!pllc_hip_ok(W, hipMemsetAsync(c.f[f], 0, n * sizeof(T), s->st))) return -1;
+    if (finish(s, W)) return -1;
     s->new_stale = false;
     // initial_condition (493-526): c on every cell, Poiseuille inlet face
     std::vector<double> h(n);
     if (c_full) memcpy(h.data(), c_full, n * 8);
     else if (pllc_ic_fill(&s->p, PLLC_IC_ERF, 0, 0.0, h.data())) return -1;
-    if (!pllc_hip_ok(W, hipMemcpyAsync(s->c.f[C_NEW], h.data(), n * 8, hipMemcpyHostToDevice, s->c.st))) return -1;
+    if (!h2d<T>(s, c.f[C_NEW], h.data(), n, W)) return -1;
     const Geo g = make_geo(&s->p);
     std::vector<double> plane((size_t)d.NY * d.NZ, 0.0);
     auto pl = [&](int j, int k) -> double& { return plane[(size_t)j * d.NZ + k]; };
@@ -334,87 +399,97 @@ extern "C" int pllc_sim_init(pllc_sim* s, const double* c_full)
         pl(j, d.nz + 1) = pl(j, d.nz);
     }
     for (int k = 0; k <= d.nz + 1; k++) { pl(0, k) = -pl(1, k); pl(d.ny + 1, k) = -pl(d.ny, k); }
-    if (!pllc_hip_ok(W, hipMemcpyAsync(s->c.f[U_NEW] + IX(d, 3, 0, 0), plane.data(), plane.size() * 8,
-                                       hipMemcpyHostToDevice, s->c.st)))
-        return -1;
-    Ops<double>::velo_ghost(s->c);   // update_ghost_points (81); phi is 0
-    update_old_copy(s);              // (82-83)
-    double* mu[4] = {s->c.f[MUO_CELL], s->c.f[MUO_XY], s->c.f[MUO_XZ], s->c.f[MUO_YZ]};
-    Ops<double>::viscosity(s->c, s->c.f[C_OLD], mu);   // (142)
+    if (!h2d<T>(s, c.f[U_NEW] + IX(d, 3, 0, 0), plane.data(), plane.size(), W)) return -1;
+    Ops<T>::velo_ghost(c);   // update_ghost_points (81); phi is 0
+    update_old_copy<T>(s, c);  // (82-83)
+    T* mu[4] = {c.f[MUO_CELL], c.f[MUO_XY], c.f[MUO_XZ], c.f[MUO_YZ]};
+    Ops<T>::viscosity(c, c.f[C_OLD], mu);   // (142)
     // Timestep capped by the diffusion limit (150-154)
     double del_min = (d.dx < d.dy) ? d.dx : d.dy;
     del_min = (del_min < d.dz) ? del_min : d.dz;
     const double diff_dt = 0.1 * s->p.Pe * del_min * del_min;
     s->Timestep = (diff_dt < s->p.Timestep) ? diff_dt : s->p.Timestep;
-    s->c.ph.Timestep = s->Timestep;
-    Ops<double>::set_scalar(s->c, S_DT, 0.0);
-    Ops<double>::fill(s->c, s->c.tval, 1, 0.0);
+    c.ph.Timestep = s->Timestep;
+    Ops<T>::set_scalar(c, S_DT, 0.0);
+    (void)hipMemsetAsync(c.tval, 0, 8, s->st);
     return finish(s, W);
+}
+
+extern "C" int pllc_sim_init(pllc_sim* s, const double* c_full)
+{
+    return s->prec ? init_t<float>(s, s->cf, c_full) : init_t<double>(s, s->cd, c_full);
+}
+
+template <typename T>
+static int upload_t(pllc_sim* s, Ctx<T>& c, int field, const double* host)
+{
+    normalize<T>(s, c);
+    if (finish(s, "pllc_sim_upload")) return -1;
+    if (!h2d<T>(s, c.f[field], host, npts(s), "pllc_sim_upload")) return -1;
+    return finish(s, "pllc_sim_upload");
+}
+
+template <typename T>
+static int download_t(pllc_sim* s, Ctx<T>& c, int field, double* host)
+{
+    normalize<T>(s, c);
+    return d2h<T>(s, host, c.f[field], npts(s), "pllc_sim_download") ? 0 : -1;
 }
 
 extern "C" int pllc_sim_upload(pllc_sim* s, int field, const double* host)
 {
     if (field < 0 || field >= NSTATE) { pllc_set_error("pllc_sim_upload", "bad field"); return -1; }
-    normalize(s);
-    if (!pllc_hip_ok("pllc_sim_upload",
-                     hipMemcpyAsync(s->c.f[field], host, npts(s) * 8, hipMemcpyHostToDevice, s->c.st)))
-        return -1;
-    return finish(s, "pllc_sim_upload");
+    return s->prec ? upload_t<float>(s, s->cf, field, host) : upload_t<double>(s, s->cd, field, host);
 }
 
 extern "C" int pllc_sim_download(pllc_sim* s, int field, double* host)
 {
     if (field < 0 || field >= NSTATE) { pllc_set_error("pllc_sim_download", "bad field"); return -1; }
-    normalize(s);
-    if (!pllc_hip_ok("pllc_sim_download",
-                     hipMemcpyAsync(host, s->c.f[field], npts(s) * 8, hipMemcpyDeviceToHost, s->c.st)))
-        return -1;
-    return finish(s, "pllc_sim_download");
+    return s->prec ? download_t<float>(s, s->cf, field, host) : download_t<double>(s, s->cd, field, host);
 }
 
-static double read_scalar(pllc_sim* s, int idx)
+static double read_scalar(pllc_sim* s, const double* dev)
 {
     double v = NAN;
-    if (!pllc_hip_ok("read_scalar", hipMemcpyAsync(&v, s->c.scal + idx, 8, hipMemcpyDeviceToHost, s->c.st))) return NAN;
+    if (!pllc_hip_ok("read_scalar", hipMemcpyAsync(&v, dev, 8, hipMemcpyDeviceToHost, s->st))) return NAN;
     if (finish(s, "read_scalar")) return NAN;
     return v;
 }
 
-extern "C" double pllc_sim_op(pllc_sim* s, int op, double dt, int rk, double vbulk)
+template <typename T>
+static double op_t(pllc_sim* s, Ctx<T>& c, int op, double dt, int rk, double vbulk)
 {
-    if (rk < 0 || rk > 2 || op < 0 || op >= PLLC_NOPS) { pllc_set_error("pllc_sim_op", "bad op or rk"); return NAN; }
-    normalize(s);
-    Ctx<double>& c = s->c;
+    normalize<T>(s, c);
     const double g = kRK1[rk], r = kRK2[rk];
-    Ops<double>::set_scalar(c, S_DT, dt);
-    Ops<double>::set_scalar(c, S_RK + 4 * rk, vbulk);
-    double* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
+    Ops<T>::set_scalar(c, S_DT, dt);
+    Ops<T>::set_scalar(c, S_RK + 4 * rk, vbulk);
+    T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
     switch (op) {
     case PLLC_OP_CONCENTRATION:
-        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
-        Ops<double>::concentration(c, g, r, rk, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW]);
+        Ops<T>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<T>::concentration(c, g, r, rk, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW]);
         break;
-    case PLLC_OP_VISCOSITY: Ops<double>::viscosity(c, c.f[C_NEW], mun); break;
+    case PLLC_OP_VISCOSITY: Ops<T>::viscosity(c, c.f[C_NEW], mun); break;
     case PLLC_OP_XMOM:
     case PLLC_OP_YMOM:
     case PLLC_OP_ZMOM:
-        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
-        Ops<double>::momentum(c, op - PLLC_OP_XMOM, g, r, rk);
+        Ops<T>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<T>::momentum(c, op - PLLC_OP_XMOM, g, r, rk);
         break;
     case PLLC_OP_VELO_BOUNDARY:
-    case PLLC_OP_VELOGHOST: Ops<double>::velo_ghost(c); break;
-    case PLLC_OP_DIVERGENCE: Ops<double>::divergence(c, g, r, true, false); break;
-    case PLLC_OP_PHISOLVER: Ops<double>::poisson(c, g, r, true); break;
-    case PLLC_OP_PHI_GHOSTS: Ops<double>::phi_ghost(c); break;
-    case PLLC_OP_VELO_CORRECTION: Ops<double>::velo_correction(c, g, r, rk); break;
-    case PLLC_OP_PRESSURE_GRADIENT: Ops<double>::pressure_gradient(c, g, r, 7); break;
-    case PLLC_OP_UPDATE_OLD: update_old_copy(s); break;
+    case PLLC_OP_VELOGHOST: Ops<T>::velo_ghost(c); break;
+    case PLLC_OP_DIVERGENCE: Ops<T>::divergence(c, g, r, true, false); break;
+    case PLLC_OP_PHISOLVER: Ops<T>::poisson(c, g, r, true); break;
+    case PLLC_OP_PHI_GHOSTS: Ops<T>::phi_ghost(c); break;
+    case PLLC_OP_VELO_CORRECTION: Ops<T>::velo_correction(c, g, r, rk); break;
+    case PLLC_OP_PRESSURE_GRADIENT: Ops<T>::pressure_gradient(c, g, r, 7); break;
+    case PLLC_OP_UPDATE_OLD: update_old_copy<T>(s, c); break;
     case PLLC_OP_BULK_VELO:
-        Ops<double>::bulk(c, rk, 0.0, false);
-        return read_scalar(s, S_RK + 4 * rk);
+        Ops<T>::bulk(c, rk, 0.0, false);
+        return read_scalar(s, c.scal + S_RK + 4 * rk);
     case PLLC_OP_FIND_CFL: {
-        Ops<double>::cfl(c);
-        const double maxall = read_scalar(s, S_CFL);
+        Ops<T>::cfl(c);
+        const double maxall = read_scalar(s, c.scal + S_CFL);
         return c.ph.CFLmax / maxall;
     }
     default: break;
@@ -422,47 +497,60 @@ extern "C" double pllc_sim_op(pllc_sim* s, int op, double dt, int rk, double vbu
     return finish(s, "pllc_sim_op") ? NAN : 0.0;
 }
 
+extern "C" double pllc_sim_op(pllc_sim* s, int op, double dt, int rk, double vbulk)
+{
+    if (rk < 0 || rk > 2 || op < 0 || op >= PLLC_NOPS) { pllc_set_error("pllc_sim_op", "bad op or rk"); return NAN; }
+    return s->prec ? op_t<float>(s, s->cf, op, dt, rk, vbulk) : op_t<double>(s, s->cd, op, dt, rk, vbulk);
+}
+
 // One time step of pllcart3d.c:158-196 on device-resident scalars.  The
 // new->old copies of update_old_* become pointer swaps: every location of
 // a *new* field that the next substage reads is rewritten before it is read,
 // or is never written and equal in both buffers; the one read of new before
 // its rewrite (concentration's velocities, 171) takes the old buffer, which
-// the reference's copy makes identical.
-static void fused_step(pllc_sim* s)
+// the reference's copy makes identical.  xmom, ymom and zmom share their
+// sweep launches (they are independent of each other).
+template <typename T>
+static void fused_step(pllc_sim* s, Ctx<T>& c)
 {
-    Ctx<double>& c = s->c;
-    Ops<double>::cfl(c);
+    Ops<T>::cfl(c);
     for (int rk = 0; rk < 3; rk++) {
         const double g = kRK1[rk], r = kRK2[rk];
-        Ops<double>::bulk(c, rk, g + r, true);
-        Ops<double>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
-        Ops<double>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
-        double* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
-        Ops<double>::viscosity(c, c.f[C_NEW], mun);
-        Ops<double>::momentum3(c, g, r, rk);
-        Ops<double>::velo_ghost(c);
-        Ops<double>::divergence(c, g, r, false, true);
-        Ops<double>::poisson(c, g, r, false);
-        Ops<double>::phi_ghost(c);
-        Ops<double>::velo_correction(c, g, r, rk);
-        Ops<double>::velo_ghost(c);
-        Ops<double>::pressure_gradient(c, g, r, 7);
+        Ops<T>::bulk(c, rk, g + r, true);
+        Ops<T>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
+        Ops<T>::viscosity(c, c.f[C_NEW], mun);
+        Ops<T>::momentum3(c, g, r, rk);
+        Ops<T>::velo_ghost(c);
+        Ops<T>::divergence(c, g, r, false, true);
+        Ops<T>::poisson(c, g, r, false);
+        Ops<T>::phi_ghost(c);
+        Ops<T>::velo_correction(c, g, r, rk);
+        Ops<T>::velo_ghost(c);
+        Ops<T>::pressure_gradient(c, g, r, 7);
         for (auto& pr : kPairs) std::swap(c.f[pr[0]], c.f[pr[1]]);
         s->new_stale = true;
     }
 }
 
+static void step_any(pllc_sim* s)
+{
+    if (s->prec) fused_step<float>(s, s->cf);
+    else fused_step<double>(s, s->cd);
+}
+
+static Ctx<double>& scal_ctx(pllc_sim* s) { return s->cd; }
+
 extern "C" int pllc_sim_run(pllc_sim* s, int nsteps, double* tval, double* dt)
 {
-    for (int n = 0; n < nsteps; n++) fused_step(s);
+    for (int n = 0; n < nsteps; n++) step_any(s);
     if (!pllc_hip_ok("pllc_sim_run", hipGetLastError())) return -1;
-    if (tval) {
-        if (!pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(tval, s->c.tval, 8, hipMemcpyDeviceToHost, s->c.st))) return -1;
-    }
-    if (dt) {
-        if (!pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(dt, s->c.scal + S_DT, 8, hipMemcpyDeviceToHost, s->c.st)))
-            return -1;
-    }
+    const double* tv = s->prec ? s->cf.tval : s->cd.tval;
+    const double* sc = s->prec ? s->cf.scal : s->cd.scal;
+    if (tval && !pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(tval, tv, 8, hipMemcpyDeviceToHost, s->st))) return -1;
+    if (dt && !pllc_hip_ok("pllc_sim_run", hipMemcpyAsync(dt, sc + S_DT, 8, hipMemcpyDeviceToHost, s->st))) return -1;
+    (void)scal_ctx;
     return finish(s, "pllc_sim_run");
 }
 
@@ -470,11 +558,11 @@ extern "C" int pllc_sim_synchronize(pllc_sim* s) { return finish(s, "pllc_sim_sy
 
 extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* ms)
 {
-    for (int n = 0; n < warmup; n++) fused_step(s);
+    for (int n = 0; n < warmup; n++) step_any(s);
     if (finish(s, "pllc_sim_time_steps")) return -1;
-    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e0, s->c.st))) return -1;
-    for (int n = 0; n < nsteps; n++) fused_step(s);
-    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e1, s->c.st))) return -1;
+    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e0, s->st))) return -1;
+    for (int n = 0; n < nsteps; n++) step_any(s);
+    if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e1, s->st))) return -1;
     if (finish(s, "pllc_sim_time_steps")) return -1;
     float f = 0.f;
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventElapsedTime(&f, s->e0, s->e1))) return -1;
@@ -491,7 +579,7 @@ static void timer_hook(void* user, const char* name, int phase)
         if (hipEventCreate(&e) != hipSuccess) return;
         s->tev.push_back(e);
     }
-    (void)hipEventRecord(s->tev[s->tn++], s->c.st);
+    (void)hipEventRecord(s->tev[s->tn++], s->st);
     (void)phase;
 }
 
@@ -503,10 +591,11 @@ extern "C" int pllc_sim_time_kernel(pllc_sim* s, const char* name, int nsteps, d
     if (finish(s, "pllc_sim_time_kernel")) return -1;
     s->tname = name;
     s->tn = 0;
-    s->c.hook = timer_hook;
-    s->c.hook_user = s;
-    for (int n = 0; n < nsteps; n++) fused_step(s);
-    s->c.hook = nullptr;
+    s->cd.hook = s->cf.hook = timer_hook;
+    s->cd.hook_user = s->cf.hook_user = s;
+    for (int n = 0; n < nsteps; n++) step_any(s);
+    s->cd.hook = nullptr;
+    s->cf.hook = nullptr;
     if (finish(s, "pllc_sim_time_kernel")) return -1;
     double tot = 0.0;
     int cnt = 0;
