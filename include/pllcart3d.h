@@ -134,7 +134,9 @@ void mesh_generation(Grid *);                                          /* decl:5
 void initial_condition(Grid *, Conc *, Velo *);                        /* decl:6  pllcart3d.c:493 */
 void concentration(double, double, double, double, Grid *, double ***, double ***, double ***,
                    Conc *);                                            /* decl:7  pllcart3d.c:555 */
+double slope_parameter(Grid *grid, double ***cn);                      /* decl:8  pllcart3d.c:722 */
 void calculate_viscosity(Grid *, double ***, Visc *);                  /* decl:9  pllcart3d.c:843 */
+void calculate_viscosity_nmvp(Grid *grid, double ***cn, Visc *mu, double cm); /* decl:10 pllcart3d.c:889 */
 void xmom(double, double, double, double, Grid *, Visc *, Visc *, double ***, double ***, double ***,
           Velo *);                                                     /* decl:11 pllcart3d.c:952 */
 void ymom(double, double, double, double, Grid *, Visc *, Visc *, double ***, double ***, double ***,
@@ -162,6 +164,18 @@ void update_old_velo(Velo *, Velo *, Velo *);                          /* decl:3
 void update_old_visc(Visc *, Visc *);                                  /* decl:39 pllcart3d.c:3436 */
 double calculate_weno(double *, int, int);                             /* decl:40 pllcart3d.c:3455 */
 int tridiag_1rhs(int Nmax, double *a, double *b, double *c, double *rhs); /* decl:42 pllcart3d.c:3511 */
+
+/* output, restart and diagnostics: the reference's file formats; fields
+   are read from (or, for intermediate, written to) the device mirrors */
+void write_to_file(int, Grid *, double ***, double ***, double ***, double ***); /* decl:27 pllcart3d.c:3047 */
+void intermediate(int *, int *, double *, double *, Grid *, Conc *, Velo *, Velo *,
+                  Velo *);                                             /* decl:29 pllcart3d.c:3181 */
+void print_to_file(double *y, Grid *grid, int fno);                    /* decl:33 pllcart3d.c:3347 */
+void midfield(int, int, double, double, double, double, Grid *, Conc *, Velo *, Velo *,
+              Velo *);                                                 /* decl:35 pllcart3d.c:3121 */
+void find_max(Grid *, Velo *, Velo *, Velo *, Conc *);                 /* decl:41 pllcart3d.c:3539 */
+void find_wmax(double tval, Grid *grid, Velo *w);                      /* decl:44 pllcart3d.c:3606 */
+void conc_average(Grid *grid, double ***cn, double tval);              /* decl:45 pllcart3d.c:3635 */
 
 /* pll_tridiag.c:152-158: N systems of n rows, solved in place; on return
    rhs holds the solution and bcof/ccof the eliminated factors */
@@ -193,6 +207,8 @@ typedef struct pllc_params {
     double Re, Pe, Vratio, Fx, Fy; /* physics (param.input:38-46) */
     double Inf_pos, Inf_thick;     /* front (param.input:48-50) */
     double CFLmax, Timestep;       /* (param.input:52,36) */
+    int Nmvp;                      /* non-monotonic viscosity law (param.input:64) */
+    double Mum, Lambda;            /* its peak ratio and end-slope target (param.input:66-68) */
 } pllc_params;
 
 /* field ids of pllc_sim_upload / pllc_sim_download */
@@ -247,6 +263,8 @@ double pllc_sim_op(pllc_sim *s, int op, double dt, int rk, double vbulk);
    vbulk and tval; tval/dt (nullable) receive the values after the last step */
 int pllc_sim_run(pllc_sim *s, int nsteps, double *tval, double *dt);
 double pllc_sim_timestep(const pllc_sim *s);
+/* slope parameter cm found at set-up when p->Nmvp == 1 (slope_parameter) */
+double pllc_sim_slope_parameter(const pllc_sim *s);
 int pllc_sim_synchronize(pllc_sim *s);
 /* Fill c_full (nx_max*ny_max*nz_max doubles) with an initial
    concentration: PLLC_IC_ERF is initial_condition's profile

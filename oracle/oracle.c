@@ -39,6 +39,7 @@ struct orc_sim {
     double *pk;              /* packed [nx][ny][nz] Poisson block */
     double *pk2;
     double *lam_y, *lam_z;   /* Poisson eigenvalues (pllcart3d.c:2050-2051) */
+    double cm;               /* slope parameter of the non-monotonic law */
 };
 
 #define IX(s, i, j, k) ((((size_t)(i)) * (size_t)(s)->NY + (size_t)(j)) * (size_t)(s)->NZ + (size_t)(k))
@@ -133,6 +134,7 @@ size_t orc_dims(const orc_sim *s, int *dims)
 
 double *orc_field(orc_sim *s, int id) { return (id >= 0 && id < ORC_NFIELDS) ? s->f[id] : NULL; }
 double orc_timestep(const orc_sim *s) { return s->Timestep; }
+double orc_cm(const orc_sim *s) { return s->cm; }
 
 /* ------------------------------------------------------------ line solvers */
 
@@ -433,6 +435,113 @@ static void viscosity(orc_sim *s, const double *cn, double **mu)
                 }
         }
     }
+}
+
+/* calculate_viscosity_nmvp (pllcart3d.c:889-948): mu = Mum e^M sin(psi),
+ * psi = psi1 (1-eta) + psi0 eta, eta = (1+a) c / (1 + a c), on the same
+ * points and 4-point means as calculate_viscosity */
+static double nmvp_mu(double cavg, double a, double mum, double psi0, double psi1)
+{
+    double eta = ((1 + a) * cavg) / (1 + a * cavg);
+    double psi = psi1 * (1 - eta) + psi0 * eta;
+    return mum * sin(psi);
+}
+
+static void viscosity_nmvp(orc_sim *s, const double *cn, double **mu, double cm)
+{
+    const rng_t g = s->g;
+    const double alpha = exp(s->M), mum = s->p.Mum * alpha;
+    const double psi0 = asin(alpha / mum), psi1 = s->PI - asin(1 / mum);
+    const double etam = (s->PI / 2 - psi0) / (psi1 - psi0);
+    const double a = ((1 - cm) - etam) / ((1 - cm) * (etam - 1));
+    int i;
+#pragma omp parallel for schedule(static)
+    for (i = g.is - 1; i <= g.ie + 1; i++) {
+        int j, k;
+        double cavg;
+        for (j = g.js - 1; j <= g.je + 1; j++)
+            for (k = g.ks - 1; k <= g.ke + 1; k++) mu[0][IX(s, i, j, k)] = nmvp_mu(cn[IX(s, i, j, k)], a, mum, psi0, psi1);
+        if (i >= g.is) {
+            for (j = g.js; j <= g.je + 1; j++)
+                for (k = g.ks; k <= g.ke; k++) {
+                    cavg = 0.25 * (cn[IX(s, i - 1, j - 1, k)] + cn[IX(s, i, j - 1, k)] + cn[IX(s, i - 1, j, k)] + cn[IX(s, i, j, k)]);
+                    mu[1][IX(s, i, j, k)] = nmvp_mu(cavg, a, mum, psi0, psi1);
+                }
+            for (j = g.js; j <= g.je; j++)
+                for (k = g.ks; k <= g.ke + 1; k++) {
+                    cavg = 0.25 * (cn[IX(s, i - 1, j, k - 1)] + cn[IX(s, i, j, k - 1)] + cn[IX(s, i - 1, j, k)] + cn[IX(s, i, j, k)]);
+                    mu[2][IX(s, i, j, k)] = nmvp_mu(cavg, a, mum, psi0, psi1);
+                }
+        }
+        if (i >= g.is && i <= g.ie)
+            for (j = g.js; j <= g.je + 1; j++)
+                for (k = g.ks; k <= g.ke + 1; k++) {
+                    cavg = 0.25 * (cn[IX(s, i, j - 1, k - 1)] + cn[IX(s, i, j, k - 1)] + cn[IX(s, i, j - 1, k)] + cn[IX(s, i, j, k)]);
+                    mu[3][IX(s, i, j, k)] = nmvp_mu(cavg, a, mum, psi0, psi1);
+                }
+    }
+}
+
+/* slope_parameter (pllcart3d.c:722-840) at one rank: bisection on cm so
+ * that the end slopes of mu along the centre line match Lambda.  Note the
+ * reference's psi here runs psi0 -> psi1 (783), the reverse of the
+ * viscosity law's.  Returns -1 after 500 iterations (the reference exits). */
+double orc_slope_parameter(const double *c1d, int n, double M, double Mum, double Lambda)
+{
+    const double PIv = 4.0 * atan(1.0);
+    const double alpha = exp(M), mum = Mum * alpha;
+    const double psi0 = asin(alpha / mum), psi1 = PIv - asin(1 / mum);
+    const double etam = (PIv / 2 - psi0) / (psi1 - psi0);
+    double cL = 0.1, cH = 0.9, cm = 0.5 * (cL + cH), errd = 100, errl = 1e-5;
+    double *mucell = zalloc((size_t)n);
+    int niter = 0, i, idx1, idx2;
+    while (errd > 1e-5) {
+        double a = ((1 - cm) - etam) / ((1 - cm) * (etam - 1)), mup1, mup2, lambda;
+        for (i = 0; i < n; i++) {
+            double cavg = c1d[i];
+            double eta = ((1 + a) * cavg) / (1 + a * cavg);
+            double psi = psi0 * (1 - eta) + psi1 * eta;
+            mucell[i] = mum * sin(psi);
+        }
+        idx1 = -1;
+        for (i = 0; i < n; i++) {
+            if (idx1 > 0) break;
+            if (fabs(c1d[i]) > errl) idx1 = i;
+        }
+        idx2 = -1;
+        for (i = n - 1; i >= 0; i--) {
+            if (idx2 > 0) break;
+            if (fabs(c1d[i]) < 1 - errl) idx2 = i;
+        }
+        mup1 = (mucell[idx1 + 1] - mucell[idx1]) / (c1d[idx1 + 1] - c1d[idx1]);
+        mup2 = (mucell[idx2] - mucell[idx2 - 1]) / (c1d[idx2] - c1d[idx2 - 1]);
+        lambda = (mup1 + mup2) / (alpha + 1);
+        if (lambda > Lambda) cH = cm;
+        else cL = cm;
+        cm = 0.5 * (cL + cH);
+        errd = fabs(lambda - Lambda);
+        niter++;
+        if (niter > 500) { free(mucell); return -1.0; }
+    }
+    free(mucell);
+    return cm;
+}
+
+/* the centre line of slope_parameter (742-759): the j and k whose cell
+ * centre is nearest 0 (first hit on ties) */
+static double sim_slope_parameter(orc_sim *s, const double *cn)
+{
+    double err = 10.0, *c1d = zalloc((size_t)s->nx), cm;
+    int j, k, i, jj = s->g.js, kk = s->g.ks;
+    for (j = s->g.js; j <= s->g.je; j++)
+        if (fabs(s->Yc[j] - 0.0) < err) { err = fabs(s->Yc[j] - 0.0); jj = j; }
+    err = 10.0;
+    for (k = s->g.ks; k <= s->g.ke; k++)
+        if (fabs(s->Zc[k] - 0.0) < err) { err = fabs(s->Zc[k] - 0.0); kk = k; }
+    for (i = s->g.is; i <= s->g.ie; i++) c1d[i - s->g.is] = cn[IX(s, i, jj, kk)];
+    cm = orc_slope_parameter(c1d, s->nx, s->M, s->p.Mum, s->p.Lambda);
+    free(c1d);
+    return cm;
 }
 
 /* -------------------------------------------------- diffusion coefficients */
@@ -1100,7 +1209,12 @@ void orc_init(orc_sim *s, const double *c_full)
     (void)g;
     velo_ghosts(s, s->f[ORC_PHI]);   /* update_ghost_points (81) */
     update_old(s);                   /* update_old_conc/velo (82-83) */
-    viscosity(s, s->f[ORC_C_OLD], muo);   /* (142) */
+    if (s->p.Nmvp == 1) {            /* (126-135) */
+        s->cm = sim_slope_parameter(s, s->f[ORC_C_NEW]);
+        viscosity_nmvp(s, s->f[ORC_C_OLD], muo, s->cm);
+    } else {
+        viscosity(s, s->f[ORC_C_OLD], muo);   /* (142) */
+    }
     build_A(s, 0, muo); build_A(s, 1, muo); build_A(s, 2, muo);   /* (145-148) */
     del_min = (s->dx < s->dy) ? s->dx : s->dy;
     del_min = (del_min < s->dz) ? del_min : s->dz;
@@ -1119,7 +1233,10 @@ double orc_op(orc_sim *s, int op, double dt, int rk, double vbulk)
     case ORC_OP_CONCENTRATION:
         concentration(s, dt, g, r, vbulk, s->f[ORC_U_NEW], s->f[ORC_V_NEW], s->f[ORC_W_NEW]);
         break;
-    case ORC_OP_VISCOSITY: viscosity(s, s->f[ORC_C_NEW], mun); break;
+    case ORC_OP_VISCOSITY:
+        if (s->p.Nmvp == 1) viscosity_nmvp(s, s->f[ORC_C_NEW], mun, s->cm);   /* (173-178) */
+        else viscosity(s, s->f[ORC_C_NEW], mun);
+        break;
     case ORC_OP_XMOM: momentum(s, 0, dt, g, r, vbulk, s->f[ORC_C_NEW], muo, mun); break;
     case ORC_OP_YMOM: momentum(s, 1, dt, g, r, vbulk, s->f[ORC_C_NEW], muo, mun); break;
     case ORC_OP_ZMOM: momentum(s, 2, dt, g, r, vbulk, s->f[ORC_C_NEW], muo, mun); break;

@@ -30,6 +30,8 @@ typedef struct orc_params {
     double Re, Pe, Vratio, Fx, Fy; /* physics (param.input:38-46) */
     double Inf_pos, Inf_thick;     /* front (param.input:48-50) */
     double CFLmax, Timestep;       /* (param.input:52,36) */
+    int Nmvp;                      /* non-monotonic viscosity law (param.input:64) */
+    double Mum, Lambda;            /* its peak ratio and end-slope target (param.input:66-68) */
 } orc_params;
 
 /* field ids (same numbering as the product's handle API) */
@@ -61,6 +63,10 @@ void orc_destroy(orc_sim *s);
 size_t orc_dims(const orc_sim *s, int *dims);
 double *orc_field(orc_sim *s, int id);
 double orc_timestep(const orc_sim *s);
+/* slope parameter cm of the non-monotonic law found at set-up (Nmvp = 1) */
+double orc_cm(const orc_sim *s);
+/* slope_parameter's bisection (pllcart3d.c:763-822) on a centre-line profile */
+double orc_slope_parameter(const double *c1d, int n, double M, double Mum, double Lambda);
 /* Set-up as in pllcart3d.c:75-154.  c_full (nullable) replaces the erf IC
  * of initial_condition (pllcart3d.c:501-508) on every cell incl. ghosts. */
 void orc_init(orc_sim *s, const double *c_full);

@@ -48,6 +48,10 @@ def load():
         L.orc_field.argtypes = [C.c_void_p, C.c_int]
         L.orc_timestep.restype = C.c_double
         L.orc_timestep.argtypes = [C.c_void_p]
+        L.orc_cm.restype = C.c_double
+        L.orc_cm.argtypes = [C.c_void_p]
+        L.orc_slope_parameter.restype = C.c_double
+        L.orc_slope_parameter.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_init.argtypes = [C.c_void_p, dp]
         L.orc_op.restype = C.c_double
         L.orc_op.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
@@ -107,6 +111,10 @@ class Oracle:
     @property
     def timestep(self):
         return self.L.orc_timestep(self.h)
+
+    @property
+    def cm(self):
+        return self.L.orc_cm(self.h)
 
 
 def set_threads(n):

@@ -98,6 +98,11 @@ class Sim:
     def timestep(self):
         return self.L.pllc_sim_timestep(self.h)
 
+    @property
+    def cm(self):
+        """Slope parameter of the non-monotonic viscosity law (Nmvp = 1)."""
+        return self.L.pllc_sim_slope_parameter(self.h)
+
     def time_steps(self, warmup, nsteps):
         ms = C.c_double(0.0)
         self._chk(self.L.pllc_sim_time_steps(self.h, warmup, nsteps, C.byref(ms)), "pllc_sim_time_steps")

@@ -16,7 +16,8 @@ PARAM_FIELDS = ["X1", "X2", "Y1", "Y2", "Z1", "Z2", "Re", "Pe", "Vratio", "Fx", 
 
 class Params(C.Structure):
     """pllc_params: one run's set-up (param.input values of the time step)."""
-    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + [(n, C.c_double) for n in PARAM_FIELDS]
+    _fields_ = ([("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + [(n, C.c_double) for n in PARAM_FIELDS] +
+                [("Nmvp", C.c_int), ("Mum", C.c_double), ("Lambda", C.c_double)])
 
 
 FIELDS = ["C_OLD", "C_NEW", "C_N", "C_NM1",
@@ -66,6 +67,8 @@ def load():
     L.pllc_sim_run.argtypes = [vp, C.c_int, dp, dp]
     L.pllc_sim_timestep.restype = C.c_double
     L.pllc_sim_timestep.argtypes = [vp]
+    L.pllc_sim_slope_parameter.restype = C.c_double
+    L.pllc_sim_slope_parameter.argtypes = [vp]
     L.pllc_sim_synchronize.argtypes = [vp]
     L.pllc_ic_fill.argtypes = [C.POINTER(Params), C.c_int, C.c_ulonglong, C.c_double, dp]
     L.pllc_sim_time_steps.argtypes = [vp, C.c_int, C.c_int, dp]

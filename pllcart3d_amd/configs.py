@@ -22,9 +22,10 @@ CONFIGS = {
 
 
 def make_params(nx, ny, nz, Vratio=2.0, Pe=500.0, Inf_thick=0.05, Re=1.0, Inf_pos=1.0,
-                CFLmax=0.3, Timestep=2e-3, X=(0.0, 4.0), Y=(-0.5, 0.5), Z=(-0.5, 0.5), Fx=0.0, Fy=0.0):
+                CFLmax=0.3, Timestep=2e-3, X=(0.0, 4.0), Y=(-0.5, 0.5), Z=(-0.5, 0.5), Fx=0.0, Fy=0.0,
+                Nmvp=0, Mum=1.5, Lambda=0.0):
     return Params(nx, ny, nz, X[0], X[1], Y[0], Y[1], Z[0], Z[1], Re, Pe, Vratio, Fx, Fy,
-                  Inf_pos, Inf_thick, CFLmax, Timestep)
+                  Inf_pos, Inf_thick, CFLmax, Timestep, Nmvp, Mum, Lambda)
 
 
 def config_params(name):

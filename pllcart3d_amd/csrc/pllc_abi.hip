@@ -93,6 +93,7 @@ void phys_from_globals(pllc::Phys& ph)
     ph.Re = Re; ph.Pe = Pe; ph.M = M; ph.Vnorm = Vnorm; ph.Fx = Fx; ph.Fy = Fy;
     ph.CFLmax = CFLmax; ph.Timestep = Timestep;
     ph.Y1 = Y1; ph.Y2 = Y2; ph.Z1 = Z1; ph.Z2 = Z2;
+    ph.nmvp = 0; ph.a = ph.mum = ph.psi0 = ph.psi1 = 0.0;
 }
 
 // device context for the grid's size (tables, scratch fields)
@@ -522,6 +523,48 @@ extern "C" void calculate_viscosity(Grid* grid, double*** cn, Visc* mu)
     done(W);
 }
 
+// slope_parameter (722-840) at one rank: the centre-line profile comes from
+// the device, the bisection runs on the host (as on the reference's rank 0)
+extern "C" double slope_parameter(Grid* grid, double*** cn)
+{
+    const char* W = "slope_parameter";
+    pllc::Ctx<double>* c = ctx_for(grid, W);
+    double* pc = dev_of(cn, W);
+    if (!c || !pc) return NAN;
+    int jj, kk;
+    pllc_centre_line(grid->Yc, grid->jstart, grid->jend, grid->Zc, grid->kstart, grid->kend, &jj, &kk);
+    const int n = grid->iend - grid->istart + 1;
+    std::vector<double> line(n);
+    for (int i = 0; i < n; i++)
+        if (!pllc_hip_ok(W, hipMemcpyAsync(&line[i], pc + pllc::IX(c->d, grid->istart + i, jj, kk), 8,
+                                           hipMemcpyDeviceToHost, c->st)))
+            return NAN;
+    if (!pllc_hip_ok(W, hipStreamSynchronize(c->st))) return NAN;
+    const double cm = pllc_slope_parameter(line.data(), n, M, Mum, Lambda);
+    FILE* fp = fopen("Outscreen", "a");
+    if (cm < 0) {
+        if (fp) { fprintf(fp, "Lambda did not converge in 500 iterations!"); fclose(fp); }
+        exit(1);
+    }
+    if (fp) { fprintf(fp, "cm = %f\n", cm); fclose(fp); }
+    return cm;
+}
+
+// calculate_viscosity_nmvp (889-948)
+extern "C" void calculate_viscosity_nmvp(Grid* grid, double*** cn, Visc* mu, double cm)
+{
+    const char* W = "calculate_viscosity_nmvp";
+    pllc::Ctx<double>* c = ctx_for(grid, W);
+    if (!c) return;
+    double* src = dev_of(cn, W);
+    double* dst[4] = {dev_of(mu->cell, W), dev_of(mu->xy, W), dev_of(mu->xz, W), dev_of(mu->yz, W)};
+    if (!src || !dst[0] || !dst[1] || !dst[2] || !dst[3]) return;
+    pllc_set_nmvp(c->ph, M, Mum, cm);
+    pllc::Ops<double>::viscosity(*c, src, dst);
+    c->ph.nmvp = 0;
+    done(W);
+}
+
 // concentration (555-719)
 extern "C" void concentration(double dt, double rk_gamma, double rk_rho, double vbulk, Grid* grid, double*** un,
                               double*** vn, double*** wn, Conc* cc)
@@ -893,4 +936,216 @@ extern "C" void fdct_3d(int ndim1, int ndim2, int num_transforms, double*** data
 extern "C" void idct_3d(int ndim1, int ndim2, int num_transforms, double*** datain, double*** dataout)
 {
     dct_entry(ndim1, ndim2, num_transforms, datain, dataout, true, "idct_3d");
+}
+
+// ================================================= output and diagnostics
+//
+// The reference writes these files from host memory; here each entry point
+// first brings the device fields it reads to the host (or reduces them on
+// the device), so the files have the reference's formats and contents.
+
+namespace {
+bool to_host_all(std::initializer_list<double***> arrays)
+{
+    for (double*** a : arrays)
+        if (pllc_to_host(a)) return false;
+    return true;
+}
+}  // namespace
+
+// write_to_file (3047-3092): caf%04dR%03d.bin, float32 {u,v,w,c} at cell
+// centres in i-j-k order, every second cell per axis above 2e7 cells
+extern "C" void write_to_file(int fileno, Grid* grid, double*** un, double*** vn, double*** wn, double*** cn)
+{
+    if (!to_host_all({un, vn, wn, cn})) return;
+    char fn[64];
+    snprintf(fn, sizeof(fn), "caf%04dR%03d.bin", fileno, Rank);
+    FILE* fp = fopen(fn, "wb");
+    if (!fp) { pllc_set_error("write_to_file", "cannot open output file"); return; }
+    const bool sub = (double)Nx_cell_global * Ny_cell_global * Nz_cell_global > 2e7;
+    for (int i = grid->istart; i <= grid->iend; i++)
+        for (int j = grid->jstart; j <= grid->jend; j++)
+            for (int k = grid->kstart; k <= grid->kend; k++) {
+                float data[4];
+                data[0] = (float)(0.5 * (un[i][j][k] + un[i + 1][j][k]));
+                data[1] = (float)(0.5 * (vn[i][j][k] + vn[i][j + 1][k]));
+                data[2] = (float)(0.5 * (wn[i][j][k] + wn[i][j][k + 1]));
+                data[3] = (float)cn[i][j][k];
+                if (!sub || ((i % 2 == 0) & (j % 2 == 0) & (k % 2 == 0))) fwrite(data, sizeof(float), 4, fp);
+            }
+    fclose(fp);
+}
+
+// midfield (3121-3178): middata (6 scalars, text) and midfield000.bin, 14
+// FP64 records of nz_max values per (i,j) incl. ghosts
+extern "C" void midfield(int tstep, int fileno, double tval, double outtime, double outmid, double cavgtime, Grid* grid,
+                         Conc* c, Velo* u, Velo* v, Velo* w)
+{
+    double*** rec[14] = {c->old, c->explicit_nm1, u->old, u->new_, u->explicit_nm1, u->pg, v->old,
+                         v->new_, v->explicit_nm1, v->pg, w->old, w->new_, w->explicit_nm1, w->pg};
+    for (double*** a : rec)
+        if (pllc_to_host(a)) return;
+    FILE* f1 = fopen("middata", "w");
+    if (!f1) { pllc_set_error("midfield", "cannot open middata"); return; }
+    fprintf(f1, "%d\n%d\n%lf\n%lf\n%lf\n%lf\n", tstep, fileno, tval, outtime, outmid, cavgtime);
+    fclose(f1);
+    char fn[64];
+    snprintf(fn, sizeof(fn), "midfield%03d.bin", Rank);
+    FILE* f2 = fopen(fn, "wb");
+    if (!f2) { pllc_set_error("midfield", "cannot open midfield file"); return; }
+    for (int i = 0; i < grid->nx_max; i++)
+        for (int j = 0; j < grid->ny_max; j++)
+            for (double*** a : rec) fwrite(a[i][j], sizeof(double), grid->nz_max, f2);
+    fclose(f2);
+}
+
+// intermediate (3181-3228): read the restart back and put it on the device
+extern "C" void intermediate(int* tstep, int* fileno, double* tval, double* outtime, Grid* grid, Conc* c, Velo* u,
+                             Velo* v, Velo* w)
+{
+    FILE* f1 = fopen("middata", "r");
+    if (!f1) { pllc_set_error("intermediate", "cannot open middata"); return; }
+    int ok = fscanf(f1, "%d\n", tstep) + fscanf(f1, "%d\n", fileno) + fscanf(f1, "%lf\n", tval) +
+             fscanf(f1, "%lf\n", outtime);
+    fclose(f1);
+    if (ok != 4) { pllc_set_error("intermediate", "short middata"); return; }
+    char fn[64];
+    snprintf(fn, sizeof(fn), "midfield%03d.bin", Rank);
+    FILE* f2 = fopen(fn, "rb");
+    if (!f2) { pllc_set_error("intermediate", "cannot open midfield file"); return; }
+    double*** rec[14] = {c->old, c->explicit_nm1, u->old, u->new_, u->explicit_nm1, u->pg, v->old,
+                         v->new_, v->explicit_nm1, v->pg, w->old, w->new_, w->explicit_nm1, w->pg};
+    bool full = true;
+    for (int i = 0; i < grid->nx_max; i++)
+        for (int j = 0; j < grid->ny_max; j++)
+            for (double*** a : rec) full = full && fread(a[i][j], sizeof(double), grid->nz_max, f2) == (size_t)grid->nz_max;
+    fclose(f2);
+    if (!full) { pllc_set_error("intermediate", "short midfield file"); return; }
+    for (double*** a : rec) pllc_to_device(a);
+}
+
+// find_max (3539-3603): maxima of |u|,|v|,|w|,|c| (new) with their indices
+// over every point, the NaN guard on c.old, c.new, c.explicit_n (exit(1)
+// after the Outscreen line, 3568-3588)
+extern "C" void find_max(Grid* grid, Velo* u, Velo* v, Velo* w, Conc* c)
+{
+    const char* W = "find_max";
+    DropIn& d = din();
+    const long n = (long)grid->nx_max * grid->ny_max * grid->nz_max;
+    double *pu = dev_of(u->new_, W), *pv = dev_of(v->new_, W), *pw = dev_of(w->new_, W), *pc = dev_of(c->new_, W),
+           *pco = dev_of(c->old, W), *pcx = dev_of(c->explicit_n, W);
+    if (!pu || !pv || !pw || !pc || !pco || !pcx) return;
+    const int nb = 512;
+    DevBuf part;
+    if (!part.get((size_t)nb * 9, W)) return;
+    pllc::diag_findmax_partial(n, pu, pv, pw, pc, pco, pcx, part.p, nb, d.st);
+    std::vector<double> h((size_t)nb * 9);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost, d.st))) return;
+    if (!pllc_hip_ok(W, hipStreamSynchronize(d.st))) return;
+    double mv[4] = {0, 0, 0, 0};
+    long mi[4] = {-1, -1, -1, -1}, nanat = -1;
+    for (int b = 0; b < nb; b++) {
+        for (int f = 0; f < 4; f++) {
+            const double x = h[b * 9 + 2 * f];
+            const long ix = (long)h[b * 9 + 2 * f + 1];
+            if (ix < 0) continue;
+            if (mi[f] < 0 || fabs(x) > fabs(mv[f]) || (fabs(x) == fabs(mv[f]) && ix > mi[f])) { mv[f] = x; mi[f] = ix; }
+        }
+        const long nx = (long)h[b * 9 + 8];
+        if (nx >= 0 && (nanat < 0 || nx < nanat)) nanat = nx;
+    }
+    const long sj = grid->nz_max, si = (long)grid->ny_max * grid->nz_max;
+    if (nanat >= 0) {
+        double vals[3];
+        const double* src[3] = {pco, pc, pcx};
+        for (int q = 0; q < 3; q++)
+            if (!pllc_hip_ok(W, hipMemcpy(&vals[q], src[q] + nanat, 8, hipMemcpyDeviceToHost))) return;
+        const char tag = vals[0] != vals[0] ? 'a' : (vals[1] != vals[1] ? 'b' : 'c');
+        FILE* fp = fopen("Outscreen", "a");
+        if (fp) {
+            fprintf(fp, "nan in %c %d %d %d \n", tag, (int)(nanat / si), (int)(nanat % si / sj), (int)(nanat % sj));
+            fclose(fp);
+        }
+        exit(1);
+    }
+    int ix[4][3];
+    for (int f = 0; f < 4; f++) {
+        const long l = mi[f] < 0 ? 0 : mi[f];
+        ix[f][0] = (int)(l / si); ix[f][1] = (int)(l % si / sj); ix[f][2] = (int)(l % sj);
+    }
+    FILE* fp = fopen("Outscreen", "a");
+    if (!fp) return;
+    fprintf(fp, "max\n");
+    fprintf(fp, "  %2d %2d %2d %lf\n  %2d %2d %2d %lf\n  %2d %2d %2d %lf\n  %2d %2d %2d %lf\n\n", ix[0][0], ix[0][1],
+            ix[0][2], mv[0], ix[1][0], ix[1][1], ix[1][2], mv[1], ix[2][0], ix[2][1], ix[2][2], mv[2], ix[3][0],
+            ix[3][1], ix[3][2], mv[3]);
+    fclose(fp);
+}
+
+// find_wmax (3606-3632): max of w over interior cells, appended with tval
+// to wmax.bin as two floats
+extern "C" void find_wmax(double tval, Grid* grid, Velo* w)
+{
+    const char* W = "find_wmax";
+    pllc::Ctx<double>* c = ctx_for(grid, W);
+    double* pw = dev_of(w->new_, W);
+    if (!c || !pw) return;
+    const int nb = 256;
+    DevBuf part;
+    if (!part.get(nb, W)) return;
+    pllc::diag_wmax(c->d, pw, part.p, nb, c->st);
+    std::vector<double> h(nb);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(h.data(), part.p, nb * 8, hipMemcpyDeviceToHost, c->st))) return;
+    if (!pllc_hip_ok(W, hipStreamSynchronize(c->st))) return;
+    double m = 0.0;
+    for (double x : h) m = x > m ? x : m;
+    float data[2] = {(float)m, (float)tval};
+    FILE* fp = fopen("wmax.bin", "ab");
+    if (!fp) return;
+    fwrite(data, sizeof(float), 2, fp);
+    fclose(fp);
+}
+
+// conc_average (3635-3728): y, z and yz means of c, appended as float32 to
+// cavgy.bin, cavgz.bin and cavgyz.bin (the last one preceded by tval)
+extern "C" void conc_average(Grid* grid, double*** cn, double tval)
+{
+    const char* W = "conc_average";
+    pllc::Ctx<double>* c = ctx_for(grid, W);
+    double* pc = dev_of(cn, W);
+    if (!c || !pc) return;
+    const pllc::Dims& D = c->d;
+    const size_t na = (size_t)D.nx * D.nz, nbz = (size_t)D.nx * D.ny, nc = D.nx;
+    DevBuf buf;
+    if (!buf.get((na + nbz + nc) / 2 + 8, W)) return;
+    float* fa = (float*)buf.p;
+    pllc::diag_cavg(D, pc, fa, fa + na, fa + na + nbz, c->st);
+    std::vector<float> h(na + nbz + nc);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(h.data(), fa, h.size() * 4, hipMemcpyDeviceToHost, c->st))) return;
+    if (!pllc_hip_ok(W, hipStreamSynchronize(c->st))) return;
+    FILE* fp = fopen("cavgy.bin", "ab");
+    if (fp) { fwrite(h.data(), 4, na, fp); fclose(fp); }
+    fp = fopen("cavgz.bin", "ab");
+    if (fp) { fwrite(h.data() + na, 4, nbz, fp); fclose(fp); }
+    fp = fopen("cavgyz.bin", "ab");
+    if (fp) {
+        const float t = (float)tval;
+        fwrite(&t, 4, 1, fp);
+        fwrite(h.data() + na + nbz, 4, nc, fp);
+        fclose(fp);
+    }
+}
+
+// print_to_file (3347-3379): float32 values of a 1D x-array, interior cells
+extern "C" void print_to_file(double* y, Grid* grid, int fno)
+{
+    char fn[64];
+    snprintf(fn, sizeof(fn), "cval%02d.bin", fno);
+    FILE* fp = fopen(fn, "wb");
+    if (!fp) return;
+    for (int i = grid->istart; i <= grid->iend; i++) {
+        const float data = (float)y[i];
+        fwrite(&data, sizeof(float), 1, fp);
+    }
+    fclose(fp);
 }

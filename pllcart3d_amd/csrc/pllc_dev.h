@@ -28,7 +28,24 @@ struct Rng { int is, ie, js, je, ks, ke; };
 struct Phys {
     double Re, Pe, M, Vnorm, Fx, Fy, CFLmax, Timestep;
     double Y1, Y2, Z1, Z2;
+    // viscosity law: exp(M (c - Vnorm)) (calculate_viscosity, 853) or, with
+    // nmvp, mum sin(psi1 (1-eta) + psi0 eta), eta = (1+a) c / (1 + a c)
+    // (calculate_viscosity_nmvp, 895-909)
+    int nmvp;
+    double a, mum, psi0, psi1;
 };
+
+template <typename T>
+__device__ __forceinline__ T visc_law(const Phys& ph, T cavg)
+{
+    if (ph.nmvp) {
+        const T a = T(ph.a);
+        const T eta = ((T(1) + a) * cavg) / (T(1) + a * cavg);
+        const T psi = T(ph.psi1) * (T(1) - eta) + T(ph.psi0) * eta;
+        return T(ph.mum) * sin(psi);
+    }
+    return exp(T(ph.M) * (cavg - T(ph.Vnorm)));
+}
 
 // field kinds (advected quantity of an H operator / ADI update)
 enum { FC = 0, FU = 1, FV = 2, FW = 3 };

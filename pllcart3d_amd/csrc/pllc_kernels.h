@@ -90,6 +90,11 @@ void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int
 void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,
              hipStream_t st);
 void scale(double* a, long n, double div, hipStream_t st);
+// diagnostics (find_max, conc_average, find_wmax) on FP64 device fields
+void diag_findmax_partial(long n, const double* u, const double* v, const double* w, const double* c,
+                          const double* co, const double* cx, double* part, int nb, hipStream_t st);
+void diag_cavg(const Dims& d, const double* cn, float* ay, float* az, float* ayz, hipStream_t st);
+void diag_wmax(const Dims& d, const double* w, double* part, int nb, hipStream_t st);
 
 // ';'-separated list of the kernel class names passed to Ctx::hook
 extern const char* kernel_class_names;

@@ -520,8 +520,9 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
 
 // ------------------------------------------------------------ viscosity
 
-// calculate_viscosity (843-886): cell centres on a one-cell ring, and
-// edges from the 4-point mean of c (mean first, then exp).
+// calculate_viscosity (843-886) / calculate_viscosity_nmvp (889-948): cell
+// centres on a one-cell ring, and edges from the 4-point mean of c (mean
+// first, then the law).
 template <typename T>
 __global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restrict__ cn, T* __restrict__ cell,
                                               T* __restrict__ xy, T* __restrict__ xz, T* __restrict__ yz)
@@ -530,22 +531,21 @@ __global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restri
     const int j = blockIdx.y * blockDim.y + threadIdx.y;       // 0 .. ny+1
     const int i = blockIdx.z + 2;                              // 2 .. nx+3
     if (k > d.nz + 1 || j > d.ny + 1) return;
-    const T M = T(ph.M), Vn = T(ph.Vnorm);
     const long o = IX(d, i, j, k);
-    cell[o] = exp(M * (cn[o] - Vn));
+    cell[o] = visc_law<T>(ph, cn[o]);
     const bool ix = i >= 3, iy = j >= 1, iz = k >= 1;
     const bool jx = j <= d.ny, kx = k <= d.nz, ii = i <= d.nx + 2;
     if (ix && iy && kx && iz) {
         const T cavg = T(0.25) * (cn[o - d.sx - d.sy] + cn[o - d.sy] + cn[o - d.sx] + cn[o]);
-        xy[o] = exp(M * (cavg - Vn));
+        xy[o] = visc_law<T>(ph, cavg);
     }
     if (ix && iy && jx && iz) {
         const T cavg = T(0.25) * (cn[o - d.sx - 1] + cn[o - 1] + cn[o - d.sx] + cn[o]);
-        xz[o] = exp(M * (cavg - Vn));
+        xz[o] = visc_law<T>(ph, cavg);
     }
     if (ix && ii && iy && iz) {
         const T cavg = T(0.25) * (cn[o - d.sy - 1] + cn[o - 1] + cn[o - d.sy] + cn[o]);
-        yz[o] = exp(M * (cavg - Vn));
+        yz[o] = visc_law<T>(ph, cavg);
     }
 }
 
@@ -1076,6 +1076,136 @@ __global__ void k_fill(T* __restrict__ a, long n, T v)
 {
     for (long l = (long)blockIdx.x * blockDim.x + threadIdx.x; l < n; l += (long)gridDim.x * blockDim.x) a[l] = v;
 }
+
+#ifdef PLLC_F64_TU
+// ------------------------------------------------------------ diagnostics
+
+// find_max (3539-3603) pass 1 over every point incl. ghosts: per block the
+// last index of the largest |x| of each of u, v, w, c ("<=" keeps the later
+// hit), and the first index where c.old, c.new or c.explicit_n is NaN.
+__device__ __forceinline__ bool later_max(double b, long ib, double a, long ia)
+{
+    if (ib < 0) return false;
+    if (ia < 0) return true;
+    if (fabs(b) > fabs(a)) return true;
+    if (fabs(b) < fabs(a)) return false;
+    return ib > ia;
+}
+
+__global__ __launch_bounds__(256) void k_findmax_partial(long n, const double* __restrict__ u,
+                                                         const double* __restrict__ v, const double* __restrict__ w,
+                                                         const double* __restrict__ c, const double* __restrict__ co,
+                                                         const double* __restrict__ cx, double* __restrict__ part)
+{
+    __shared__ double sv[4][256];
+    __shared__ long si[4][256];
+    __shared__ long snan[256];
+    const int t = threadIdx.x;
+    const double* q[4] = {u, v, w, c};
+    double bv[4] = {0, 0, 0, 0};
+    long bi[4] = {-1, -1, -1, -1};
+    long nanat = -1;
+    for (long l = (long)blockIdx.x * blockDim.x + t; l < n; l += (long)gridDim.x * blockDim.x) {
+        for (int f = 0; f < 4; f++) {
+            const double x = q[f][l];
+            if (fabs(bv[f]) <= fabs(x) || bi[f] < 0) {
+                if (!(x != x)) { bv[f] = x; bi[f] = l; }
+            }
+        }
+        if (nanat < 0) {
+            const double a1 = co[l], b1 = c[l], c1 = cx[l];
+            if (a1 != a1 || b1 != b1 || c1 != c1) nanat = l;
+        }
+    }
+    for (int f = 0; f < 4; f++) { sv[f][t] = bv[f]; si[f][t] = bi[f]; }
+    snan[t] = nanat;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int f = 0; f < 4; f++)
+                if (later_max(sv[f][t + s], si[f][t + s], sv[f][t], si[f][t])) { sv[f][t] = sv[f][t + s]; si[f][t] = si[f][t + s]; }
+            const long a = snan[t], b = snan[t + s];
+            if (b >= 0 && (a < 0 || b < a)) snan[t] = b;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        for (int f = 0; f < 4; f++) { part[blockIdx.x * 9 + 2 * f] = sv[f][0]; part[blockIdx.x * 9 + 2 * f + 1] = (double)si[f][0]; }
+        part[blockIdx.x * 9 + 8] = (double)snan[0];
+    }
+}
+
+// conc_average (3635-3728): y means per (i,k), z means per (i,j) and yz
+// means per i of c over interior cells, each summed in the reference's
+// order (one thread per output), divided as the reference divides.
+__global__ void k_cavg(Dims d, const double* __restrict__ cn, float* __restrict__ ay, float* __restrict__ az,
+                       float* __restrict__ ayz)
+{
+    const long nA = (long)d.nx * d.nz, nB = (long)d.nx * d.ny, nC = d.nx;
+    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nA) {
+        const int i = 3 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        double s = 0;
+        for (int j = 1; j <= d.ny; j++) s += cn[IX(d, i, j, k)];
+        ay[l] = (float)(s / d.ny);
+        return;
+    }
+    l -= nA;
+    if (l < nB) {
+        const int i = 3 + (int)(l / d.ny), j = 1 + (int)(l % d.ny);
+        double s = 0;
+        for (int k = 1; k <= d.nz; k++) s += cn[IX(d, i, j, k)];
+        az[l] = (float)(s / d.nz);
+        return;
+    }
+    l -= nB;
+    if (l < nC) {
+        const int i = 3 + (int)l;
+        double s = 0;
+        for (int j = 1; j <= d.ny; j++)
+            for (int k = 1; k <= d.nz; k++) s += cn[IX(d, i, j, k)];
+        ayz[l] = (float)(s / (d.ny * d.nz));
+    }
+}
+
+// find_wmax (3606-3632): the largest w over interior cells (not |w|), >= 0
+__global__ __launch_bounds__(256) void k_wmax(Dims d, const double* __restrict__ w, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const long np = (long)d.nx * d.ny * d.nz;
+    double m = 0.0;
+    for (long l = (long)blockIdx.x * blockDim.x + t; l < np; l += (long)gridDim.x * blockDim.x) {
+        const int i = 3 + (int)(l / ((long)d.ny * d.nz)), r = (int)(l % ((long)d.ny * d.nz));
+        const double x = w[IX(d, i, 1 + r / d.nz, 1 + r % d.nz)];
+        m = (x > m) ? x : m;
+    }
+    red[t] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) red[t] = red[t + s] > red[t] ? red[t + s] : red[t];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = red[0];
+}
+
+void diag_findmax_partial(long n, const double* u, const double* v, const double* w, const double* c,
+                          const double* co, const double* cx, double* part, int nb, hipStream_t st)
+{
+    k_findmax_partial<<<nb, 256, 0, st>>>(n, u, v, w, c, co, cx, part);
+}
+
+void diag_cavg(const Dims& d, const double* cn, float* ay, float* az, float* ayz, hipStream_t st)
+{
+    const long n = (long)d.nx * d.nz + (long)d.nx * d.ny + d.nx;
+    k_cavg<<<(unsigned)((n + 127) / 128), 128, 0, st>>>(d, cn, ay, az, ayz);
+}
+
+void diag_wmax(const Dims& d, const double* w, double* part, int nb, hipStream_t st)
+{
+    k_wmax<<<nb, 256, 0, st>>>(d, w, part);
+}
+#endif  // PLLC_F64_TU
 
 #ifdef PLLC_F64_TU
 // batched single-line Thomas for the Tds entry points: lines stored

@@ -69,6 +69,76 @@ void pllc_eigenvalues(int n, double h, double* lam)
     for (int j = 0; j < n; j++) lam[j] = 2.0 * (cos(PIv * (double)j / (double)n) - 1.0) / (h * h);
 }
 
+// ------------------------------------------- non-monotonic viscosity law
+
+// the law's constants for slope parameter cm (calculate_viscosity_nmvp,
+// pllcart3d.c:895-900)
+void pllc_set_nmvp(Phys& ph, double M, double Mum, double cm)
+{
+    const double PIv = 4.0 * atan(1.0);
+    const double alpha = exp(M), mum = Mum * alpha;
+    const double psi0 = asin(alpha / mum), psi1 = PIv - asin(1 / mum);
+    const double etam = (PIv / 2 - psi0) / (psi1 - psi0);
+    ph.nmvp = 1;
+    ph.mum = mum;
+    ph.psi0 = psi0;
+    ph.psi1 = psi1;
+    ph.a = ((1 - cm) - etam) / ((1 - cm) * (etam - 1));
+}
+
+// slope_parameter's bisection (pllcart3d.c:763-822) on the centre-line
+// profile c1d: cm such that the mean end slope of mu(c) matches Lambda.
+// As in the reference, psi runs psi0 -> psi1 here (783).  Returns -1 when
+// 500 iterations do not converge (the reference exits there, 813-818).
+double pllc_slope_parameter(const double* c1d, int n, double M, double Mum, double Lambda)
+{
+    const double PIv = 4.0 * atan(1.0);
+    const double alpha = exp(M), mum = Mum * alpha;
+    const double psi0 = asin(alpha / mum), psi1 = PIv - asin(1 / mum);
+    const double etam = (PIv / 2 - psi0) / (psi1 - psi0);
+    double cL = 0.1, cH = 0.9, cm = 0.5 * (cL + cH), errd = 100;
+    const double errl = 1e-5;
+    std::vector<double> mu(n);
+    for (int niter = 1; errd > 1e-5; niter++) {
+        const double a = ((1 - cm) - etam) / ((1 - cm) * (etam - 1));
+        for (int i = 0; i < n; i++) {
+            const double eta = ((1 + a) * c1d[i]) / (1 + a * c1d[i]);
+            const double psi = psi0 * (1 - eta) + psi1 * eta;
+            mu[i] = mum * sin(psi);
+        }
+        // first index (from 1 on, see the break test) above errl, last below 1-errl
+        int idx1 = -1, idx2 = -1;
+        for (int i = 0; i < n && !(idx1 > 0); i++)
+            if (fabs(c1d[i]) > errl) idx1 = i;
+        for (int i = n - 1; i >= 0 && !(idx2 > 0); i--)
+            if (fabs(c1d[i]) < 1 - errl) idx2 = i;
+        if (idx1 < 0 || idx2 < 1 || idx1 + 1 >= n) return -1.0;
+        const double mup1 = (mu[idx1 + 1] - mu[idx1]) / (c1d[idx1 + 1] - c1d[idx1]);
+        const double mup2 = (mu[idx2] - mu[idx2 - 1]) / (c1d[idx2] - c1d[idx2 - 1]);
+        const double lambda = (mup1 + mup2) / (alpha + 1);
+        if (lambda > Lambda) cH = cm;
+        else cL = cm;
+        cm = 0.5 * (cL + cH);
+        errd = fabs(lambda - Lambda);
+        if (niter > 500) return -1.0;
+    }
+    return cm;
+}
+
+// the centre line of slope_parameter (742-759): j and k of the cell centres
+// nearest 0, first hit on ties
+void pllc_centre_line(const double* Yc, int js, int je, const double* Zc, int ks, int ke, int* jj, int* kk)
+{
+    double err = 10.0;
+    *jj = js;
+    for (int j = js; j <= je; j++)
+        if (fabs(Yc[j] - 0.0) < err) { err = fabs(Yc[j] - 0.0); *jj = j; }
+    err = 10.0;
+    *kk = ks;
+    for (int k = ks; k <= ke; k++)
+        if (fabs(Zc[k] - 0.0) < err) { err = fabs(Zc[k] - 0.0); *kk = k; }
+}
+
 // -------------------------------------------------------- IC recipes
 
 namespace {
@@ -237,6 +307,7 @@ struct pllc_sim {
     std::vector<void*> owned;
     bool new_stale = false;       // old/new pairs swapped by the fused step
     double Timestep = 0.0;
+    double cm = 0.0;              // slope parameter (Nmvp = 1)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // kernel-class timer
     const char* tname = nullptr;
@@ -261,6 +332,7 @@ void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
     ph.Re = p->Re; ph.Pe = p->Pe; ph.M = p->Vratio; ph.Vnorm = 0.0;
     ph.Fx = p->Fx; ph.Fy = p->Fy; ph.CFLmax = p->CFLmax; ph.Timestep = p->Timestep;
     ph.Y1 = p->Y1; ph.Y2 = p->Y2; ph.Z1 = p->Z1; ph.Z2 = p->Z2;
+    ph.nmvp = 0; ph.a = ph.mum = ph.psi0 = ph.psi1 = 0.0;
 }
 
 template <typename T>
@@ -324,6 +396,7 @@ extern "C" size_t pllc_sim_dims(const pllc_sim* s, int* dims)
 
 extern "C" int pllc_sim_precision(const pllc_sim* s) { return s->prec; }
 extern "C" double pllc_sim_timestep(const pllc_sim* s) { return s->Timestep; }
+extern "C" double pllc_sim_slope_parameter(const pllc_sim* s) { return s->cm; }
 
 static int finish(pllc_sim* s, const char* where)
 {
@@ -402,6 +475,17 @@ static int init_t(pllc_sim* s, Ctx<T>& c, const double* c_full)
     if (!h2d<T>(s, c.f[U_NEW] + IX(d, 3, 0, 0), plane.data(), plane.size(), W)) return -1;
     Ops<T>::velo_ghost(c);   // update_ghost_points (81); phi is 0
     update_old_copy<T>(s, c);  // (82-83)
+    if (s->p.Nmvp == 1) {
+        // slope_parameter on c->new after the initial condition (133)
+        int jj, kk;
+        pllc_centre_line(g.Yc.data(), 1, d.ny, g.Zc.data(), 1, d.nz, &jj, &kk);
+        std::vector<double> line(d.nx);
+        for (int i = 0; i < d.nx; i++) line[i] = h[(size_t)IX(d, 3 + i, jj, kk)];
+        const double cm = pllc_slope_parameter(line.data(), d.nx, s->p.Vratio, s->p.Mum, s->p.Lambda);
+        if (cm < 0) { pllc_set_error(W, "slope parameter did not converge in 500 iterations"); return -1; }
+        s->cm = cm;
+        pllc_set_nmvp(c.ph, s->p.Vratio, s->p.Mum, cm);
+    }
     T* mu[4] = {c.f[MUO_CELL], c.f[MUO_XY], c.f[MUO_XZ], c.f[MUO_YZ]};
     Ops<T>::viscosity(c, c.f[C_OLD], mu);   // (142)
     // Timestep capped by the diffusion limit (150-154)

@@ -1,0 +1,118 @@
+"""The drop-in surface end to end: run3_amd (the reference's main call
+sequence through the reference-named entry points of libpllcart3d_amd.so,
+param.input read by input()) vs the FP64 oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pllcart3d_amd as P
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# param.input values in the reference's order (param.input:1-68), Nmvp = 0
+PARAMS = [("Nx_cell_global", 24), ("Ny_cell_global", 12), ("Nz_cell_global", 10), ("Nx_ghost", 3),
+          ("Ny_ghost", 1), ("Nz_ghost", 1), ("Initial", 1), ("Ugrid", 1), ("Niter_print", 1000),
+          ("Out_mid", 100.0), ("X1", 0.0), ("X2", 4.0), ("Y1", -0.5), ("Y2", 0.5), ("Z1", -0.5), ("Z2", 0.5),
+          ("Timemax", 100.0), ("Timestep", 2e-3), ("Re", 1.0), ("Pe", 300.0), ("Vratio", 3.0), ("Fx", 0.0),
+          ("Fy", 0.0), ("Inf_pos", 1.0), ("Inf_thick", 0.1), ("CFLmax", 0.3), ("Out_inter", 100.0),
+          ("Wavpert", 0), ("Ampli", 0.1), ("Lamb", 3.0), ("Subharm", 0), ("Nmvp", 0), ("Mum", 1.5),
+          ("Lambda", 0.0)]
+
+
+def write_param_input(path):
+    with open(path, "w") as f:
+        for name, val in PARAMS:
+            f.write("//%s\n%r\n" % (name, val))
+
+
+def test_driver_matches_oracle(tmp_path, oracle_mod):
+    write_param_input(tmp_path / "param.input")
+    nsteps = 4
+    exe = os.path.join(os.path.dirname(P.__file__), "run3_amd")
+    print("[dropin] run3_amd %d steps 24x12x10" % nsteps, flush=True)
+    r = subprocess.run([exe, str(nsteps), "dump.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "timetaken" in (tmp_path / "Outscreen").read_text()
+    shape = (30, 14, 12)
+    dump = np.fromfile(tmp_path / "dump.bin").reshape(8, *shape)
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
+    orc = oracle_mod.Oracle(prm)
+    orc.init()
+    orc.run(nsteps)
+    for q, f in enumerate(["C_NEW", "U_NEW", "V_NEW", "W_NEW", "PHI", "U_PG", "V_PG", "W_PG"]):
+        e = rel_l2(dump[q], orc.field(f))
+        print("[dropin] %s rel L2 %.2e" % (f, e), flush=True)
+        assert e <= 1e-11, (f, e)
+
+
+def test_driver_nmvp_matches_oracle(tmp_path, oracle_mod):
+    """run3_amd with Nmvp = 1 (the reference's shipped param.input setting):
+    slope_parameter + calculate_viscosity_nmvp through the drop-in."""
+    _run(tmp_path, 3, {"Nmvp": 1, "Mum": 1.5, "Lambda": 0.0}, "dump.bin")
+    dump = np.fromfile(tmp_path / "dump.bin").reshape(8, 30, 14, 12)
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1, Nmvp=1, Mum=1.5, Lambda=0.0)
+    orc = oracle_mod.Oracle(prm)
+    orc.init()
+    orc.run(3)
+    assert ("cm = %f" % orc.cm) in (tmp_path / "Outscreen").read_text()
+    for q, f in enumerate(["C_NEW", "U_NEW", "V_NEW", "W_NEW", "PHI", "U_PG", "V_PG", "W_PG"]):
+        e = rel_l2(dump[q], orc.field(f))
+        print("[dropin nmvp] %s rel L2 %.2e" % (f, e), flush=True)
+        assert e <= 1e-11, (f, e)
+
+
+def _run(tmp, nsteps, overrides, dump=None):
+    vals = dict(PARAMS)
+    vals.update(overrides)
+    with open(tmp / "param.input", "w") as f:
+        for name, _ in PARAMS:
+            f.write("//%s\n%r\n" % (name, vals[name]))
+    exe = os.path.join(os.path.dirname(P.__file__), "run3_amd")
+    args = [exe, str(nsteps)] + ([dump] if dump else [])
+    r = subprocess.run(args, cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_driver_outputs_and_restart(tmp_path, oracle_mod):
+    """caf files, conc_average, find_max and the midfield/intermediate
+    restart: 3 steps + restart + 2 steps equals 5 straight steps."""
+    shape = (30, 14, 12)
+    a, b, c = tmp_path / "a", tmp_path / "b", tmp_path / "c"
+    for d in (a, b, c):
+        d.mkdir()
+    io = {"Out_inter": 0.0035, "Out_mid": 0.005, "Niter_print": 1}
+    print("[dropin] outputs + restart", flush=True)
+    _run(a, 3, io)
+    for f in ("middata", "midfield000.bin"):
+        (b / f).write_bytes((a / f).read_bytes())
+    _run(b, 2, dict(io, Initial=0), "dump.bin")
+    _run(c, 5, io, "dump.bin")
+    db = np.fromfile(b / "dump.bin").reshape(8, *shape)
+    dc = np.fromfile(c / "dump.bin").reshape(8, *shape)
+    for q in range(8):
+        assert rel_l2(db[q], dc[q]) <= 1e-13, q
+
+    # caf0002 holds the state after step 4 (writes at t=0, 0.004, 0.008)
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
+    orc = oracle_mod.Oracle(prm)
+    orc.init()
+    orc.run(1)
+    cyz = orc.field("C_NEW")[3:27, 1:13, 1:11].mean(axis=(1, 2))
+    rec = np.fromfile(c / "cavgyz.bin", dtype=np.float32)
+    assert abs(rec[0] - 0.002) < 1e-6
+    assert np.allclose(rec[1:25], cyz, rtol=1e-6, atol=1e-7)
+    orc.run(3)
+    u, v, w, cc = (orc.field(n) for n in ("U_NEW", "V_NEW", "W_NEW", "C_NEW"))
+    ref = np.stack([0.5 * (u[3:27, 1:13, 1:11] + u[4:28, 1:13, 1:11]),
+                    0.5 * (v[3:27, 1:13, 1:11] + v[3:27, 2:14, 1:11]),
+                    0.5 * (w[3:27, 1:13, 1:11] + w[3:27, 1:13, 2:12]),
+                    cc[3:27, 1:13, 1:11]], axis=-1).astype(np.float32)
+    caf = np.fromfile(c / "caf0002R000.bin", dtype=np.float32).reshape(24, 12, 10, 4)
+    assert np.allclose(caf, ref, rtol=1e-6, atol=1e-7)
+    assert (c / "caf0000R000.bin").exists() and (c / "caf0001R000.bin").exists()
+    out = (c / "Outscreen").read_text()
+    assert out.count("max\n") == 5 and "timetaken" in out

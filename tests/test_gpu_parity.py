@@ -92,3 +92,18 @@ def test_fused_steps_track_oracle(oracle_mod):
         w = _compare(orc, sim, ["C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "V_PG", "W_PG"],
                      what="step %d" % (n + 1))
         print("[steps] %d worst %.2e" % (n + 1, max(w.values())), flush=True)
+
+
+def test_nmvp_steps_track_oracle(oracle_mod):
+    """Non-monotonic viscosity law (Nmvp = 1): the slope parameter found at
+    set-up and whole fused steps."""
+    prm, orc, sim = _pair(oracle_mod, Nmvp=1, Mum=1.5, Lambda=0.0)
+    print("[nmvp] cm oracle %.15f gpu %.15f" % (orc.cm, sim.cm), flush=True)
+    assert sim.cm == orc.cm and 0.1 < sim.cm < 0.9
+    _compare(orc, sim, ["MUO_CELL", "MUO_XY", "MUO_XZ", "MUO_YZ"], what="nmvp init")
+    for n in range(6):
+        orc.run(1)
+        sim.run(1)
+        w = _compare(orc, sim, ["C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "MUO_CELL", "MUO_YZ"],
+                     what="nmvp step %d" % (n + 1))
+        print("[nmvp] %d worst %.2e" % (n + 1, max(w.values())), flush=True)

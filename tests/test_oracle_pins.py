@@ -386,3 +386,74 @@ def test_uniform_state_is_steady(O):
     o.run(2)
     assert np.all(o.field("C_OLD") == 0.37)
     assert np.all(o.field("V_OLD") == 0.0)
+
+
+# ------------------------------------------------ non-monotonic viscosity
+
+def _nmvp_consts(M, Mum):
+    alpha = np.exp(M)
+    mum = Mum * alpha
+    return alpha, mum, np.arcsin(alpha / mum), np.pi - np.arcsin(1 / mum)
+
+
+def test_nmvp_law_end_values_and_peak(O):
+    """calculate_viscosity_nmvp: mu(0) = 1, mu(1) = e^M, and on a ramp of c
+    the profile rises to the single peak Mum e^M and falls to e^M; the edge
+    means of a field constant in y and z reduce to the face values."""
+    nx, ny, nz, M, Mum = 240, 3, 2, 2.5, 1.5
+    prm = _params(nx, ny, nz, Vratio=M, Nmvp=1, Mum=Mum, Lambda=0.0)
+    o = O.Oracle(prm)
+    o.init()
+    assert 0.1 < o.cm < 0.9
+    names = ("MUN_CELL", "MUN_XY", "MUN_XZ", "MUN_YZ")
+    for c0, mu0 in ((0.0, 1.0), (1.0, np.exp(M))):
+        o.field("C_NEW")[...] = c0
+        o.op("VISCOSITY")
+        for n in names:
+            f = o.field(n)[3:nx + 3, 1:ny + 1, 1:nz + 1]
+            assert np.allclose(f, mu0, rtol=1e-14, atol=0), (c0, n)
+    ramp = (np.arange(nx + 6) - 2.0) / (nx + 1)          # c = 0 at i = 2, 1 at i = nx+3
+    o.field("C_NEW")[...] = ramp[:, None, None]
+    o.op("VISCOSITY")
+    cell = o.field("MUN_CELL")[2:nx + 4, 1, 1]
+    assert abs(cell[0] - 1.0) < 1e-14 and abs(cell[-1] - np.exp(M)) < 1e-13 * np.exp(M)
+    top = int(np.argmax(cell))
+    assert 0 < top < len(cell) - 1
+    assert np.all(np.diff(cell[:top + 1]) > 0) and np.all(np.diff(cell[top:]) < 0)
+    assert Mum * np.exp(M) * (1 - 1e-4) < cell[top] <= Mum * np.exp(M) * (1 + 1e-15)
+    s = np.s_[3:nx + 3, 1:ny + 1, 1:nz + 1]
+    assert np.allclose(o.field("MUN_YZ")[s], o.field("MUN_CELL")[s], rtol=1e-15, atol=0)
+    assert np.allclose(o.field("MUN_XY")[s], o.field("MUN_XZ")[s], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("M,Mum,Lam", [(2.5, 1.5, 0.0), (3.0, 1.2, 0.2), (1.5, 2.0, -0.1)])
+def test_slope_parameter_matches_analytic_root(O, M, Mum, Lam):
+    """slope_parameter's bisection on a fine linear profile c = 0..1 finds
+    the root of lambda(cm) = (mu'(0) + mu'(1)) / (e^M + 1) = Lambda, with
+    mu' taken analytically for the law psi = psi0 + (psi1-psi0) eta and
+    eta = (1+a)c/(1+ac); the law peaks at mu = Mum e^M at c = 1 - cm."""
+    from scipy.optimize import brentq
+    alpha, mum, p0, p1 = _nmvp_consts(M, Mum)
+    etam = (np.pi / 2 - p0) / (p1 - p0)
+
+    def a_of(cm):
+        return ((1 - cm) - etam) / ((1 - cm) * (etam - 1))
+
+    def dmu(c, cm):
+        a = a_of(cm)
+        eta = (1 + a) * c / (1 + a * c)
+        return mum * np.cos(p0 + (p1 - p0) * eta) * (p1 - p0) * (1 + a) / (1 + a * c) ** 2
+
+    lam = lambda cm: (dmu(0.0, cm) + dmu(1.0, cm)) / (alpha + 1) - Lam
+    root = brentq(lam, 0.1 + 1e-9, 0.9 - 1e-9, xtol=1e-14)
+    n = 20001
+    c1d = np.linspace(0.0, 1.0, n)
+    L = O.load()
+    cm = L.orc_slope_parameter(dp(c1d), n, M, Mum, Lam)
+    assert abs(cm - root) < 2e-3, (cm, root)
+    # peak of that orientation at c = 1 - cm (bisection midpoint accuracy)
+    cc = np.linspace(0, 1, 200001)
+    a = a_of(cm)
+    mu = mum * np.sin(p0 + (p1 - p0) * (1 + a) * cc / (1 + a * cc))
+    assert abs(cc[np.argmax(mu)] - (1 - cm)) < 1e-4
+    assert abs(mu.max() - mum) < 1e-9 * mum
