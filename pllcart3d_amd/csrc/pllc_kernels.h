@@ -48,6 +48,9 @@ struct Ctx {
     double* part = nullptr;   // reduction partials
     double* tval = nullptr;   // device copy of tval
     long npart = 0;
+    // ADI line solver: 0 partitioned chunks (k_sweep_part*), 1 one thread
+    // per line with the reference's Thomas arithmetic (k_sweep*)
+    int line_solver = 0;
     // optional per-launch hook (phase 0 before, 1 after a launch of the
     // named kernel class), used for HIP-event timing of one kernel class
     void (*hook)(void* user, const char* name, int phase) = nullptr;

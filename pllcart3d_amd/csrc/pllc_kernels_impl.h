@@ -1270,6 +1270,9 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_sweep<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 2>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep_z_lds<T>(Dims, LineSets<T>, const double*, T, T);                         \
+    template __global__ void k_sweep_part<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                 \
+    template __global__ void k_sweep_part<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                 \
+    template __global__ void k_sweep_part_z<T>(Dims, LineSets<T>, const double*, T, T, int);                   \
     template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
     template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
@@ -1287,6 +1290,317 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long);        \
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
+
+// ------------------------------------------------ partitioned line solve
+//
+// The same systems as k_sweep, cut into P chunks of at most PM rows per
+// line, one thread per chunk (the partition method for tridiagonal systems
+// used by hybrid batched solvers).  A thread eliminates its chunk down to
+// rows of the form A_t x_first + x_t + C_t x_last = D_t; its first and last
+// rows couple only to the neighbouring chunks' boundary unknowns and make a
+// reduced tridiagonal system of 2P rows per line, solved in LDS by one
+// thread per line; every row then follows from its chunk's two boundary
+// values.  Compared with one thread per line this gives P times the
+// threads, keeps the eliminated rows in registers (no Thomas factors in
+// HBM: S and mu are read once, the solution written once) and reads each
+// row with one coalesced access per wavefront.  (I - fac A) is strictly
+// diagonally dominant, so the result equals the Thomas solution to rounding.
+constexpr int PM = 16;
+
+// rows [x0, x0+m) of a line: forward elimination as rows arrive, then the
+// backward pass and the first row's fold; leaves the two reduced rows in
+// A/C/D[0] and A/C/D[m-1]
+template <typename T>
+struct PChunk {
+    T A[PM], C[PM], D[PM];
+
+    // row t of the chunk (t is a constant once the callers' loops unroll)
+    __device__ __forceinline__ void fwd(int t, T a, T b, T c, T d)
+    {
+        if (t < 2) {
+            const T rb = T(1.0) / b;
+            D[t] = d * rb; A[t] = a * rb; C[t] = c * rb;
+        } else {
+            const T rb = T(1.0) / (b - a * C[t - 1]);
+            D[t] = rb * (d - a * D[t - 1]);
+            A[t] = -rb * a * A[t - 1];
+            C[t] = rb * c;
+        }
+    }
+
+    __device__ __forceinline__ void fold(int m)
+    {
+#pragma unroll
+        for (int t = PM - 3; t >= 1; t--) {
+            if (t <= m - 3) {
+                D[t] = D[t] - C[t] * D[t + 1];
+                A[t] = A[t] - C[t] * A[t + 1];
+                C[t] = -C[t] * C[t + 1];
+            }
+        }
+        const T rb = T(1.0) / (T(1.0) - C[0] * A[1]);
+        D[0] = rb * (D[0] - C[0] * D[1]);
+        A[0] = rb * A[0];
+        C[0] = -rb * C[0] * C[1];
+    }
+
+    // the chunk's last row (runtime m, static register indices)
+    __device__ __forceinline__ void last(int m, T& a, T& c, T& d) const
+    {
+#pragma unroll
+        for (int t = 2; t < PM; t++)
+            if (t == m - 1) { a = A[t]; c = C[t]; d = D[t]; }
+    }
+};
+
+// the reduced system of one line: rows e = 2s (first row of chunk s) and
+// e = 2s+1 (last row), unit diagonal; stored [3][E][LPB] with the line
+// index fastest.  Thomas in place: D ends as the boundary values.
+template <typename T>
+__device__ __forceinline__ void pchunk_reduced(T* __restrict__ red, int E, int LPB, int q)
+{
+    T* rA = red;
+    T* rC = red + E * LPB;
+    T* rD = red + 2 * E * LPB;
+    T cp = rC[q], dp = rD[q];
+    for (int e = 1; e < E; e++) {
+        const int o = e * LPB + q;
+        const T a = rA[o];
+        const T rb = T(1.0) / (T(1.0) - a * cp);
+        cp = rC[o] * rb;
+        dp = (rD[o] - a * dp) * rb;
+        rC[o] = cp;
+        rD[o] = dp;
+    }
+    T xn = dp;
+    for (int e = E - 2; e >= 0; e--) {
+        const int o = e * LPB + q;
+        xn = rD[o] - rC[o] * xn;
+        rD[o] = xn;
+    }
+}
+
+// raw row of line set L at row x (x in [0, n)): coefficients from the
+// viscosity value of the row below (araw) and above (craw), boundary rows
+// closed as in k_sweep; a at row 0 and c at row n-1 are dropped after
+template <typename T, int DIR>
+__device__ __forceinline__ void prow(const LineSet<T>& L, int x, int n, T fac, T araw, T craw, T bcor, T& a, T& b,
+                                     T& c, T& rhs)
+{
+    const bool cst = L.mu == nullptr;
+    const T braw = cst ? L.cb : -(araw + craw);
+    a = -fac * araw;
+    b = T(1.0) - fac * braw;
+    c = -fac * craw;
+    if (x == 0) { apply_bc(L.bc0, b, a, rhs); a = T(0.0); }
+    if (x == n - 1) {
+        if (DIR == 0) rhs -= c * bcor;
+        else apply_bc(L.bc1, b, c, rhs);
+        c = T(0.0);
+    }
+}
+
+// x- and y-lines (rows strided by sx / sy): block (LPB lines along k,
+// P chunks); a wavefront reads one row of consecutive k, coalesced
+template <typename T, int DIR>
+__global__ __launch_bounds__(256) void k_sweep_part(Dims d, LineSets<T> sets, const double* __restrict__ scal,
+                                                    int slot, T g, T r)
+{
+    extern __shared__ unsigned char pllc_smem[];
+    T* red = reinterpret_cast<T*>(pllc_smem);
+    const int LPB = blockDim.x, P = blockDim.y, E = 2 * P;
+    const int q = threadIdx.x, s = threadIdx.y;
+    const LineSet<T>& L = sets.s[blockIdx.y];
+    const Rng R = L.R;
+    const int n0 = R.ke - R.ks + 1;
+    const int n1 = DIR == 0 ? R.je - R.js + 1 : R.ie - R.is + 1;
+    const long l = (long)blockIdx.x * LPB + q;
+    const bool live = l < (long)n0 * n1;
+    const int a0 = live ? (int)(l % n0) : 0, a1 = live ? (int)(l / n0) : 0;
+    int i, j, k, n;
+    long st;
+    if (DIR == 0) { k = R.ks + a0; j = R.js + a1; i = R.is; n = R.ie - R.is + 1; st = d.sx; }
+    else { k = R.ks + a0; i = R.is + a1; j = R.js; n = R.je - R.js + 1; st = d.sy; }
+    const int x0 = (int)(((long)s * n) / P);
+    const int m = (int)(((long)(s + 1) * n) / P) - x0;
+    const long o0 = IX(d, i, j, k);
+    T* __restrict__ S = L.S + o0;
+    const bool cst = L.mu == nullptr;
+    const T* __restrict__ mu = cst ? nullptr : L.mu + o0 + L.mu_off;
+    const T del = L.del, h = L.h;
+    const T dt = T(scal[S_DT]);
+    const T fac = T(0.5) * dt * (g + r);
+
+    PChunk<T> ch;
+    T qo[PM];
+    if (live) {
+        T bcor = T(0.0);
+        if (DIR == 0 && s == P - 1) {
+            // explicit outflow face iend+1 (677-683 and matching lines)
+            const long ob = o0 + (long)n * st;
+            const T vbulk = T(scal[S_RK + 4 * slot]);
+            const T nb = -vbulk * (L.qold[ob] - L.qold[ob - st]) / T(d.dx);
+            L.N[ob] = nb;
+            bcor = dt * (g * nb + r * L.Nm1[ob]);
+            L.qnew[ob] = L.qold[ob] + bcor;
+        }
+        T sv[PM], mv[PM + 1];
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < m) sv[t] = S[(long)(x0 + t) * st];
+        if (!cst) {
+#pragma unroll
+            for (int t = 0; t <= PM; t++)
+                if (t <= m) mv[t] = mu[(long)(x0 + t) * st];
+        }
+        T araw = cst ? L.ca : del * mv[0] / h;
+#pragma unroll
+        for (int t = 0; t < PM; t++) {
+            if (t < m) {
+                const T craw = cst ? L.cc : del * mv[t + 1] / h;
+                T a, b, c, rhs = sv[t];
+                prow<T, DIR>(L, x0 + t, n, fac, araw, craw, bcor, a, b, c, rhs);
+                ch.fwd(t, a, b, c, rhs);
+                araw = craw;
+            }
+        }
+        ch.fold(m);
+        T la, lc, ld;
+        ch.last(m, la, lc, ld);
+        const int e0 = 2 * s;
+        red[e0 * LPB + q] = ch.A[0];
+        red[(E + e0) * LPB + q] = ch.C[0];
+        red[(2 * E + e0) * LPB + q] = ch.D[0];
+        red[(e0 + 1) * LPB + q] = la;
+        red[(E + e0 + 1) * LPB + q] = lc;
+        red[(2 * E + e0 + 1) * LPB + q] = ld;
+    }
+    __syncthreads();
+    if (live && s == 0) pchunk_reduced<T>(red, E, LPB, q);
+    __syncthreads();
+    if (!live) return;
+    if (DIR == 0) {
+        // all loads of q_old before the first store of q_new
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < m) qo[t] = L.qold[o0 + (long)(x0 + t) * st];
+    }
+    const T X0 = red[(2 * E + 2 * s) * LPB + q], X1 = red[(2 * E + 2 * s + 1) * LPB + q];
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        if (t < m) {
+            const T sol = t == 0 ? X0 : t == m - 1 ? X1 : ch.D[t] - ch.A[t] * X0 - ch.C[t] * X1;
+            const long o = (long)(x0 + t) * st;
+            if (DIR == 0) {
+                const T val = qo[t] + sol;
+                T* qn = L.qnew;
+                qn[o0 + o] = val;
+                if (L.ghost_c) {
+                    // c_new's ghost copies of this value: y ghosts (701-704),
+                    // then z ghosts incl. the y-ghost rows (707-712)
+                    const int ii = R.is + x0 + t;
+                    if (j == R.js) qn[IX(d, ii, j - 1, k)] = val;
+                    if (j == R.je) qn[IX(d, ii, j + 1, k)] = val;
+                    if (k == R.ks) {
+                        qn[IX(d, ii, j, k - 1)] = val;
+                        if (j == R.js) qn[IX(d, ii, j - 1, k - 1)] = val;
+                        if (j == R.je) qn[IX(d, ii, j + 1, k - 1)] = val;
+                    }
+                    if (k == R.ke) {
+                        qn[IX(d, ii, j, k + 1)] = val;
+                        if (j == R.js) qn[IX(d, ii, j - 1, k + 1)] = val;
+                        if (j == R.je) qn[IX(d, ii, j + 1, k + 1)] = val;
+                    }
+                }
+            } else {
+                S[o] = sol;
+            }
+        }
+    }
+}
+
+// z-lines (rows contiguous along k): the block's LPB lines (one i, LPB
+// consecutive j) are staged through LDS with consecutive threads on
+// consecutive k, solved by chunk from LDS, and written back the same way.
+// w is the LDS row pitch of a line (at least n + 1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
+                                                      T r, int w)
+{
+    extern __shared__ unsigned char pllc_smem[];
+    const int LPB = blockDim.x, P = blockDim.y, E = 2 * P, NT = LPB * P;
+    const int q = threadIdx.x, s = threadIdx.y, tid = s * LPB + q;
+    const LineSet<T>& L = sets.s[blockIdx.z];
+    const Rng R = L.R;
+    const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
+    const int i = R.is + blockIdx.y;
+    const int nl = (i <= R.ie) ? min(LPB, nj - (int)(blockIdx.x * LPB)) : 0;
+    if (nl <= 0) return;   // whole block
+    // tiles tS[LPB][w], tM[LPB][w] (w >= n + 1: rows and the mu row above
+    // the last), then the reduced rows
+    T* tS = reinterpret_cast<T*>(pllc_smem);
+    T* tM = tS + LPB * w;
+    T* red = tM + LPB * w;
+    const long base = IX(d, i, R.js + blockIdx.x * LPB, R.ks);
+    const bool cst = L.mu == nullptr;
+    const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
+    const T del = L.del, h = L.h;
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+
+    for (int lq = 0; lq < nl; lq++) {
+        const long gb = base + (long)lq * d.sy;
+        for (int x = tid; x < n; x += NT) tS[lq * w + x] = L.S[gb + x];
+        if (!cst)
+            for (int x = tid; x <= n; x += NT) tM[lq * w + x] = mu[gb + x];
+    }
+    __syncthreads();
+
+    const bool live = q < nl;
+    const int x0 = (int)(((long)s * n) / P);
+    const int m = (int)(((long)(s + 1) * n) / P) - x0;
+    PChunk<T> ch;
+    if (live) {
+        const T* rs = tS + q * w + x0;
+        const T* rm = tM + q * w + x0;
+        T araw = cst ? L.ca : del * rm[0] / h;
+#pragma unroll
+        for (int t = 0; t < PM; t++) {
+            if (t < m) {
+                const T craw = cst ? L.cc : del * rm[t + 1] / h;
+                T a, b, c, rhs = rs[t];
+                prow<T, 2>(L, x0 + t, n, fac, araw, craw, T(0.0), a, b, c, rhs);
+                ch.fwd(t, a, b, c, rhs);
+                araw = craw;
+            }
+        }
+        ch.fold(m);
+        T la, lc, lD;
+        ch.last(m, la, lc, lD);
+        const int e0 = 2 * s;
+        red[e0 * LPB + q] = ch.A[0];
+        red[(E + e0) * LPB + q] = ch.C[0];
+        red[(2 * E + e0) * LPB + q] = ch.D[0];
+        red[(e0 + 1) * LPB + q] = la;
+        red[(E + e0 + 1) * LPB + q] = lc;
+        red[(2 * E + e0 + 1) * LPB + q] = lD;
+    }
+    __syncthreads();
+    if (live && s == 0) pchunk_reduced<T>(red, E, LPB, q);
+    __syncthreads();
+    if (live) {
+        const T X0 = red[(2 * E + 2 * s) * LPB + q], X1 = red[(2 * E + 2 * s + 1) * LPB + q];
+        T* ws = tS + q * w + x0;
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < m) ws[t] = t == 0 ? X0 : t == m - 1 ? X1 : ch.D[t] - ch.A[t] * X0 - ch.C[t] * X1;
+    }
+    __syncthreads();
+    for (int lq = 0; lq < nl; lq++) {
+        const long gb = base + (long)lq * d.sy;
+        for (int x = tid; x < n; x += NT) L.S[gb + x] = tS[lq * w + x];
+    }
+}
+
 
 #ifdef PLLC_F64_TU
 PLLC_INST(double)
@@ -1364,6 +1678,16 @@ static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T
     return L;
 }
 
+// chunks per line of the partitioned solve for lines of nmin..nmax rows
+// (every chunk between 3 and PM rows), or 0 when the lines do not allow it
+static inline int part_chunks(int nmin, int nmax)
+{
+    if (nmin < 3) return 0;
+    const int P = (nmax + PM - 1) / PM;
+    if (P > 64 || nmin / P < 3) return 0;
+    return P;
+}
+
 // y, z then x sweeps of up to three fields at once (one launch per direction)
 template <typename T>
 static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* const* S, T* const* CC,
@@ -1383,17 +1707,46 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
                                      : (long)(R.ie - R.is + 1) * (R.je - R.js + 1);
             lines = nl > lines ? nl : lines;
         }
+        int nmax = 0, nmin = 1 << 30;
+        for (int q = 0; q < nset; q++) {
+            const Rng& R = ls.s[q].R;
+            const int n = DIR == 0 ? R.ie - R.is + 1 : DIR == 1 ? R.je - R.js + 1 : R.ke - R.ks + 1;
+            nmax = std::max(nmax, n);
+            nmin = std::min(nmin, n);
+        }
+        const int P = c.line_solver == 0 ? part_chunks(nmin, nmax) : 0;
+        if (P > 0 && DIR != 2) {
+            const int LPB = std::min(64, std::max(1, 256 / P));
+            const dim3 blk(LPB, P, 1), grd((unsigned)((lines + LPB - 1) / LPB), nset, 1);
+            const size_t sm = (size_t)6 * P * LPB * sizeof(T);
+            if (DIR == 0)
+                PLLC_L(c, nm[0], (k_sweep_part<T, 0><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+            else
+                PLLC_L(c, nm[1], (k_sweep_part<T, 1><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+            continue;
+        }
+        int nj = 0, ni = 0;
+        for (int q = 0; q < nset; q++) {
+            const Rng& R = ls.s[q].R;
+            nj = std::max(nj, R.je - R.js + 1);
+            ni = std::max(ni, R.ie - R.is + 1);
+        }
+        if (P > 0) {
+            // z-lines: LPB lines of an i-plane per block, tiles of pitch w
+            const int w = (nmax + 1) | 1;
+            int LPB = 32;
+            while (LPB > 1 && ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 40960) LPB /= 2;
+            while (LPB > 1 && LPB * P > 256) LPB /= 2;
+            const size_t sm = ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T);
+            const dim3 blk(LPB, P, 1), gz((nj + LPB - 1) / LPB, ni, nset);
+            PLLC_L(c, nm[2], (k_sweep_part_z<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w)));
+            continue;
+        }
         const dim3 grd((unsigned)((lines + 255) / 256), nset, 1);
         if (DIR == 0) PLLC_L(c, nm[0], (k_sweep<T, 0><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
         else if (DIR == 1) PLLC_L(c, nm[1], (k_sweep<T, 1><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
         else {
             // z-lines: one block per 64 lines of an i-plane
-            int nj = 0, ni = 0;
-            for (int q = 0; q < nset; q++) {
-                const Rng& R = ls.s[q].R;
-                nj = std::max(nj, R.je - R.js + 1);
-                ni = std::max(ni, R.ie - R.is + 1);
-            }
             const dim3 gz((nj + ZL - 1) / ZL, ni, nset);
             PLLC_L(c, nm[2], (k_sweep_z_lds<T><<<gz, ZL, 0, c.st>>>(c.d, ls, c.scal, g, r)));
         }

@@ -261,6 +261,12 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
     const size_t ts = sizeof(T);
     if (!get((void**)&c.P, npk * ts) || !get((void**)&c.Q, npk * ts) || !get((void**)&c.QC, npk * ts)) return false;
     if (!get((void**)&c.stale, 16 * ts) || !get((void**)&c.scal, S_SIZE * 8) || !get((void**)&c.tval, 8)) return false;
+    {
+        // PLLC_LINE_SOLVER=thomas selects the one-thread-per-line ADI
+        // solver with the reference's Thomas arithmetic
+        const char* ls = getenv("PLLC_LINE_SOLVER");
+        c.line_solver = (ls && strcmp(ls, "thomas") == 0) ? 1 : 0;
+    }
     c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
     c.npart = std::max(c.npart, (long)(2 * 1056 + 2 * (((long)d.ny * d.nz + 255) / 256) + 64));
     if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;

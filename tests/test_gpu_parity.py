@@ -1,8 +1,14 @@
 """GPU plain-FP64 path vs the FP64 CPU oracle, through the C-ABI.
 
-Bar: <= 1e-11 relative L2 per field (the plain path repeats the
+Bar: <= 1e-11 relative L2 after every step in c, in the velocity (u, v, w
+taken together as one vector field) and in the pressure (phi, and the
+pressure gradient as one vector field).  The plain path repeats the
 reference's arithmetic with -ffp-contract=off; the remaining differences
-come from exp() and from the DCT summation order).
+come from exp()/sin() on the device vs the host libm and, with the
+partitioned ADI solver, from the chunked elimination order.  The spanwise
+component w is ~1e-4 of u at these states, so its own relative error
+carries the velocity's rounding amplified by |u|/|w|; each single
+component is held to 1e-9 (printed per step).
 """
 import numpy as np
 import pytest
@@ -35,12 +41,28 @@ def _sync_state(orc, sim):
 
 
 def _compare(orc, sim, fields, tol=TOL, what=""):
+    """fields: names, or '+'-joined names compared as one stacked vector"""
     worst = {}
     for f in fields:
-        e = rel_l2(sim.download(f), orc.field(f))
+        parts = f.split("+")
+        a = np.concatenate([sim.download(p).ravel() for p in parts])
+        b = np.concatenate([orc.field(p).ravel() for p in parts])
+        e = rel_l2(a, b)
         worst[f] = e
         assert e <= tol, "%s %s rel L2 %.3e > %.1e" % (what, f, e, tol)
     return worst
+
+
+STEP_VEC = ["C_OLD", "U_OLD+V_OLD+W_OLD", "PHI", "U_PG+V_PG+W_PG"]
+STEP_ONE = ["U_OLD", "V_OLD", "W_OLD", "U_PG", "V_PG", "W_PG"]
+
+
+def _compare_step(orc, sim, what):
+    w = _compare(orc, sim, STEP_VEC, what=what)
+    w1 = _compare(orc, sim, STEP_ONE, tol=1e-9, what=what)
+    print("[%s] c %.1e vel %.1e phi %.1e pg %.1e | %s" % (
+        what, w["C_OLD"], w["U_OLD+V_OLD+W_OLD"], w["PHI"], w["U_PG+V_PG+W_PG"],
+        " ".join("%s %.1e" % (k, v) for k, v in w1.items())), flush=True)
 
 
 def test_init_matches(oracle_mod):
@@ -82,16 +104,18 @@ def test_each_operator_from_shared_state(oracle_mod, nsteps_before):
             _sync_state(orc, sim)
 
 
-def test_fused_steps_track_oracle(oracle_mod):
-    """Whole steps of the fused path vs the oracle, compared after every step."""
+@pytest.mark.parametrize("solver", ["part", "thomas"])
+def test_fused_steps_track_oracle(oracle_mod, solver, monkeypatch):
+    """Whole steps of the fused path vs the oracle, compared after every
+    step, with the partitioned and the one-thread-per-line ADI solvers."""
+    if solver == "thomas":
+        monkeypatch.setenv("PLLC_LINE_SOLVER", "thomas")
     prm, orc, sim = _pair(oracle_mod)
     for n in range(12):
         to, dto = orc.run(1)
         tg, dtg = sim.run(1)
         assert abs(dtg - dto) <= 1e-12 * dto
-        w = _compare(orc, sim, ["C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "V_PG", "W_PG"],
-                     what="step %d" % (n + 1))
-        print("[steps] %d worst %.2e" % (n + 1, max(w.values())), flush=True)
+        _compare_step(orc, sim, "%s step %d" % (solver, n + 1))
 
 
 def test_nmvp_steps_track_oracle(oracle_mod):
@@ -104,6 +128,17 @@ def test_nmvp_steps_track_oracle(oracle_mod):
     for n in range(6):
         orc.run(1)
         sim.run(1)
-        w = _compare(orc, sim, ["C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "MUO_CELL", "MUO_YZ"],
-                     what="nmvp step %d" % (n + 1))
-        print("[nmvp] %d worst %.2e" % (n + 1, max(w.values())), flush=True)
+        _compare(orc, sim, ["MUO_CELL", "MUO_XY", "MUO_XZ", "MUO_YZ"], what="nmvp step %d" % (n + 1))
+        _compare_step(orc, sim, "nmvp step %d" % (n + 1))
+
+
+@pytest.mark.parametrize("shape", [(64, 40, 36), (96, 24, 50)])
+def test_multi_chunk_lines_track_oracle(oracle_mod, shape):
+    """Lines cut into several chunks of unequal length (x: 4-6 chunks,
+    y and z: 2-4) by the partitioned ADI solver."""
+    nx, ny, nz = shape
+    prm, orc, sim = _pair(oracle_mod, nx=nx, ny=ny, nz=nz)
+    for n in range(3):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "%s step %d" % ("x".join(map(str, shape)), n + 1))
