@@ -174,6 +174,8 @@ void print_to_file(double *y, Grid *grid, int fno);                    /* decl:3
 void midfield(int, int, double, double, double, double, Grid *, Conc *, Velo *, Velo *,
               Velo *);                                                 /* decl:35 pllcart3d.c:3121 */
 void find_max(Grid *, Velo *, Velo *, Velo *, Conc *);                 /* decl:41 pllcart3d.c:3539 */
+void shift_conc(Grid *grid, Conc *c);                                   /* decl:31 pllcart3d.c:3231 */
+void spline(double *x, double *xx, double *y, double *yy, Grid *grid); /* decl:32 pllcart3d.c:3280 */
 void find_wmax(double tval, Grid *grid, Velo *w);                      /* decl:44 pllcart3d.c:3606 */
 void conc_average(Grid *grid, double ***cn, double tval);              /* decl:45 pllcart3d.c:3635 */
 
@@ -263,6 +265,9 @@ double pllc_sim_op(pllc_sim *s, int op, double dt, int rk, double vbulk);
    vbulk and tval; tval/dt (nullable) receive the values after the last step */
 int pllc_sim_run(pllc_sim *s, int nsteps, double *tval, double *dt);
 double pllc_sim_timestep(const pllc_sim *s);
+/* the wave perturbation (shift_conc) on c_old with Ampli, Lamb, Subharm
+ * of param.input */
+int pllc_sim_shift_conc(pllc_sim *s, double Ampli, double Lamb, int Subharm);
 /* slope parameter cm found at set-up when p->Nmvp == 1 (slope_parameter) */
 double pllc_sim_slope_parameter(const pllc_sim *s);
 int pllc_sim_synchronize(pllc_sim *s);

@@ -1273,3 +1273,68 @@ double orc_run(orc_sim *s, int nsteps, double *tval)
     if (tval) *tval = t;
     return dt;
 }
+
+/* ------------------------------------------------- wave perturbation */
+
+/* spline (pllcart3d.c:3280-3344): natural cubic spline of (x, y) on a
+ * uniform grid of spacing delta, evaluated at xx.  The second derivatives
+ * solve the (1,4,1) system of the interior points with pll_tridiag; each
+ * xx takes the last interval [x_i, x_i+1] containing it (the search does
+ * not stop at the first hit), then xx > x[N-1] takes y[N-1] and xx < x[1]
+ * takes y[0]. */
+void orc_spline(int N, double delta, const double *x, const double *xx, const double *y, double *yy)
+{
+    double *fpp = zalloc((size_t)N), *a = zalloc((size_t)N), *b = zalloc((size_t)N);
+    double *c = zalloc((size_t)N), *r = zalloc((size_t)N);
+    int i, ii;
+    for (i = 1; i < N - 1; i++) {
+        a[i - 1] = 1.0;
+        b[i - 1] = 4.0;
+        c[i - 1] = 1.0;
+        r[i - 1] = 6.0 / delta / delta * (y[i + 1] - 2.0 * y[i] + y[i - 1]);
+    }
+    fpp[0] = 0;
+    fpp[N - 1] = 0;
+    pll_line(N - 2, a, b, c, r);
+    for (i = 1; i < N - 1; i++) fpp[i] = r[i - 1];
+    for (ii = 0; ii < N; ii++) {
+        for (i = 0; i < N - 1; i++) {
+            if (x[i] <= xx[ii] && xx[ii] <= x[i + 1]) {
+                yy[ii] = fpp[i] * (x[i + 1] - xx[ii]) * (x[i + 1] - xx[ii]) * (x[i + 1] - xx[ii]) / (6 * delta)
+                       + fpp[i + 1] * (xx[ii] - x[i]) * (xx[ii] - x[i]) * (xx[ii] - x[i]) / (6 * delta)
+                       + (y[i] / delta - delta / 6 * fpp[i]) * (x[i + 1] - xx[ii])
+                       + (y[i + 1] / delta - delta / 6 * fpp[i + 1]) * (xx[ii] - x[i]);
+            }
+        }
+        if (xx[ii] > x[N - 1]) yy[ii] = y[N - 1];
+        else if (xx[ii] < x[1]) yy[ii] = y[0];
+    }
+    free(fpp); free(a); free(b); free(c); free(r);
+}
+
+/* shift_conc (pllcart3d.c:3231-3277): c_old on every line (j, k), ghosts
+ * included, re-sampled at x - meshdx(z) by spline; yy persists across
+ * lines as in the reference (every point is written on every line). */
+void orc_shift_conc(orc_sim *s, double Ampli, double Lamb, int Subharm)
+{
+    const int N = s->NX;
+    double *x = zalloc((size_t)N), *xx = zalloc((size_t)N), *y = zalloc((size_t)N), *yy = zalloc((size_t)N);
+    double *c = s->f[ORC_C_OLD], beta, meshdx = 0.0;
+    int i, j, k;
+    for (k = 0; k < s->NZ; k++) {
+        beta = 2 * s->PI / Lamb;
+        if (Subharm == 0) meshdx = Ampli * cos(beta * s->Zc[k]);
+        else if (Subharm == 1)
+            meshdx = Ampli * cos(beta * (s->Zc[k] + Lamb / 2)) + Ampli * cos(beta / 2 * (s->Zc[k] + Lamb / 2));
+        for (j = 0; j < s->NY; j++) {
+            for (i = 0; i < N; i++) {
+                y[i] = c[IX(s, i, j, k)];
+                x[i] = s->Xc[i];
+                xx[i] = x[i] - meshdx;
+            }
+            orc_spline(N, s->dx, x, xx, y, yy);
+            for (i = 0; i < N; i++) c[IX(s, i, j, k)] = yy[i];
+        }
+    }
+    free(x); free(xx); free(y); free(yy);
+}

@@ -76,6 +76,10 @@ double orc_op(orc_sim *s, int op, double dt, int rk, double vbulk);
 /* Whole steps of the while loop (pllcart3d.c:156-196).  Returns last dt. */
 double orc_run(orc_sim *s, int nsteps, double *tval);
 void orc_set_threads(int n);
+/* wave perturbation: shift_conc (pllcart3d.c:3231) on c_old, and spline
+ * (3280) on one line of N points of spacing delta */
+void orc_shift_conc(orc_sim *s, double Ampli, double Lamb, int Subharm);
+void orc_spline(int N, double delta, const double *x, const double *xx, const double *y, double *yy);
 
 /* ---- unit kernels ---- */
 double orc_weno(const double *d1, int kk, int sign);

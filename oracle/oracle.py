@@ -58,6 +58,8 @@ def load():
         L.orc_run.restype = C.c_double
         L.orc_run.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_shift_conc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.orc_spline.argtypes = [C.c_int, C.c_double, dp, dp, dp, dp]
         L.orc_weno.restype = C.c_double
         L.orc_weno.argtypes = [dp, C.c_int, C.c_int]
         L.orc_tridiag_1rhs.argtypes = [C.c_int, dp, dp, dp, dp]
@@ -107,6 +109,9 @@ class Oracle:
         t = C.c_double(tval)
         dt = self.L.orc_run(self.h, nsteps, C.byref(t))
         return t.value, dt
+
+    def shift_conc(self, ampli, lamb, subharm=0):
+        self.L.orc_shift_conc(self.h, ampli, lamb, subharm)
 
     @property
     def timestep(self):

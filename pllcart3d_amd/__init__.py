@@ -98,6 +98,10 @@ class Sim:
     def timestep(self):
         return self.L.pllc_sim_timestep(self.h)
 
+    def shift_conc(self, ampli, lamb, subharm=0):
+        """The wave perturbation (shift_conc) on c_old."""
+        self._chk(self.L.pllc_sim_shift_conc(self.h, ampli, lamb, subharm), "pllc_sim_shift_conc")
+
     @property
     def cm(self):
         """Slope parameter of the non-monotonic viscosity law (Nmvp = 1)."""

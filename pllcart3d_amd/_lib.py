@@ -67,6 +67,7 @@ def load():
     L.pllc_sim_run.argtypes = [vp, C.c_int, dp, dp]
     L.pllc_sim_timestep.restype = C.c_double
     L.pllc_sim_timestep.argtypes = [vp]
+    L.pllc_sim_shift_conc.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.pllc_sim_slope_parameter.restype = C.c_double
     L.pllc_sim_slope_parameter.argtypes = [vp]
     L.pllc_sim_synchronize.argtypes = [vp]

@@ -6,8 +6,9 @@
  * initial condition (Initial = 1) or the restart files (Initial = 0), then
  * the RK3-CN loop until Timemax with the reference's output cadence:
  * conc_average every 0.1 time units, find_max every Niter_print steps,
- * midfield every Out_mid, write_to_file every Out_inter.  The wave
- * perturbation (Wavpert, 242-256) is not part of this build.
+ * midfield every Out_mid, write_to_file every Out_inter, and with Wavpert
+ * the wave perturbation once tval reaches 2 (242-256) and find_wmax on
+ * restarted runs (203-205).
  *
  * usage: run3_amd [nsteps [dumpfile]]
  *   nsteps > 0 runs that many steps instead of stopping at Timemax;
@@ -41,6 +42,7 @@ int main(int argc, char *argv[])
     int tstep = 0, rkstep, fileno = 0, nsteps = argc > 1 ? atoi(argv[1]) : 0;
     double del_min, diff_dt, tval = 0.0, cfl_dt, dt = 0.0, rk_gamma, rk_rho, vbulk = 0.0, t1, t2;
     double outtime = 0.0, outmid = 0.0, cavgtime = 0.0, cm = 0.0;
+    int shift_conc_key = 0;
     Velo *u, *v, *w;
     Conc *c;
     Visc *mu_old, *mu_new;
@@ -51,10 +53,6 @@ int main(int argc, char *argv[])
     t1 = wall();
     input();
     check("input");
-    if (Wavpert != 0) {
-        fprintf(stderr, "run3_amd: Wavpert = 0 is required\n");
-        return 1;
-    }
     grid = (Grid *)malloc(sizeof(Grid));
     grid_setup(grid);
     check("grid_setup");
@@ -158,6 +156,7 @@ int main(int argc, char *argv[])
             conc_average(grid, c->new, tval);
             cavgtime = cavgtime + 0.1;
         }
+        if (Initial == 0 && Wavpert == 1) find_wmax(tval, grid, w);
         if ((Niter_print > 0) && (tstep % Niter_print) == 0) {
             fp = fopen("Outscreen", "a");
             if (fp) { fprintf(fp, "t = %f\n", tval); fclose(fp); }
@@ -184,6 +183,14 @@ int main(int argc, char *argv[])
             if (fp) { fprintf(fp, "%d %f\n", fileno, outtime); fclose(fp); }
             fileno = fileno + 1;
             outtime = outtime + Out_inter;
+        }
+        if (tval >= 2 && shift_conc_key == 0 && Wavpert == 1) {
+            fp = fopen("Outscreen", "a");
+            if (fp) { fprintf(fp, "\n\n=====> ADD WAVE PERTURBATION <=====\n\n"); fclose(fp); }
+            shift_conc(grid, c);
+            fp = fopen("wmax.bin", "wb");
+            if (fp) fclose(fp);
+            shift_conc_key = 1;
         }
         check("time step");
     }

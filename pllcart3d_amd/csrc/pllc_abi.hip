@@ -1149,3 +1149,52 @@ extern "C" void print_to_file(double* y, Grid* grid, int fno)
     }
     fclose(fp);
 }
+
+// ---------------------------------------------------- wave perturbation
+
+// spline (3280-3344): one line of grid->nx_max points on the device
+extern "C" void spline(double* x, double* xx, double* y, double* yy, Grid* grid)
+{
+    const char* W = "spline";
+    DropIn& d = din();
+    if (!d.have_stream) return;
+    const int N = grid->nx_max;
+    if (N < 4) { pllc_set_error(W, "nx_max < 4"); return; }
+    DevBuf buf;
+    if (!buf.get((size_t)5 * N, W)) return;
+    double *dx_ = buf.p, *dxx = buf.p + N, *dy = buf.p + 2 * N, *cp = buf.p + 3 * N, *fp = buf.p + 4 * N;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(dx_, x, (size_t)N * 8, hipMemcpyHostToDevice, d.st))) return;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(dxx, xx, (size_t)N * 8, hipMemcpyHostToDevice, d.st))) return;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(dy, y, (size_t)N * 8, hipMemcpyHostToDevice, d.st))) return;
+    pllc::spline_line(N, grid->dx, dx_, dxx, dy, cp, fp, d.st);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(yy, dy, (size_t)N * 8, hipMemcpyDeviceToHost, d.st))) return;
+    pllc_hip_ok(W, hipStreamSynchronize(d.st));
+}
+
+// shift_conc (3231-3277): c->old on every (j,k) line, ghosts included,
+// re-sampled at Xc - meshdx(Zc[k]); meshdx as in 3245-3251
+extern "C" void shift_conc(Grid* grid, Conc* c)
+{
+    const char* W = "shift_conc";
+    pllc::Ctx<double>* cx = ctx_for(grid, W);
+    double* q = dev_of(c->old, W);
+    if (!cx || !q) return;
+    const int NX = grid->nx_max, NZ = grid->nz_max;
+    std::vector<double> h((size_t)NX + NZ);
+    for (int i = 0; i < NX; i++) h[i] = grid->Xc[i];
+    for (int k = 0; k < NZ; k++) {
+        const double beta = 2 * PI / Lamb;
+        double meshdx = 0.0;
+        if (Subharm == 0) meshdx = Ampli * cos(beta * grid->Zc[k]);
+        else if (Subharm == 1)
+            meshdx = Ampli * cos(beta * (grid->Zc[k] + Lamb / 2)) + Ampli * cos(beta / 2 * (grid->Zc[k] + Lamb / 2));
+        h[(size_t)NX + k] = meshdx;
+    }
+    DevBuf buf;
+    if (!buf.get(h.size(), W)) return;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(buf.p, h.data(), h.size() * 8, hipMemcpyHostToDevice, cx->st))) return;
+    pllc::Ops<double>::shift_conc(*cx, q, buf.p, buf.p + NX);
+    if (!done(W)) return;
+    // the buffer is freed at scope end: wait for the kernel first
+    pllc_hip_ok(W, hipStreamSynchronize(cx->st));
+}

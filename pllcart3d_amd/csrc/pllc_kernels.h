@@ -78,6 +78,9 @@ struct Ops {
     static void cfl(Ctx<T>& c);
     static void bulk(Ctx<T>& c, int slot, double alpha, bool advance_t);
     static void copy(Ctx<T>& c, const T* a, T* b, long n);
+    // shift_conc on field q: every (j,k) line re-sampled at Xc - shift[k]
+    // (x: the NX cell centres, shift: NZ values, both on the device)
+    static void shift_conc(Ctx<T>& c, T* q, const double* x, const double* shift);
     static void fill(Ctx<T>& c, T* a, long n, T v);
     static void set_scalar(Ctx<T>& c, int idx, double v);
     // dense 2D DCT of a packed block: REDFT10 (forward) or REDFT01
@@ -86,6 +89,10 @@ struct Ops {
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)
+// spline (pllcart3d.c:3280) of one line of N points on the device, in place
+// on y; cp and fp are N-value scratch
+void spline_line(int N, double delta, const double* x, const double* xx, double* y, double* cp, double* fp,
+                 hipStream_t st);
 void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
                   hipStream_t st);
 void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int n, hipStream_t st);

@@ -1291,6 +1291,83 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
 
+// ------------------------------------------------- wave perturbation
+//
+// spline (pllcart3d.c:3280-3344) on lines of N points, in place: the
+// natural spline's second derivatives from the (1,4,1) system of the
+// interior points, solved with pll_tridiag's one-rank arithmetic
+// (pll_tridiag.c:21-47), then evaluation at xx (or x - shift[k]) on the last
+// interval holding it, with the reference's end rules (xx > x[N-1] takes
+// y[N-1], xx < x[1] takes y[0]).  Line l starts at (l / nk) sj + (l % nk) sk,
+// rows are rs apart; fp and cp are scratch with the same layout (cp also
+// holds the new values until the line is rewritten).  One thread per line.
+template <typename T>
+__global__ void k_spline_lines(int N, long nlines, int nk, long sj, long sk, long rs, T delta,
+                               const double* __restrict__ x, const double* __restrict__ xx,
+                               const double* __restrict__ shift, T* __restrict__ y, T* __restrict__ cp,
+                               T* __restrict__ fp)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    const long o = (l / nk) * sj + (l % nk) * sk;
+    T* __restrict__ Y = y + o;
+    T* __restrict__ C = cp + o;
+    T* __restrict__ F = fp + o;
+    const T zero = T(0.0), one = T(1.0), four = T(4.0);
+    const int n = N - 2;
+    auto rhs = [&](int i) {
+        return T(6.0) / delta / delta * (Y[(long)(i + 1) * rs] - T(2.0) * Y[(long)i * rs] + Y[(long)(i - 1) * rs]);
+    };
+    // rows q = 0..n-1 (point i = q + 1) kept at line position i
+    T b = four - zero * one;
+    T c = one / b;
+    T r = (rhs(1) - zero * one) / b;
+    C[rs] = c;
+    F[rs] = r;
+    for (int q = 1; q < n; q++) {
+        const long p = (long)(q + 1) * rs;
+        b = (four - c * one);
+        c = one / b;
+        r = (rhs(q + 1) - r * one) / b;
+        C[p] = c;
+        F[p] = r;
+    }
+    r = r - c * zero;
+    F[(long)n * rs] = r;
+    for (int q = n - 2; q >= 0; q--) {
+        const long p = (long)(q + 1) * rs;
+        r = F[p] - C[p] * r;
+        F[p] = r;
+    }
+    const T sh = shift ? T(shift[l % nk]) : zero;
+    const T xl = T(x[N - 1]), x1 = T(x[1]);
+    for (int ii = 0; ii < N; ii++) {
+        const T xv = xx ? T(xx[ii]) : T(x[ii]) - sh;
+        T v;
+        if (xv > xl) v = Y[(long)(N - 1) * rs];
+        else if (xv < x1) v = Y[0];
+        else {
+            // the largest i <= N-2 with x[i] <= xv (the reference's last hit)
+            int lo = 1, hi = N - 2;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (T(x[mid]) <= xv) lo = mid;
+                else hi = mid - 1;
+            }
+            const int i = lo;
+            const T xi = T(x[i]), xi1 = T(x[i + 1]);
+            const T fi = F[(long)i * rs];
+            const T fi1 = (i + 1 == N - 1) ? zero : F[(long)(i + 1) * rs];
+            v = fi * (xi1 - xv) * (xi1 - xv) * (xi1 - xv) / (T(6) * delta)
+              + fi1 * (xv - xi) * (xv - xi) * (xv - xi) / (T(6) * delta)
+              + (Y[(long)i * rs] / delta - delta / T(6) * fi) * (xi1 - xv)
+              + (Y[(long)(i + 1) * rs] / delta - delta / T(6) * fi1) * (xv - xi);
+        }
+        C[(long)ii * rs] = v;
+    }
+    for (int ii = 0; ii < N; ii++) Y[(long)ii * rs] = C[(long)ii * rs];
+}
+
 // ------------------------------------------------ partitioned line solve
 //
 // The same systems as k_sweep, cut into P chunks of at most PM rows per
@@ -1939,6 +2016,14 @@ void Ops<T>::set_scalar(Ctx<T>& c, int idx, double v)
     k_set_scalar<T><<<1, 64, 0, c.st>>>(c.scal, idx, v);
 }
 
+template <typename T>
+void Ops<T>::shift_conc(Ctx<T>& c, T* q, const double* x, const double* shift)
+{
+    const long nl = (long)c.d.NY * c.d.NZ;
+    k_spline_lines<T><<<(unsigned)((nl + 127) / 128), 128, 0, c.st>>>(c.d.NX, nl, c.d.NZ, c.d.sy, 1, c.d.sx, T(c.d.dx),
+                                                                      x, nullptr, shift, q, c.f[F_CC], c.f[F_SRC]);
+}
+
 #ifdef PLLC_F64_TU
 template struct Ops<double>;
 #else
@@ -1946,6 +2031,12 @@ template struct Ops<float>;
 #endif
 
 #ifdef PLLC_F64_TU
+void spline_line(int N, double delta, const double* x, const double* xx, double* y, double* cp, double* fp,
+                 hipStream_t st)
+{
+    k_spline_lines<double><<<1, 64, 0, st>>>(N, 1, 1, 0, 0, 1, delta, x, xx, nullptr, y, cp, fp);
+}
+
 void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
                   hipStream_t st)
 {

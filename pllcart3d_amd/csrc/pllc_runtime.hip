@@ -587,6 +587,42 @@ static double op_t(pllc_sim* s, Ctx<T>& c, int op, double dt, int rk, double vbu
     return finish(s, "pllc_sim_op") ? NAN : 0.0;
 }
 
+// the wave perturbation of the reference's loop (shift_conc, pllcart3d.c
+// 242-255 and 3231-3277) on c_old
+template <typename T>
+static int shift_t(pllc_sim* s, Ctx<T>& c, double Ampli, double Lamb, int Subharm)
+{
+    const char* W = "pllc_sim_shift_conc";
+    normalize<T>(s, c);
+    const Geo g = make_geo(&s->p);
+    const double PIv = 4.0 * atan(1.0);
+    std::vector<double> h((size_t)g.NX + g.NZ);
+    for (int i = 0; i < g.NX; i++) h[i] = g.Xc[i];
+    for (int k = 0; k < g.NZ; k++) {
+        const double beta = 2 * PIv / Lamb;
+        double meshdx = 0.0;
+        if (Subharm == 0) meshdx = Ampli * cos(beta * g.Zc[k]);
+        else if (Subharm == 1)
+            meshdx = Ampli * cos(beta * (g.Zc[k] + Lamb / 2)) + Ampli * cos(beta / 2 * (g.Zc[k] + Lamb / 2));
+        h[(size_t)g.NX + k] = meshdx;
+    }
+    double* dev = nullptr;
+    if (!pllc_hip_ok(W, hipMalloc((void**)&dev, h.size() * 8))) return -1;
+    int rc = -1;
+    if (pllc_hip_ok(W, hipMemcpyAsync(dev, h.data(), h.size() * 8, hipMemcpyHostToDevice, s->st))) {
+        Ops<T>::shift_conc(c, c.f[C_OLD], dev, dev + g.NX);
+        rc = finish(s, W);
+    }
+    if (!pllc_hip_ok(W, hipStreamSynchronize(s->st))) rc = -1;
+    if (!pllc_hip_ok(W, hipFree(dev))) rc = -1;
+    return rc;
+}
+
+extern "C" int pllc_sim_shift_conc(pllc_sim* s, double Ampli, double Lamb, int Subharm)
+{
+    return s->prec ? shift_t<float>(s, s->cf, Ampli, Lamb, Subharm) : shift_t<double>(s, s->cd, Ampli, Lamb, Subharm);
+}
+
 extern "C" double pllc_sim_op(pllc_sim* s, int op, double dt, int rk, double vbulk)
 {
     if (rk < 0 || rk > 2 || op < 0 || op >= PLLC_NOPS) { pllc_set_error("pllc_sim_op", "bad op or rk"); return NAN; }

@@ -116,3 +116,33 @@ def test_driver_outputs_and_restart(tmp_path, oracle_mod):
     assert (c / "caf0000R000.bin").exists() and (c / "caf0001R000.bin").exists()
     out = (c / "Outscreen").read_text()
     assert out.count("max\n") == 5 and "timetaken" in out
+
+
+
+def test_spline_entry_point_matches_oracle(oracle_mod):
+    """spline() of the drop-in on host arrays (one line, evaluated on the
+    device) vs the oracle's restatement: bitwise."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(os.path.dirname(P.__file__), "libpllcart3d_amd.so"))
+    O = oracle_mod.load()
+    dpt = C.POINTER(C.c_double)
+
+    class Grid(C.Structure):   # structures_3d.h Grid_data (include/pllcart3d.h)
+        _fields_ = [(n, C.c_int) for n in ("istart", "iend", "jstart", "jend", "kstart", "kend",
+                                           "nx_cell", "ny_cell", "nz_cell", "nx_max", "ny_max", "nz_max",
+                                           "nmax_xyz", "nx_global_offset")] + \
+                   [(n, C.c_double) for n in ("dx", "dy", "dz")] + [(n, dpt) for n in ("X", "Y", "Z", "Xc", "Yc", "Zc")]
+
+    N, dx = 36, 0.125
+    g = Grid()
+    g.nx_max, g.dx = N, dx
+    x = (np.arange(N) - 2.5) * dx
+    rng = np.random.default_rng(11)
+    y = np.cos(1.7 * x) + 0.05 * rng.normal(size=N)
+    for shift in (0.04, -0.09, 0.0):
+        xx = x - shift
+        got = np.zeros(N)
+        L.spline(oracle_mod.dp(x), oracle_mod.dp(xx), oracle_mod.dp(y), oracle_mod.dp(got), C.byref(g))
+        ref = np.zeros(N)
+        O.orc_spline(N, dx, oracle_mod.dp(x), oracle_mod.dp(xx), oracle_mod.dp(y), oracle_mod.dp(ref))
+        assert np.array_equal(got, ref), (shift, np.abs(got - ref).max())

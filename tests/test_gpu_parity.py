@@ -142,3 +142,21 @@ def test_multi_chunk_lines_track_oracle(oracle_mod, shape):
         orc.run(1)
         sim.run(1)
         _compare_step(orc, sim, "%s step %d" % ("x".join(map(str, shape)), n + 1))
+
+
+@pytest.mark.parametrize("subharm", [0, 1])
+def test_shift_conc_matches_oracle(oracle_mod, subharm):
+    """The wave perturbation (shift_conc + spline) after a few steps, then
+    more steps from the shifted state."""
+    prm, orc, sim = _pair(oracle_mod)
+    orc.run(2)
+    sim.run(2)
+    orc.shift_conc(0.1, 0.7, subharm)
+    sim.shift_conc(0.1, 0.7, subharm)
+    e = rel_l2(sim.download("C_OLD"), orc.field("C_OLD"))
+    print("[shift] subharm %d c_old rel L2 %.2e" % (subharm, e), flush=True)
+    assert e <= 1e-14
+    for n in range(2):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "shift step %d" % (n + 1))

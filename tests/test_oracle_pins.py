@@ -457,3 +457,70 @@ def test_slope_parameter_matches_analytic_root(O, M, Mum, Lam):
     mu = mum * np.sin(p0 + (p1 - p0) * (1 + a) * cc / (1 + a * cc))
     assert abs(cc[np.argmax(mu)] - (1 - cm)) < 1e-4
     assert abs(mu.max() - mum) < 1e-9 * mum
+
+
+# ------------------------------------------------------ wave perturbation
+
+def test_spline_is_the_natural_cubic_spline(O):
+    """spline: scipy's natural cubic spline inside [x1, x_{N-1}], y[0]
+    below x1 (also on [x0, x1)) and y[N-1] beyond the last point; linear data
+    is reproduced exactly."""
+    from scipy.interpolate import CubicSpline
+    L = O.load()
+    N, dx = 40, 0.1
+    x = (np.arange(N) - 2.5) * dx
+    rng = np.random.default_rng(9)
+    y = np.sin(2.0 * x) + 0.1 * rng.normal(size=N)
+    xx = np.concatenate([x - 0.037, x + 0.061, [x[0] + 0.5 * dx, x[-1] + 1e-3, x[0] - 1.0]])
+    # the routine evaluates N probes per call: feed them in chunks of N
+    out = []
+    for c0 in range(0, len(xx), N):
+        probe = np.full(N, x[5])
+        part = xx[c0:c0 + N]
+        probe[:len(part)] = part
+        res = np.zeros(N)
+        L.orc_spline(N, dx, dp(x), dp(probe), dp(y), dp(res))
+        out.append(res[:len(part)])
+    got = np.concatenate(out)
+    ref = CubicSpline(x, y, bc_type="natural")(xx)
+    inside = (xx >= x[1]) & (xx <= x[-1])
+    assert np.allclose(got[inside], ref[inside], rtol=0, atol=1e-12)
+    assert np.all(got[(xx < x[1])] == y[0])
+    assert np.all(got[(xx > x[-1])] == y[-1])
+    lin = 0.3 + 2.0 * x
+    res = np.zeros(N)
+    probe = x - 0.037
+    L.orc_spline(N, dx, dp(x), dp(probe), dp(lin), dp(res))
+    ok = probe >= x[1]
+    assert np.allclose(res[ok], 0.3 + 2.0 * probe[ok], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("subharm", [0, 1])
+def test_shift_conc_moves_each_line_by_meshdx(O, subharm):
+    """shift_conc: every (j, k) line of c_old, ghosts included, becomes its
+    natural spline sampled at x - meshdx(z_k), meshdx = A cos(2 pi z / L)
+    (+ the half-wavenumber subharmonic)."""
+    from scipy.interpolate import CubicSpline
+    nx, ny, nz = 20, 4, 6
+    prm = _params(nx, ny, nz)
+    o = O.Oracle(prm)
+    o.init()
+    rng = np.random.default_rng(4)
+    c0 = rng.uniform(0, 1, size=o.shape)
+    o.field("C_OLD")[...] = c0
+    A, lam = 0.13, 0.8
+    o.shift_conc(A, lam, subharm)
+    got = o.field("C_OLD")
+    dx, dz = 4.0 / nx, 1.0 / nz
+    xc = (np.arange(nx + 6) - 3 + 0.5) * dx
+    zc = -0.5 + (np.arange(nz + 2) - 1 + 0.5) * dz
+    for k in range(nz + 2):
+        md = A * np.cos(2 * np.pi / lam * zc[k])
+        if subharm:
+            md = A * np.cos(2 * np.pi / lam * (zc[k] + lam / 2)) + A * np.cos(np.pi / lam * (zc[k] + lam / 2))
+        xx = xc - md
+        for j in range(ny + 2):
+            ref = CubicSpline(xc, c0[:, j, k], bc_type="natural")(xx)
+            ref[xx > xc[-1]] = c0[-1, j, k]
+            ref[xx < xc[1]] = c0[0, j, k]
+            assert np.allclose(got[:, j, k], ref, rtol=0, atol=1e-12), (j, k)
