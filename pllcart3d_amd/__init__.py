@@ -98,6 +98,9 @@ class Sim:
     def timestep(self):
         return self.L.pllc_sim_timestep(self.h)
 
+    def synchronize(self):
+        self._chk(self.L.pllc_sim_synchronize(self.h), "pllc_sim_synchronize")
+
     def shift_conc(self, ampli, lamb, subharm=0):
         """The wave perturbation (shift_conc) on c_old."""
         self._chk(self.L.pllc_sim_shift_conc(self.h, ampli, lamb, subharm), "pllc_sim_shift_conc")

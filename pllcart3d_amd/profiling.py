@@ -1,0 +1,105 @@
+"""Profiling workload and rocprofv3 summaries for the pllcart3d time step.
+
+    python pllcart3d_amd/profiling.py run [--config cfg2] [--steps 2] [--precision plain|mixed]
+        set-up, one UPDATE_OLD (12 whole-field copies of known size: the
+        calibration of the HBM counters) and `steps` fused steps; run it
+        under rocprofv3 --kernel-trace --stats or --pmc FETCH_SIZE / WRITE_SIZE.
+
+    python pllcart3d_amd/profiling.py summary --trace T.db [--fetch F.db --write W.db]
+        a markdown summary: per-kernel durations from the kernel trace and the
+        HBM bytes per dispatch from the two counter passes, scaled by what the
+        same counters read for the copy kernel (one field read, one written).
+"""
+import argparse
+import collections
+import os
+import re
+import sqlite3
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _short(name):
+    s = re.sub(r"\(.*", "", name).replace("void ", "").replace("pllc::", "")
+    return s.strip()
+
+
+def run(args):
+    sys.path.insert(0, ROOT)
+    import pllcart3d_amd as P
+    from pllcart3d_amd.configs import CONFIGS, config_params
+    prm = config_params(args.config)
+    cfg = CONFIGS[args.config]
+    sim = P.Sim(prm, precision=args.precision)
+    sim.init(P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"]))
+    sim.op("UPDATE_OLD")
+    sim.run(args.steps)
+    sim.synchronize()
+    print("profile run: %s %s, %d steps, %d points per field" % (args.config, args.precision, args.steps, sim.n),
+          flush=True)
+
+
+def _counters(db, cname):
+    con = sqlite3.connect(db)
+    d = collections.defaultdict(list)
+    for name, v in con.execute("select kernel_name, value from counters_collection where counter_name=?",
+                               (cname,)):
+        d[_short(name)].append(v)
+    return d
+
+
+def summary(args):
+    out = []
+    con = sqlite3.connect(args.trace)
+    rows = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    out.append("| kernel | calls | total ms | avg us | % |")
+    out.append("|---|---:|---:|---:|---:|")
+    for name, calls, tot, avg, pct in sorted(rows, key=lambda r: -r[2]):
+        out.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (_short(name)[:60], calls, tot / 1e6, avg / 1e3, pct))
+    if args.fetch and args.write:
+        f = _counters(args.fetch, "FETCH_SIZE")
+        w = _counters(args.write, "WRITE_SIZE")
+        copy = [k for k in f if k.startswith("k_copy")]
+        if not copy:
+            raise SystemExit("no copy kernel in the counter passes")
+        ck = copy[0]
+        field_kib = args.points * args.bytes / 1024.0
+        cf = sum(f[ck]) / len(f[ck])
+        cw = sum(w[ck]) / len(w[ck])
+        sf, sw = field_kib / cf, field_kib / cw
+        out.append("")
+        out.append("Calibration: `%s` copies one field of %d points x %d B = %.0f KiB; per dispatch "
+                   "FETCH_SIZE reads %.0f KiB and WRITE_SIZE %.0f KiB, so the factors are %.3f (fetch) "
+                   "and %.3f (write)." % (ck, args.points, args.bytes, field_kib, cf, cw, sf, sw))
+        out.append("")
+        out.append("| kernel | dispatches | HBM read MB | HBM write MB | read+write fields |")
+        out.append("|---|---:|---:|---:|---:|")
+        for k in sorted(f, key=lambda k: -sum(f[k]) / len(f[k])):
+            rf = sum(f[k]) / len(f[k]) * sf
+            wf = (sum(w[k]) / len(w[k]) * sw) if k in w else float("nan")
+            out.append("| `%s` | %d | %.1f | %.1f | %.2f |" % (k[:60], len(f[k]), rf * 1024 / 1e6, wf * 1024 / 1e6,
+                                                               (rf + wf) / field_kib))
+    text = "\n".join(out) + "\n"
+    sys.stdout.write(text)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--config", default="cfg2")
+    r.add_argument("--steps", type=int, default=2)
+    r.add_argument("--precision", default="plain", choices=["plain", "mixed"])
+    s = sub.add_parser("summary")
+    s.add_argument("--trace", required=True)
+    s.add_argument("--fetch")
+    s.add_argument("--write")
+    s.add_argument("--points", type=int, default=262 * 130 * 130, help="points of one field (with ghosts)")
+    s.add_argument("--bytes", type=int, default=8, help="bytes per value")
+    a = ap.parse_args()
+    run(a) if a.cmd == "run" else summary(a)
+
+
+if __name__ == "__main__":
+    main()
