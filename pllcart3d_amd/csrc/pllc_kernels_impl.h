@@ -991,10 +991,10 @@ __global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ p
 // REDFT10 / REDFT01 as matrices):
 //   TRB = true : C[b] (M x N) = A[b] (M x K) * B^T, B stored N x K
 //   TRB = false: C[b] (M x N) = A[b] (M x K) * B[b], B stored K x N
-// 64x64 tile per block of 256 threads, 4x4 per thread, K chunks of 16.
-// Every output is accumulated over k in increasing order; for T = double
-// the product and the sum are rounded separately (no FMA), i.e. the direct
-// sums of the FFTW definitions evaluated left to right.
+// Every output is accumulated over k in increasing order with fused
+// multiply-adds.  FFTW's fast transforms round differently from any direct
+// sum, so the dense sums agree with the reference's REDFT10/01 to rounding,
+// as the oracle's direct sums do.
 template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
@@ -1038,8 +1038,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
             for (int p = 0; p < 4; p++)
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
-                    if (sizeof(T) == 8) acc[p][q] = acc[p][q] + av[p] * bv[q];
-                    else acc[p][q] = fma(av[p], bv[q], acc[p][q]);
+                    acc[p][q] = fma(av[p], bv[q], acc[p][q]);
                 }
         }
         __syncthreads();

@@ -319,7 +319,21 @@ struct pllc_sim {
     const char* tname = nullptr;
     std::vector<hipEvent_t> tev;
     size_t tn = 0;
+    // two fused steps captured as one HIP graph: six old/new swaps leave the
+    // pointer table as it was, so the graph replays while the table (gtab)
+    // and the launch parameters stay those of the capture.  PLLC_GRAPH=0
+    // keeps every step eager.
+    bool use_graph = true;
+    hipGraphExec_t gx = nullptr;
+    std::vector<void*> gtab;
 };
+
+static void graph_drop(pllc_sim* s)
+{
+    if (s->gx) (void)hipGraphExecDestroy(s->gx);
+    s->gx = nullptr;
+    s->gtab.clear();
+}
 
 template <typename T> static Ctx<T>& ctx(pllc_sim* s);
 template <> Ctx<double>& ctx<double>(pllc_sim* s) { return s->cd; }
@@ -377,6 +391,8 @@ extern "C" pllc_sim* pllc_sim_create2(const pllc_params* p, int precision)
          pllc_hip_ok("pllc_sim_create", hipEventCreate(&s->e1));
     if (!ok) { pllc_sim_destroy(s); return nullptr; }
     s->Timestep = p->Timestep;
+    const char* gr = getenv("PLLC_GRAPH");
+    s->use_graph = !(gr && strcmp(gr, "0") == 0);
     return s;
 }
 
@@ -386,6 +402,7 @@ extern "C" void pllc_sim_destroy(pllc_sim* s)
 {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
+    graph_drop(s);
     for (void* p : s->owned) (void)hipFree(p);
     for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
     if (s->e0) (void)hipEventDestroy(s->e0);
@@ -507,6 +524,7 @@ static int init_t(pllc_sim* s, Ctx<T>& c, const double* c_full)
 
 extern "C" int pllc_sim_init(pllc_sim* s, const double* c_full)
 {
+    graph_drop(s);
     return s->prec ? init_t<float>(s, s->cf, c_full) : init_t<double>(s, s->cd, c_full);
 }
 
@@ -666,11 +684,68 @@ static void step_any(pllc_sim* s)
     else fused_step<double>(s, s->cd);
 }
 
+template <typename T>
+static std::vector<void*> ptr_table(Ctx<T>& c)
+{
+    return std::vector<void*>(c.f, c.f + NFIELDS);
+}
+
+// two steps through the captured graph; false (nothing launched, pointer
+// table unchanged) when capture is not possible
+template <typename T>
+static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
+{
+    if (!s->use_graph || c.hook) return false;
+    std::vector<void*> tab = ptr_table(c);
+    if (s->gx && tab != s->gtab) graph_drop(s);
+    if (!s->gx) {
+        if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        fused_step<T>(s, c);
+        fused_step<T>(s, c);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->st, &g);
+        (void)hipGetLastError();
+        if (e != hipSuccess || !g) {
+            s->use_graph = false;
+            return false;   // six swaps: the table is back to tab
+        }
+        hipGraphExec_t gx = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            s->use_graph = false;
+            return false;
+        }
+        s->gx = gx;
+        s->gtab = tab;
+    }
+    if (hipGraphLaunch(s->gx, s->st) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_drop(s);
+        s->use_graph = false;
+        return false;
+    }
+    s->new_stale = true;
+    return true;
+}
+
+// n fused steps: pairs through the graph, the rest eager
+static void steps(pllc_sim* s, int n)
+{
+    while (n >= 2) {
+        const bool ok = s->prec ? step_pair_graph<float>(s, s->cf) : step_pair_graph<double>(s, s->cd);
+        if (!ok) break;
+        n -= 2;
+    }
+    for (; n > 0; n--) step_any(s);
+}
+
 static Ctx<double>& scal_ctx(pllc_sim* s) { return s->cd; }
 
 extern "C" int pllc_sim_run(pllc_sim* s, int nsteps, double* tval, double* dt)
 {
-    for (int n = 0; n < nsteps; n++) step_any(s);
+    steps(s, nsteps);
     if (!pllc_hip_ok("pllc_sim_run", hipGetLastError())) return -1;
     const double* tv = s->prec ? s->cf.tval : s->cd.tval;
     const double* sc = s->prec ? s->cf.scal : s->cd.scal;
@@ -684,10 +759,10 @@ extern "C" int pllc_sim_synchronize(pllc_sim* s) { return finish(s, "pllc_sim_sy
 
 extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* ms)
 {
-    for (int n = 0; n < warmup; n++) step_any(s);
+    steps(s, warmup);
     if (finish(s, "pllc_sim_time_steps")) return -1;
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e0, s->st))) return -1;
-    for (int n = 0; n < nsteps; n++) step_any(s);
+    steps(s, nsteps);
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventRecord(s->e1, s->st))) return -1;
     if (finish(s, "pllc_sim_time_steps")) return -1;
     float f = 0.f;

@@ -160,3 +160,30 @@ def test_shift_conc_matches_oracle(oracle_mod, subharm):
         orc.run(1)
         sim.run(1)
         _compare_step(orc, sim, "shift step %d" % (n + 1))
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_graph_replays_equal_eager_steps(precision, monkeypatch):
+    """Pairs of steps replayed from the captured HIP graph equal eager steps
+    bit for bit, replay after replay, with downloads (eager copies and
+    synchronisations) between the replays."""
+    prm = P.make_params(**SMALL)
+    c0 = P.ic_fill(prm, P.IC_NOISE, 5, 0.05)
+    monkeypatch.setenv("PLLC_GRAPH", "0")
+    eager = P.Sim(prm, precision=precision)
+    monkeypatch.setenv("PLLC_GRAPH", "1")
+    graph = P.Sim(prm, precision=precision)
+    eager.init(c0)
+    graph.init(c0)
+    for rep in range(4):
+        eager.run(2)
+        graph.run(2)
+        for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "MUO_CELL"):
+            a, b = graph.download(f), eager.download(f)
+            assert np.array_equal(a, b), (precision, rep, f, np.abs(a - b).max())
+        print("[graph] %s replay %d equal" % (precision, rep + 1), flush=True)
+    # an odd count (one eager step) and then pairs again from the other parity
+    eager.run(3)
+    graph.run(3)
+    for f in ("C_OLD", "U_OLD", "PHI"):
+        assert np.array_equal(graph.download(f), eager.download(f)), (precision, "odd", f)
