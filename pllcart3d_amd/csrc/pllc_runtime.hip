@@ -324,6 +324,10 @@ struct pllc_sim {
     // and the launch parameters stay those of the capture.  PLLC_GRAPH=0
     // keeps every step eager.
     bool use_graph = true;
+    // the next substage ends with update_old copies instead of swaps: set
+    // after an edit of c_old alone (shift_conc), so that the never-written
+    // points of c_new (inlet ghosts, ...) reach c_old as in the reference
+    bool copy_next = false;
     hipGraphExec_t gx = nullptr;
     std::vector<void*> gtab;
 };
@@ -534,6 +538,9 @@ static int upload_t(pllc_sim* s, Ctx<T>& c, int field, const double* host)
     normalize<T>(s, c);
     if (finish(s, "pllc_sim_upload")) return -1;
     if (!h2d<T>(s, c.f[field], host, npts(s), "pllc_sim_upload")) return -1;
+    // old and new may now differ where a step never writes: the next
+    // substage copies (as update_old_* does) instead of swapping
+    s->copy_next = true;
     return finish(s, "pllc_sim_upload");
 }
 
@@ -629,6 +636,7 @@ static int shift_t(pllc_sim* s, Ctx<T>& c, double Ampli, double Lamb, int Subhar
     int rc = -1;
     if (pllc_hip_ok(W, hipMemcpyAsync(dev, h.data(), h.size() * 8, hipMemcpyHostToDevice, s->st))) {
         Ops<T>::shift_conc(c, c.f[C_OLD], dev, dev + g.NX);
+        s->copy_next = true;
         rc = finish(s, W);
     }
     if (!pllc_hip_ok(W, hipStreamSynchronize(s->st))) rc = -1;
@@ -673,8 +681,14 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::velo_correction(c, g, r, rk);
         Ops<T>::velo_ghost(c);
         Ops<T>::pressure_gradient(c, g, r, 7);
-        for (auto& pr : kPairs) std::swap(c.f[pr[0]], c.f[pr[1]]);
-        s->new_stale = true;
+        if (s->copy_next) {
+            update_old_copy<T>(s, c);
+            s->copy_next = false;
+            s->new_stale = false;
+        } else {
+            for (auto& pr : kPairs) std::swap(c.f[pr[0]], c.f[pr[1]]);
+            s->new_stale = true;
+        }
     }
 }
 
@@ -695,7 +709,7 @@ static std::vector<void*> ptr_table(Ctx<T>& c)
 template <typename T>
 static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
 {
-    if (!s->use_graph || c.hook) return false;
+    if (!s->use_graph || c.hook || s->copy_next) return false;
     std::vector<void*> tab = ptr_table(c);
     if (s->gx && tab != s->gtab) graph_drop(s);
     if (!s->gx) {
@@ -733,6 +747,10 @@ static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
 // n fused steps: pairs through the graph, the rest eager
 static void steps(pllc_sim* s, int n)
 {
+    if (s->copy_next && n > 0) {
+        step_any(s);
+        n--;
+    }
     while (n >= 2) {
         const bool ok = s->prec ? step_pair_graph<float>(s, s->cf) : step_pair_graph<double>(s, s->cd);
         if (!ok) break;
