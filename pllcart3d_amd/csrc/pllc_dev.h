@@ -70,9 +70,13 @@ __host__ __device__ inline long IX(const Dims& d, int i, int j, int k)
 //   [+2] outlet flux, [+3] deficit.
 enum { S_DT = 0, S_CFL = 1, S_RK = 4, S_SIZE = 16 };
 
+// floor of the WENO epsilon (1e-99 in calculate_weno, pllcart3d.c:3494).  The weights
+// divide by (s + epi)^2, so in FP32 the floor must keep epi^2 above the
+// smallest normal float (1.2e-38): with s = 0 (flat data) a smaller floor
+// makes every weight inf and their quotients NaN.
 template <typename T> struct Tr;
 template <> struct Tr<double> { static constexpr double tiny = 1.0e-99; };
-template <> struct Tr<float> { static constexpr float tiny = 1.0e-30f; };
+template <> struct Tr<float> { static constexpr float tiny = 1.0e-18f; };
 
 // ---------------------------------------------------------------- WENO
 
