@@ -132,6 +132,7 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
     }
     d.c.d.dx = g->dx; d.c.d.dy = g->dy; d.c.d.dz = g->dz;
     phys_from_globals(d.c.ph);
+    pllc_phys_consts(d.c.ph, d.c.d);
     return &d.c;
 }
 

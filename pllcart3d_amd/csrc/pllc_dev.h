@@ -33,7 +33,34 @@ struct Phys {
     // (calculate_viscosity_nmvp, 895-909)
     int nmvp;
     double a, mum, psi0, psi1;
+    // constants of the A coefficients and of the explicit terms, evaluated
+    // once on the host with the reference's expressions (pllc_phys_consts):
+    // cdel[F][D] = 2/(Re h) on a component's own direction, 1/(Re h) across
+    // (A_umom/A_vmom/A_wmom 1453-1532), 1/((Pe h) h) for c (A_conc 1418);
+    // edel[F-1][0/1] the two cross-viscous factors of Hu/Hv/Hw (1734-1744,
+    // 1846-1856, 1958-1970); fre = Fx/Re, Fy/Re of the body force (975, 1132)
+    double cdel[4][3];
+    double edel[3][2];
+    double fre[2];
 };
+
+// fills Phys::cdel/edel/fre from Re, Pe, Fx, Fy and the spacings
+__host__ inline void pllc_phys_consts(Phys& ph, const Dims& d)
+{
+    const double h[3] = {d.dx, d.dy, d.dz};
+    for (int D = 0; D < 3; D++) {
+        ph.cdel[0][D] = 1.0 / (ph.Pe * h[D] * h[D]);
+        for (int F = 1; F <= 3; F++) ph.cdel[F][D] = (F - 1 == D ? 2.0 : 1.0) / (ph.Re * h[D]);
+    }
+    ph.edel[0][0] = 1.0 / (ph.Re * d.dy * d.dx);
+    ph.edel[0][1] = 1.0 / (ph.Re * d.dz * d.dx);
+    ph.edel[1][0] = 1.0 / (ph.Re * d.dx * d.dy);
+    ph.edel[1][1] = 1.0 / (ph.Re * d.dz * d.dy);
+    ph.edel[2][0] = 1.0 / (ph.Re * d.dx * d.dz);
+    ph.edel[2][1] = 1.0 / (ph.Re * d.dy * d.dz);
+    ph.fre[0] = ph.Fx / ph.Re;
+    ph.fre[1] = ph.Fy / ph.Re;
+}
 
 template <typename T>
 __device__ __forceinline__ T visc_law(const Phys& ph, T cavg)
@@ -161,40 +188,30 @@ template <typename T, int F, int D>
 __device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, int i, int j, int k,
                                       T& a, T& b, T& c)
 {
-    const T Re = T(ph.Re);
+    const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
+    const T del = T(ph.cdel[F][D]);
     if (F == FC) {
-        const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
-        const T del = T(1.0) / (T(ph.Pe) * h * h);
         a = del; c = del;
         b = (D == 2) ? (-a - c) : -(a + c);
         return;
     }
     if (F == FU) {
-        if (D == 0) { const T h = T(d.dx), del = T(2.0) / (Re * h);
-            a = del * mu.cell[IX(d, i - 1, j, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; b = -(a + c); }
-        else if (D == 1) { const T h = T(d.dy), del = T(1.0) / (Re * h);
-            a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i, j + 1, k)] / h; b = -(a + c); }
-        else { const T h = T(d.dz), del = T(1.0) / (Re * h);
-            a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i, j, k + 1)] / h; b = -a - c; }
+        if (D == 0) { a = del * mu.cell[IX(d, i - 1, j, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; b = -(a + c); }
+        else if (D == 1) { a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i, j + 1, k)] / h; b = -(a + c); }
+        else { a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i, j, k + 1)] / h; b = -a - c; }
         return;
     }
     if (F == FV) {
-        if (D == 0) { const T h = T(d.dx), del = T(1.0) / (Re * h);
-            a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i + 1, j, k)] / h; }
-        else if (D == 1) { const T h = T(d.dy), del = T(2.0) / (Re * h);
-            a = del * mu.cell[IX(d, i, j - 1, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
-        else { const T h = T(d.dz), del = T(1.0) / (Re * h);
-            a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j, k + 1)] / h; }
+        if (D == 0) { a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i + 1, j, k)] / h; }
+        else if (D == 1) { a = del * mu.cell[IX(d, i, j - 1, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
+        else { a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j, k + 1)] / h; }
         b = -a - c;
         return;
     }
     // FW
-    if (D == 0) { const T h = T(d.dx), del = T(1.0) / (Re * h);
-        a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i + 1, j, k)] / h; }
-    else if (D == 1) { const T h = T(d.dy), del = T(1.0) / (Re * h);
-        a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j + 1, k)] / h; }
-    else { const T h = T(d.dz), del = T(2.0) / (Re * h);
-        a = del * mu.cell[IX(d, i, j, k - 1)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
+    if (D == 0) { a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i + 1, j, k)] / h; }
+    else if (D == 1) { a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j + 1, k)] / h; }
+    else { a = del * mu.cell[IX(d, i, j, k - 1)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
     b = -a - c;
 }
 

@@ -121,27 +121,26 @@ template <typename T, int F>
 __device__ __forceinline__ T mom_explicit(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ u,
                                           const T* __restrict__ v, const T* __restrict__ w, int i, int j, int k, T ct)
 {
-    const T Re = T(ph.Re), dx = T(d.dx), dy = T(d.dy), dz = T(d.dz);
     T del, p1, m1, p2, m2;
     if (F == FU) {
-        del = T(1.0) / (Re * dy * dx);
+        del = T(ph.edel[0][0]);
         p1 = del * mu.xy[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i - 1, j + 1, k)]);
         m1 = del * mu.xy[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i - 1, j, k)]);
-        del = T(1.0) / (Re * dz * dx);
+        del = T(ph.edel[0][1]);
         p2 = del * mu.xz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i - 1, j, k + 1)]);
         m2 = del * mu.xz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i - 1, j, k)]);
     } else if (F == FV) {
-        del = T(1.0) / (Re * dx * dy);
+        del = T(ph.edel[1][0]);
         p1 = del * mu.xy[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j - 1, k)]);
         m1 = del * mu.xy[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j - 1, k)]);
-        del = T(1.0) / (Re * dz * dy);
+        del = T(ph.edel[1][1]);
         p2 = del * mu.yz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i, j - 1, k + 1)]);
         m2 = del * mu.yz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i, j - 1, k)]);
     } else {
-        del = T(1.0) / (Re * dx * dz);
+        del = T(ph.edel[2][0]);
         p1 = del * mu.xz[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j, k - 1)]);
         m1 = del * mu.xz[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j, k - 1)]);
-        del = T(1.0) / (Re * dy * dz);
+        del = T(ph.edel[2][1]);
         p2 = del * mu.yz[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i, j + 1, k - 1)]);
         m2 = del * mu.yz[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i, j, k - 1)]);
     }
@@ -193,10 +192,10 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     T bodyforce;
     if (F == FU) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sx]);
-        bodyforce = alpha * (-(T(ph.Fx) / Re) * cval - (pg[o] / Re));
+        bodyforce = alpha * (-T(ph.fre[0]) * cval - (pg[o] / Re));
     } else if (F == FV) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sy]);
-        bodyforce = alpha * (-(T(ph.Fy) / Re) * cval - (pg[o] / Re));
+        bodyforce = alpha * (-T(ph.fre[1]) * cval - (pg[o] / Re));
     } else {
         bodyforce = alpha * (-(pg[o] / Re));
     }

@@ -357,6 +357,7 @@ void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
     ph.Fx = p->Fx; ph.Fy = p->Fy; ph.CFLmax = p->CFLmax; ph.Timestep = p->Timestep;
     ph.Y1 = p->Y1; ph.Y2 = p->Y2; ph.Z1 = p->Z1; ph.Z2 = p->Z2;
     ph.nmvp = 0; ph.a = ph.mum = ph.psi0 = ph.psi1 = 0.0;
+    pllc_phys_consts(ph, d);
 }
 
 template <typename T>
