@@ -1,9 +1,12 @@
 """Mixed-precision path (FP32 fields, stencils and line solves; FP64 CFL,
 flux and bulk reductions and the (0,0) Poisson pencil) vs the FP64 oracle.
 
-The bar is stated per field and written here: after 10 steps the
-concentration is within 1e-5 relative L2 of the FP64 solution, velocities
-within 1e-4.  The errors are printed so their size is on record.
+The bar, written here: after 10 steps the concentration is within 1e-5
+relative L2 of the FP64 solution, the velocity (u, v, w as one vector
+field) within 1e-5 and phi within 1e-4.  Every field's own error is
+printed so its size is on record; the spanwise component w is ~1e-4 of u
+at these states, so its own relative error carries the velocity's FP32
+rounding amplified by |u|/|w| and is held to 1e-2.
 """
 import numpy as np
 import pytest
@@ -13,7 +16,14 @@ from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"C_OLD": 1e-5, "U_OLD": 1e-4, "V_OLD": 1e-4, "W_OLD": 1e-3, "PHI": 1e-4}
+TOL = {"C_OLD": 1e-5, "U_OLD+V_OLD+W_OLD": 1e-5, "PHI": 1e-4, "U_OLD": 1e-5, "V_OLD": 1e-4, "W_OLD": 1e-2}
+
+
+def _err(sim, orc, name):
+    parts = name.split("+")
+    a = np.concatenate([sim.download(p).ravel() for p in parts])
+    b = np.concatenate([orc.field(p).ravel() for p in parts])
+    return rel_l2(a, b)
 
 
 @pytest.mark.parametrize("shape", [(32, 16, 16), (64, 32, 32)])
@@ -27,7 +37,7 @@ def test_mixed_tracks_fp64_oracle(oracle_mod, shape):
     for n in range(10):
         orc.run(1)
         sim.run(1)
-    errs = {f: rel_l2(sim.download(f), orc.field(f)) for f in TOL}
+    errs = {f: _err(sim, orc, f) for f in TOL}
     print("[mixed %s] after 10 steps: %s" % (shape, " ".join("%s=%.2e" % kv for kv in errs.items())), flush=True)
     for f, tol in TOL.items():
         assert errs[f] <= tol, (f, errs[f])
