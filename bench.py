@@ -75,17 +75,49 @@ def kernel_table(sim, ksteps):
     return out
 
 
-def roofline(kernels, cells, bytes_per_value):
+# Peaks (MI355X datasheet): HBM3E 8 TB/s; dense FP64 78.6 TFLOP/s (vector and
+# matrix); FP32 157.3 TFLOP/s.
+PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
+
+
+def dct_flops_per_launch(prm):
+    """Dense REDFT contractions: the four launches of a substage are two
+    along z (M = nx ny, N = K = nz) and two along y (nx batches of
+    M = K = ny, N = nz), 2 M N K flops each; the mean per launch."""
+    nx, ny, nz = prm.nx, prm.ny, prm.nz
+    return (2.0 * nx * ny * nz * nz + 2.0 * nx * ny * ny * nz) / 2.0
+
+
+def roofline(kernels, cells, bytes_per_value, prm, traffic=None):
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    avg_s = kernels[dom]["avg_ms"] * 1e-3
+    if dom == "dct":
+        fl = dct_flops_per_launch(prm)
+        ach = fl / avg_s / 1e12
+        peak = PEAK_TFLOPS[bytes_per_value]
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": (traffic or {}).get(dom), "kernel": dom,
+                "algorithmic_flops_per_launch": fl, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
     t = TOUCHES.get(dom) or SWEEP_TOUCHES.get(dom)
     if t is None:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom}
     ab = t * cells * bytes_per_value
-    ach = ab / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    ach = ab / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": (traffic or {}).get(dom), "kernel": dom,
             "algorithmic_bytes_per_launch": ab, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
+
+
+def hbm_table(kernels, cells, bytes_per_value):
+    """Achieved algorithmic GB/s of every kernel class with an AB model."""
+    out = {}
+    for k, v in kernels.items():
+        t = TOUCHES.get(k) or SWEEP_TOUCHES.get(k)
+        if t is None or not v["avg_ms"]:
+            continue
+        out[k] = round(t * cells * bytes_per_value / (v["avg_ms"] * 1e-3) / 1e9, 1)
+    return out
 
 
 def main():
@@ -123,7 +155,7 @@ def main():
         ksteps = max(2, min(5, a.steps))
         kernels = kernel_table(plain, ksteps)
         kernels_m = kernel_table(mixed, ksteps)
-        roof = roofline(kernels, cells, 8.0)
+        roof = roofline(kernels, cells, 8.0, prm)
     step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
     cpu = None
     if not a.no_cpu:
@@ -141,7 +173,9 @@ def main():
         "mixed": {"value": cells * a.steps / (ms_m * 1e-3), "ms_per_step": ms_m / a.steps, "dtype": "f32/f64",
                   "speedup_vs_plain": ms / ms_m,
                   "rel_l2_vs_fp64_after_%d_steps" % (a.warmup + a.steps): err,
-                  "roofline": roofline(kernels_m, cells, 4.0) if kernels_m else None},
+                  "roofline": roofline(kernels_m, cells, 4.0, prm) if kernels_m else None,
+                  "kernel_gbs": hbm_table(kernels_m, cells, 4.0) if kernels_m else None},
+        "kernel_gbs": hbm_table(kernels, cells, 8.0) if kernels else None,
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
         "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},
     }

@@ -175,6 +175,39 @@ __device__ __forceinline__ T upwind(const T* __restrict__ p, long st, int m, int
                  dd1(p, st, kk + 3 * sign, h), dd1(p, st, kk + 4 * sign, h));
 }
 
+// The six first differences of a line around point m: D[q] = D1 at face
+// m-3+q, q = 0..5, from the seven values pc[-3 st .. 3 st] (pc points at m).
+// Every stencil below reads them at fixed offsets, so each address is a
+// uniform offset from one per-field pointer.
+template <typename T>
+__device__ __forceinline__ void line_d1(const T* __restrict__ pc, long st, T h, T (&D)[6])
+{
+    T v[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) v[q] = pc[(long)(q - 3) * st];
+#pragma unroll
+    for (int q = 0; q < 6; q++) D[q] = (v[q + 1] - v[q]) / h;
+}
+
+// upwind() on the differences of line_d1: the same ENO2 / HJ-WENO5 choice
+// and operands (1566-1574), with the faces picked by the upwind side
+template <typename T>
+__device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, int sign, T h, T lo, T hi)
+{
+    const bool up = sign == 1;
+    if ((m <= ms + 1) || (m >= me - 1)) {
+        const int kk = up ? m - 1 : m;
+        const T d1k = up ? D[2] : D[3];
+        const T d1m = up ? D[1] : D[2];
+        const T d1p = up ? D[3] : D[4];
+        const T a = (kk < ms) ? lo : (kk > me) ? hi : (d1k - d1m) / (T(2.0) * h);
+        const T b = (kk + 1 < ms) ? lo : (kk + 1 > me) ? hi : (d1p - d1k) / (T(2.0) * h);
+        const T d2 = (fabs(a) <= fabs(b)) ? a : b;
+        return d1k + d2 * h * T(sign);
+    }
+    return weno5(up ? D[0] : D[5], up ? D[1] : D[4], up ? D[2] : D[3], up ? D[3] : D[2], up ? D[4] : D[1]);
+}
+
 // ------------------------------------------------ diffusion coefficients
 //
 // The reference stores Tri_abc arrays filled by A_conc / A_umom / A_vmom /
@@ -185,33 +218,33 @@ template <typename T>
 struct Visc4 { const T* cell; const T* xy; const T* xz; const T* yz; };
 
 template <typename T, int F, int D>
-__device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, int i, int j, int k,
-                                      T& a, T& b, T& c)
+__device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, long o, T& a, T& b, T& c)
 {
     const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
     const T del = T(ph.cdel[F][D]);
+    const long sx = d.sx, sy = d.sy;
     if (F == FC) {
         a = del; c = del;
         b = (D == 2) ? (-a - c) : -(a + c);
         return;
     }
     if (F == FU) {
-        if (D == 0) { a = del * mu.cell[IX(d, i - 1, j, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; b = -(a + c); }
-        else if (D == 1) { a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i, j + 1, k)] / h; b = -(a + c); }
-        else { a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i, j, k + 1)] / h; b = -a - c; }
+        if (D == 0) { a = del * mu.cell[o - sx] / h; c = del * mu.cell[o] / h; b = -(a + c); }
+        else if (D == 1) { a = del * mu.xy[o] / h; c = del * mu.xy[o + sy] / h; b = -(a + c); }
+        else { a = del * mu.xz[o] / h; c = del * mu.xz[o + 1] / h; b = -a - c; }
         return;
     }
     if (F == FV) {
-        if (D == 0) { a = del * mu.xy[IX(d, i, j, k)] / h; c = del * mu.xy[IX(d, i + 1, j, k)] / h; }
-        else if (D == 1) { a = del * mu.cell[IX(d, i, j - 1, k)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
-        else { a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j, k + 1)] / h; }
+        if (D == 0) { a = del * mu.xy[o] / h; c = del * mu.xy[o + sx] / h; }
+        else if (D == 1) { a = del * mu.cell[o - sy] / h; c = del * mu.cell[o] / h; }
+        else { a = del * mu.yz[o] / h; c = del * mu.yz[o + 1] / h; }
         b = -a - c;
         return;
     }
     // FW
-    if (D == 0) { a = del * mu.xz[IX(d, i, j, k)] / h; c = del * mu.xz[IX(d, i + 1, j, k)] / h; }
-    else if (D == 1) { a = del * mu.yz[IX(d, i, j, k)] / h; c = del * mu.yz[IX(d, i, j + 1, k)] / h; }
-    else { a = del * mu.cell[IX(d, i, j, k - 1)] / h; c = del * mu.cell[IX(d, i, j, k)] / h; }
+    if (D == 0) { a = del * mu.xz[o] / h; c = del * mu.xz[o + sx] / h; }
+    else if (D == 1) { a = del * mu.yz[o] / h; c = del * mu.yz[o + sy] / h; }
+    else { a = del * mu.cell[o - 1] / h; c = del * mu.cell[o] / h; }
     b = -a - c;
 }
 

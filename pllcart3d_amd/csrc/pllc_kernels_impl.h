@@ -68,22 +68,27 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     const long o = IX(d, i, j, k);
     const T dt = T(scal[S_DT]);
 
+    const T* __restrict__ cc = cold + o;
+    T D[6];
     const T uval = T(0.5) * (un[o + d.sx] + un[o]);
-    const T cxd = upwind(cold + IX(d, 0, j, k), d.sx, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    line_d1(cc, d.sx, T(d.dx), D);
+    const T cxd = upwind_d(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
     const T convx = uval * cxd;
     const T vval = T(0.5) * (vn[o + d.sy] + vn[o]);
-    const T cyd = upwind(cold + IX(d, i, 0, k), d.sy, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    line_d1(cc, d.sy, T(d.dy), D);
+    const T cyd = upwind_d(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
     const T convy = vval * cyd;
     const T wval = T(0.5) * (wn[o + 1] + wn[o]);
-    const T czd = upwind(cold + IX(d, i, j, 0), 1, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    line_d1(cc, 1, T(d.dz), D);
+    const T czd = upwind_d(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
     const T nval = -(wval * czd + convx + convy);
     N[o] = nval;
 
     T ax, bx, cx, ay, by, cy, az, bz, cz;
     Visc4<T> none{nullptr, nullptr, nullptr, nullptr};
-    coefs<T, FC, 0>(d, ph, none, i, j, k, ax, bx, cx);
-    coefs<T, FC, 1>(d, ph, none, i, j, k, ay, by, cy);
-    coefs<T, FC, 2>(d, ph, none, i, j, k, az, bz, cz);
+    coefs<T, FC, 0>(d, ph, none, o, ax, bx, cx);
+    coefs<T, FC, 1>(d, ph, none, o, ay, by, cy);
+    coefs<T, FC, 2>(d, ph, none, o, az, bz, cz);
     const T conv = (g * nval + r * Nm1[o]);
     const T axx = ax * cold[o - d.sx] + bx * cold[o] + cx * cold[o + d.sx];
     const T axy = ay * cold[o - d.sy] + by * cold[o] + cy * cold[o + d.sy];
@@ -97,21 +102,22 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
 // direction, 4-point averages of the others.
 template <typename T, int F, int D>
 __device__ __forceinline__ T mom_vel(const Dims& d, const T* __restrict__ u, const T* __restrict__ v,
-                                     const T* __restrict__ w, int i, int j, int k)
+                                     const T* __restrict__ w, long o)
 {
+    const long sx = d.sx, sy = d.sy;
     if (F == FU) {
-        if (D == 0) return u[IX(d, i, j, k)];
-        if (D == 1) return T(0.25) * (v[IX(d, i - 1, j, k)] + v[IX(d, i, j, k)] + v[IX(d, i, j + 1, k)] + v[IX(d, i - 1, j + 1, k)]);
-        return T(0.25) * (w[IX(d, i - 1, j, k)] + w[IX(d, i, j, k)] + w[IX(d, i, j, k + 1)] + w[IX(d, i - 1, j, k + 1)]);
+        if (D == 0) return u[o];
+        if (D == 1) return T(0.25) * (v[o - sx] + v[o] + v[o + sy] + v[o - sx + sy]);
+        return T(0.25) * (w[o - sx] + w[o] + w[o + 1] + w[o - sx + 1]);
     }
     if (F == FV) {
-        if (D == 0) return T(0.25) * (u[IX(d, i, j - 1, k)] + u[IX(d, i + 1, j - 1, k)] + u[IX(d, i, j, k)] + u[IX(d, i + 1, j, k)]);
-        if (D == 1) return v[IX(d, i, j, k)];
-        return T(0.25) * (w[IX(d, i, j - 1, k)] + w[IX(d, i, j - 1, k + 1)] + w[IX(d, i, j, k)] + w[IX(d, i, j, k + 1)]);
+        if (D == 0) return T(0.25) * (u[o - sy] + u[o + sx - sy] + u[o] + u[o + sx]);
+        if (D == 1) return v[o];
+        return T(0.25) * (w[o - sy] + w[o - sy + 1] + w[o] + w[o + 1]);
     }
-    if (D == 0) return T(0.25) * (u[IX(d, i, j, k - 1)] + u[IX(d, i + 1, j, k - 1)] + u[IX(d, i, j, k)] + u[IX(d, i + 1, j, k)]);
-    if (D == 1) return T(0.25) * (v[IX(d, i, j, k - 1)] + v[IX(d, i, j, k)] + v[IX(d, i, j + 1, k - 1)] + v[IX(d, i, j + 1, k)]);
-    return w[IX(d, i, j, k)];
+    if (D == 0) return T(0.25) * (u[o - 1] + u[o + sx - 1] + u[o] + u[o + sx]);
+    if (D == 1) return T(0.25) * (v[o - 1] + v[o] + v[o + sy - 1] + v[o + sy]);
+    return w[o];
 }
 
 // explicit_n of a velocity component: -convec_term + the explicit
@@ -119,30 +125,31 @@ __device__ __forceinline__ T mom_vel(const Dims& d, const T* __restrict__ u, con
 // Hv 1846-1856, Hw 1958-1970 with the live index of 1962).
 template <typename T, int F>
 __device__ __forceinline__ T mom_explicit(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ u,
-                                          const T* __restrict__ v, const T* __restrict__ w, int i, int j, int k, T ct)
+                                          const T* __restrict__ v, const T* __restrict__ w, long o, T ct)
 {
+    const long sx = d.sx, sy = d.sy;
     T del, p1, m1, p2, m2;
     if (F == FU) {
         del = T(ph.edel[0][0]);
-        p1 = del * mu.xy[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i - 1, j + 1, k)]);
-        m1 = del * mu.xy[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i - 1, j, k)]);
+        p1 = del * mu.xy[o + sy] * (v[o + sy] - v[o - sx + sy]);
+        m1 = del * mu.xy[o] * (v[o] - v[o - sx]);
         del = T(ph.edel[0][1]);
-        p2 = del * mu.xz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i - 1, j, k + 1)]);
-        m2 = del * mu.xz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i - 1, j, k)]);
+        p2 = del * mu.xz[o + 1] * (w[o + 1] - w[o - sx + 1]);
+        m2 = del * mu.xz[o] * (w[o] - w[o - sx]);
     } else if (F == FV) {
         del = T(ph.edel[1][0]);
-        p1 = del * mu.xy[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j - 1, k)]);
-        m1 = del * mu.xy[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j - 1, k)]);
+        p1 = del * mu.xy[o + sx] * (u[o + sx] - u[o + sx - sy]);
+        m1 = del * mu.xy[o] * (u[o] - u[o - sy]);
         del = T(ph.edel[1][1]);
-        p2 = del * mu.yz[IX(d, i, j, k + 1)] * (w[IX(d, i, j, k + 1)] - w[IX(d, i, j - 1, k + 1)]);
-        m2 = del * mu.yz[IX(d, i, j, k)] * (w[IX(d, i, j, k)] - w[IX(d, i, j - 1, k)]);
+        p2 = del * mu.yz[o + 1] * (w[o + 1] - w[o - sy + 1]);
+        m2 = del * mu.yz[o] * (w[o] - w[o - sy]);
     } else {
         del = T(ph.edel[2][0]);
-        p1 = del * mu.xz[IX(d, i + 1, j, k)] * (u[IX(d, i + 1, j, k)] - u[IX(d, i + 1, j, k - 1)]);
-        m1 = del * mu.xz[IX(d, i, j, k)] * (u[IX(d, i, j, k)] - u[IX(d, i, j, k - 1)]);
+        p1 = del * mu.xz[o + sx] * (u[o + sx] - u[o + sx - 1]);
+        m1 = del * mu.xz[o] * (u[o] - u[o - 1]);
         del = T(ph.edel[2][1]);
-        p2 = del * mu.yz[IX(d, i, j + 1, k)] * (v[IX(d, i, j + 1, k)] - v[IX(d, i, j + 1, k - 1)]);
-        m2 = del * mu.yz[IX(d, i, j, k)] * (v[IX(d, i, j, k)] - v[IX(d, i, j, k - 1)]);
+        p2 = del * mu.yz[o + sy] * (v[o + sy] - v[o + sy - 1]);
+        m2 = del * mu.yz[o] * (v[o] - v[o - 1]);
     }
     return -ct + p1 - m1 + p2 - m2;
 }
@@ -167,22 +174,27 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     const T dt = T(scal[S_DT]);
     const T alpha = g + r;
 
-    const T vx = mom_vel<T, F, 0>(d, u, v, w, i, j, k);
-    const T dxd = upwind(q + IX(d, 0, j, k), d.sx, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    const T* __restrict__ qc = q + o;
+    T D[6];
+    const T vx = mom_vel<T, F, 0>(d, u, v, w, o);
+    line_d1(qc, d.sx, T(d.dx), D);
+    const T dxd = upwind_d(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
     const T convx = vx * dxd;
-    const T vy = mom_vel<T, F, 1>(d, u, v, w, i, j, k);
-    const T dyd = upwind(q + IX(d, i, 0, k), d.sy, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    const T vy = mom_vel<T, F, 1>(d, u, v, w, o);
+    line_d1(qc, d.sy, T(d.dy), D);
+    const T dyd = upwind_d(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
     const T convy = vy * dyd;
-    const T vz = mom_vel<T, F, 2>(d, u, v, w, i, j, k);
-    const T dzd = upwind(q + IX(d, i, j, 0), 1, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    const T vz = mom_vel<T, F, 2>(d, u, v, w, o);
+    line_d1(qc, 1, T(d.dz), D);
+    const T dzd = upwind_d(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
     const T ct = vz * dzd + convx + convy;
-    const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, i, j, k, ct);
+    const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, o, ct);
     N[o] = nval;
 
     T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0>(d, ph, mu, i, j, k, ax, bx, cx);
-    coefs<T, F, 1>(d, ph, mu, i, j, k, ay, by, cy);
-    coefs<T, F, 2>(d, ph, mu, i, j, k, az, bz, cz);
+    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
+    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
+    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
     const T conv = (g * nval + r * Nm1[o]);
     const T axx = ax * q[o - d.sx] + bx * q[o] + cx * q[o + d.sx];
     const T axy = ay * q[o - d.sy] + by * q[o] + cy * q[o + d.sy];
@@ -844,9 +856,9 @@ __device__ __forceinline__ void pg_update(const Dims& d, const Phys& ph, const V
 {
     const long o = IX(d, i, j, k);
     T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0>(d, ph, mu, i, j, k, ax, bx, cx);
-    coefs<T, F, 1>(d, ph, mu, i, j, k, ay, by, cy);
-    coefs<T, F, 2>(d, ph, mu, i, j, k, az, bz, cz);
+    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
+    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
+    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
     const T g0 = pgrad<T, F>(d, phi, o);
     const T pxx = (ax) * pgrad<T, F>(d, phi, o - d.sx) + (bx) * g0 + (cx) * pgrad<T, F>(d, phi, o + d.sx);
     const T pyy = (ay) * pgrad<T, F>(d, phi, o - d.sy) + (by) * g0 + (cy) * pgrad<T, F>(d, phi, o + d.sy);

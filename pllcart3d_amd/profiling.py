@@ -52,11 +52,12 @@ def _counters(db, cname):
 def summary(args):
     out = []
     con = sqlite3.connect(args.trace)
+    # top_kernels durations are in microseconds
     rows = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
     out.append("| kernel | calls | total ms | avg us | % |")
     out.append("|---|---:|---:|---:|---:|")
     for name, calls, tot, avg, pct in sorted(rows, key=lambda r: -r[2]):
-        out.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (_short(name)[:60], calls, tot / 1e6, avg / 1e3, pct))
+        out.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (_short(name)[:60], calls, tot / 1e3, avg, pct))
     if args.fetch and args.write:
         f = _counters(args.fetch, "FETCH_SIZE")
         w = _counters(args.write, "WRITE_SIZE")

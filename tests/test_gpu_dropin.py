@@ -22,6 +22,26 @@ PARAMS = [("Nx_cell_global", 24), ("Ny_cell_global", 12), ("Nz_cell_global", 10)
           ("Lambda", 0.0)]
 
 
+DUMP = ["C_NEW", "U_NEW", "V_NEW", "W_NEW", "PHI", "U_PG", "V_PG", "W_PG"]
+
+
+def _check_dump(dump, orc, tag):
+    """c, velocity (u, v, w as one vector field), phi and the pressure
+    gradient (one vector field) <= 1e-11 relative L2.  With the erf front
+    the flow is two-dimensional: w and w_pg are rounding noise around 0 in
+    both solvers, so they are held to the velocity's scale instead."""
+    groups = {"c": [0], "velocity": [1, 2, 3], "phi": [4], "pg": [5, 6, 7]}
+    for name, qs in groups.items():
+        a = np.concatenate([dump[q].ravel() for q in qs])
+        b = np.concatenate([orc.field(DUMP[q]).ravel() for q in qs])
+        e = rel_l2(a, b)
+        print("[%s] %s rel L2 %.2e" % (tag, name, e), flush=True)
+        assert e <= 1e-11, (name, e)
+    uscale = np.abs(orc.field("U_NEW")).max()
+    for q in (3, 7):
+        assert np.abs(dump[q] - orc.field(DUMP[q])).max() <= 1e-11 * uscale, DUMP[q]
+
+
 def write_param_input(path):
     with open(path, "w") as f:
         for name, val in PARAMS:
@@ -42,10 +62,7 @@ def test_driver_matches_oracle(tmp_path, oracle_mod):
     orc = oracle_mod.Oracle(prm)
     orc.init()
     orc.run(nsteps)
-    for q, f in enumerate(["C_NEW", "U_NEW", "V_NEW", "W_NEW", "PHI", "U_PG", "V_PG", "W_PG"]):
-        e = rel_l2(dump[q], orc.field(f))
-        print("[dropin] %s rel L2 %.2e" % (f, e), flush=True)
-        assert e <= 1e-11, (f, e)
+    _check_dump(dump, orc, "dropin")
 
 
 def test_driver_nmvp_matches_oracle(tmp_path, oracle_mod):
@@ -58,10 +75,7 @@ def test_driver_nmvp_matches_oracle(tmp_path, oracle_mod):
     orc.init()
     orc.run(3)
     assert ("cm = %f" % orc.cm) in (tmp_path / "Outscreen").read_text()
-    for q, f in enumerate(["C_NEW", "U_NEW", "V_NEW", "W_NEW", "PHI", "U_PG", "V_PG", "W_PG"]):
-        e = rel_l2(dump[q], orc.field(f))
-        print("[dropin nmvp] %s rel L2 %.2e" % (f, e), flush=True)
-        assert e <= 1e-11, (f, e)
+    _check_dump(dump, orc, "dropin nmvp")
 
 
 def _run(tmp, nsteps, overrides, dump=None):
@@ -79,7 +93,12 @@ def _run(tmp, nsteps, overrides, dump=None):
 
 def test_driver_outputs_and_restart(tmp_path, oracle_mod):
     """caf files, conc_average, find_max and the midfield/intermediate
-    restart: 3 steps + restart + 2 steps equals 5 straight steps."""
+    restart.  midfield (pllcart3d.c:3121-3228) saves c_old, c's N_nm1 and
+    u/v/w old, new, N_nm1 and pg; a restarted run starts with every other
+    array zero (c_new among them, whose never-written points - outlet and
+    ghost rows - then reach c_old and the viscosity), so 3 steps + restart +
+    2 steps is checked against the oracle restarted the same way, not
+    against 5 straight steps."""
     shape = (30, 14, 12)
     a, b, c = tmp_path / "a", tmp_path / "b", tmp_path / "c"
     for d in (a, b, c):
@@ -92,12 +111,16 @@ def test_driver_outputs_and_restart(tmp_path, oracle_mod):
     _run(b, 2, dict(io, Initial=0), "dump.bin")
     _run(c, 5, io, "dump.bin")
     db = np.fromfile(b / "dump.bin").reshape(8, *shape)
-    dc = np.fromfile(c / "dump.bin").reshape(8, *shape)
-    for q in range(8):
-        assert rel_l2(db[q], dc[q]) <= 1e-13, q
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
+    rst = oracle_mod.Oracle(prm)
+    rst.init()
+    rst.run(3)
+    for f in ("C_NEW", "C_N", "U_N", "V_N", "W_N", "PHI", "DIV"):
+        rst.field(f)[...] = 0.0
+    rst.run(2)
+    _check_dump(db, rst, "restart")
 
     # caf0002 holds the state after step 4 (writes at t=0, 0.004, 0.008)
-    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
     orc = oracle_mod.Oracle(prm)
     orc.init()
     orc.run(1)
