@@ -998,40 +998,69 @@ __global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ p
 
 // ---------------------------------------------------------------- DCT GEMM
 
-// FP64/FP32 LDS-tiled GEMM for the dense DCT contractions (fftw_3d.h
-// REDFT10 / REDFT01 as matrices):
+// Dense DCT contractions (fftw_3d.h REDFT10 / REDFT01 as matrices) on the
+// matrix cores:
 //   TRB = true : C[b] (M x N) = A[b] (M x K) * B^T, B stored N x K
 //   TRB = false: C[b] (M x N) = A[b] (M x K) * B[b], B stored K x N
-// Every output is accumulated over k in increasing order with fused
-// multiply-adds.  FFTW's fast transforms round differently from any direct
-// sum, so the dense sums agree with the reference's REDFT10/01 to rounding,
-// as the oracle's direct sums do.
+// One 16x16x4 MFMA per step (V_MFMA_F64_16X16X4_F64 / V_MFMA_F32_16X16X4_F32):
+// lane l supplies A[l%16][k0 + l/16] and B[k0 + l/16][l%16]; its four
+// results are C[row][l%16] with row = l/16 + 4r for FP64 and 4(l/16) + r for
+// FP32, r = 0..3 (both measured on gfx950 with single-lane probes).  Block tile 64 x 128 (4 wavefronts of
+// 32 x 64, 2 x 4 MFMA tiles each), K chunks of 16 staged in LDS.  Every
+// output accumulates over k in increasing order with fused multiply-adds;
+// FFTW's fast transforms round differently from any direct sum, so the
+// results agree with the reference's REDFT10/01 to rounding.
+template <typename T> struct Mma;
+template <> struct Mma<double> {
+    typedef double acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+    static __device__ __forceinline__ acc run(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mma<float> {
+    typedef float acc __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+    static __device__ __forceinline__ acc run(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
 template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
 {
-    // 64 x 128 output tile per block, 4 x 8 per thread (rows ty+16p,
-    // columns tx+16q), K chunks of 16 staged in LDS
-    __shared__ T As[16][64 + 1];
-    __shared__ T Bs[16][128 + 1];
+    constexpr int BM = 64, BN = 128, BK = 16;
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
+    typedef typename Mma<T>::acc acc_t;
     const int b = blockIdx.z;
     A += (long)b * sA; B += (long)b * sB; C += (long)b * sC;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 128;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    T acc[4][8];
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t acc[2][4];
 #pragma unroll
-    for (int p = 0; p < 4; p++)
+    for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) acc[p][q] = T(0);
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+        for (int t = 0; t < 4; t++) acc[s][t] = acc_t{T(0), T(0), T(0), T(0)};
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        // A tile: consecutive threads on consecutive k of a row
+#pragma unroll
+        for (int p = 0; p < BM * BK / 256; p++) {
+            const int e = threadIdx.x + 256 * p;
             const int mm = e >> 4, kk = e & 15;
             const int gm = m0 + mm, gk = k0 + kk;
             As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
         }
-        for (int e = threadIdx.x; e < 128 * 16; e += 256) {
-            const int nn = TRB ? (e >> 4) : (e & 127);
-            const int kb = TRB ? (e & 15) : (e >> 7);
+#pragma unroll
+        for (int p = 0; p < BN * BK / 256; p++) {
+            const int e = threadIdx.x + 256 * p;
+            const int nn = TRB ? (e >> 4) : (e & (BN - 1));
+            const int kb = TRB ? (e & 15) : (e / BN);
             const int gn = n0 + nn, gkb = k0 + kb;
             T bv = T(0);
             if (gn < N && gkb < K) bv = TRB ? B[(long)gn * K + gkb] : B[(long)gkb * N + gn];
@@ -1039,31 +1068,31 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) {
-            T av[4], bv[8];
+        for (int kk = 0; kk < BK; kk += 4) {
+            T av[2], bv[4];
 #pragma unroll
-            for (int p = 0; p < 4; p++) av[p] = As[kk][ty + 16 * p];
+            for (int s = 0; s < 2; s++) av[s] = As[kk + lk][wm + 16 * s + li];
 #pragma unroll
-            for (int q = 0; q < 8; q++) bv[q] = Bs[kk][tx + 16 * q];
+            for (int t = 0; t < 4; t++) bv[t] = Bs[kk + lk][wn + 16 * t + li];
 #pragma unroll
-            for (int p = 0; p < 4; p++)
+            for (int s = 0; s < 2; s++)
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    acc[p][q] = fma(av[p], bv[q], acc[p][q]);
-                }
+                for (int t = 0; t < 4; t++) acc[s][t] = Mma<T>::run(av[s], bv[t], acc[s][t]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int gm = m0 + ty + 16 * p;
-        if (gm >= M) continue;
+    for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int gn = n0 + tx + 16 * q;
-            if (gn < N) C[(long)gm * N + gn] = acc[p][q];
+        for (int t = 0; t < 4; t++) {
+            const int gn = n0 + wn + 16 * t + li;
+            if (gn >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gm = m0 + wm + 16 * s + Mma<T>::row(lk, r);
+                if (gm < M) C[(long)gm * N + gn] = acc[s][t][r];
+            }
         }
-    }
 }
 
 // ------------------------------------------------------------ utilities
