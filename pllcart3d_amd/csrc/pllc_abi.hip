@@ -110,8 +110,8 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
         return nullptr;
     }
     if (d.nx != g->nx_cell || d.ny != g->ny_cell || d.nz != g->nz_cell) {
-        hipStreamSynchronize(d.st);
-        for (void* p : d.owned) hipFree(p);
+        (void)hipStreamSynchronize(d.st);
+        for (void* p : d.owned) (void)hipFree(p);
         d.owned.clear();
         d.c = pllc::Ctx<double>();
         d.c.st = d.st;
@@ -211,8 +211,8 @@ extern "C" void free3d_dble(int nrows, int ncols, int nz, double*** array3)
     (void)nrows; (void)ncols; (void)nz;
     auto it = arrs().find((const void*)array3);
     if (it == arrs().end()) return;
-    hipStreamSynchronize(din().st);
-    if (it->second.dev) hipFree(it->second.dev);
+    (void)hipStreamSynchronize(din().st);
+    if (it->second.dev) (void)hipFree(it->second.dev);
     free(it->second.host);
     free(it->second.rows);
     free(it->second.tab);

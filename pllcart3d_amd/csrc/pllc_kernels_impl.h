@@ -568,10 +568,10 @@ __global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restri
 template <typename T>
 __global__ void k_velo_ghost(Dims d, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w)
 {
-    const long nA = (long)d.nx * d.ny, nB = (long)d.nx * d.nz, nC = (long)d.ny * d.nz;
-    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nA = d.nx * d.ny, nB = d.nx * d.nz, nC = d.ny * d.nz;
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < nA) {
-        const int i = 3 + (int)(l / d.ny), j = 1 + (int)(l % d.ny);
+        const int i = 3 + l / d.ny, j = 1 + l % d.ny;
         const long lo = IX(d, i, j, 0), hi = IX(d, i, j, d.nz + 1);
         u[lo] = u[lo + 1]; v[lo] = v[lo + 1];
         u[hi] = u[hi - 1]; v[hi] = v[hi - 1];
@@ -579,7 +579,7 @@ __global__ void k_velo_ghost(Dims d, T* __restrict__ u, T* __restrict__ v, T* __
     }
     l -= nA;
     if (l < nB) {
-        const int i = 3 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const int i = 3 + l / d.nz, k = 1 + l % d.nz;
         const long lo = IX(d, i, 0, k), hi = IX(d, i, d.ny + 1, k);
         u[lo] = -u[lo + d.sy]; w[lo] = -w[lo + d.sy];
         u[hi] = -u[hi - d.sy]; w[hi] = -w[hi - d.sy];
@@ -587,7 +587,7 @@ __global__ void k_velo_ghost(Dims d, T* __restrict__ u, T* __restrict__ v, T* __
     }
     l -= nB;
     if (l < nC) {
-        const int j = 1 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const int j = 1 + l / d.nz, k = 1 + l % d.nz;
         const long lo = IX(d, 2, j, k);
         v[lo] = -v[lo + d.sx]; w[lo] = -w[lo + d.sx];
     }
@@ -597,24 +597,24 @@ __global__ void k_velo_ghost(Dims d, T* __restrict__ u, T* __restrict__ v, T* __
 template <typename T>
 __global__ void k_phi_ghost(Dims d, T* __restrict__ phi)
 {
-    const long nA = (long)d.nx * d.ny, nB = (long)d.nx * d.nz, nC = (long)d.ny * d.nz;
-    long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nA = d.nx * d.ny, nB = d.nx * d.nz, nC = d.ny * d.nz;
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < nA) {
-        const int i = 3 + (int)(l / d.ny), j = 1 + (int)(l % d.ny);
+        const int i = 3 + l / d.ny, j = 1 + l % d.ny;
         const long lo = IX(d, i, j, 0), hi = IX(d, i, j, d.nz + 1);
         phi[lo] = phi[lo + 1]; phi[hi] = phi[hi - 1];
         return;
     }
     l -= nA;
     if (l < nB) {
-        const int i = 3 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const int i = 3 + l / d.nz, k = 1 + l % d.nz;
         const long lo = IX(d, i, 0, k), hi = IX(d, i, d.ny + 1, k);
         phi[lo] = phi[lo + d.sy]; phi[hi] = phi[hi - d.sy];
         return;
     }
     l -= nB;
     if (l < nC) {
-        const int j = 1 + (int)(l / d.nz), k = 1 + (int)(l % d.nz);
+        const int j = 1 + l / d.nz, k = 1 + l % d.nz;
         const long lo = IX(d, 2, j, k), hi = IX(d, d.nx + 3, j, k);
         phi[lo] = phi[lo + d.sx]; phi[hi] = phi[hi - d.sx];
     }
@@ -814,7 +814,6 @@ __global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __rest
                                                       const double* __restrict__ part, int rk)
 {
     __shared__ double qk_in[1056], qk_out[1056];
-    __shared__ double s_def;
     const double hy = d.dy;
     for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) { qk_in[j] = part[j]; qk_out[j] = part[1056 + j]; }
     __syncthreads();
@@ -828,15 +827,18 @@ __global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __rest
         scal[S_RK + 4 * rk + 1] = qin;
         scal[S_RK + 4 * rk + 2] = qout;
         scal[S_RK + 4 * rk + 3] = deficit;
-        s_def = deficit;
     }
-    __syncthreads();
-    const T def = T(s_def);
-    const long np = (long)d.ny * d.nz;
-    for (long l = threadIdx.x; l < np; l += blockDim.x) {
-        const long o = IX(d, d.nx + 3, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz));
-        u[o] = u[o] + def;
-    }
+}
+
+// the deficit added to the outlet face (2219-2223), one point per thread
+template <typename T>
+__global__ void k_add_deficit(Dims d, T* __restrict__ u, const double* __restrict__ scal, int rk)
+{
+    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y;
+    if (k > d.nz) return;
+    const long o = IX(d, d.nx + 3, j, k);
+    u[o] = u[o] + T(scal[S_RK + 4 * rk + 3]);
 }
 
 // pressure_gradient (2964-3044): pg += grad(phi) - fac A(mu_new) grad(phi)
@@ -953,12 +955,12 @@ __global__ __launch_bounds__(256) void k_bulk_partial(Dims d, const T* __restric
     __shared__ double sv[256];
     __shared__ long si[256];
     const int t = threadIdx.x;
-    const long np = (long)d.ny * d.nz;
-    const long l = (long)blockIdx.x * blockDim.x + t;
+    const int np = d.ny * d.nz;
+    const int l = blockIdx.x * blockDim.x + t;
     double v = 0.0;
     long id = -1;
     if (l < np) {
-        const double x = (double)u[IX(d, d.nx + 2, 1 + (int)(l / d.nz), 1 + (int)(l % d.nz))];
+        const double x = (double)u[IX(d, d.nx + 2, 1 + l / d.nz, 1 + l % d.nz)];
         if (fabs(x) > 0.0) { v = x; id = l; }
     }
     sv[t] = v; si[t] = id;
@@ -1005,11 +1007,12 @@ __global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ p
 // One 16x16x4 MFMA per step (V_MFMA_F64_16X16X4_F64 / V_MFMA_F32_16X16X4_F32):
 // lane l supplies A[l%16][k0 + l/16] and B[k0 + l/16][l%16]; its four
 // results are C[row][l%16] with row = l/16 + 4r for FP64 and 4(l/16) + r for
-// FP32, r = 0..3 (both measured on gfx950 with single-lane probes).  Block tile 64 x 128 (4 wavefronts of
-// 32 x 64, 2 x 4 MFMA tiles each), K chunks of 16 staged in LDS.  Every
-// output accumulates over k in increasing order with fused multiply-adds;
-// FFTW's fast transforms round differently from any direct sum, so the
-// results agree with the reference's REDFT10/01 to rounding.
+// FP32, r = 0..3 (both measured on gfx950 with single-lane probes).  Block
+// tile 64 x 128 (4 wavefronts of 32 x 64, 2 x 4 MFMA tiles each), K chunks of
+// 16 double-buffered in LDS.  Every output accumulates over k in increasing
+// order with fused multiply-adds; FFTW's fast transforms round differently
+// from any direct sum, so the results agree with the reference's REDFT10/01
+// to rounding.
 template <typename T> struct Mma;
 template <> struct Mma<double> {
     typedef double acc __attribute__((ext_vector_type(4)));
@@ -1033,8 +1036,11 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
 {
     constexpr int BM = 64, BN = 128, BK = 16;
-    __shared__ T As[BK][BM + 1];
-    __shared__ T Bs[BK][BN + 1];
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;   // per-thread loads per chunk
+    // two LDS buffers: the next chunk's global loads are in flight (in
+    // registers) while the matrix cores work on the current one
+    __shared__ T As[2][BK][BM + 1];
+    __shared__ T Bs[2][BK][BN + 1];
     typedef typename Mma<T>::acc acc_t;
     const int b = blockIdx.z;
     A += (long)b * sA; B += (long)b * sB; C += (long)b * sC;
@@ -1047,39 +1053,56 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
     for (int s = 0; s < 2; s++)
 #pragma unroll
         for (int t = 0; t < 4; t++) acc[s][t] = acc_t{T(0), T(0), T(0), T(0)};
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        // A tile: consecutive threads on consecutive k of a row
+    T ra[NA], rb[NB];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int p = 0; p < BM * BK / 256; p++) {
+        for (int p = 0; p < NA; p++) {
             const int e = threadIdx.x + 256 * p;
             const int mm = e >> 4, kk = e & 15;
             const int gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
+            ra[p] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
         }
 #pragma unroll
-        for (int p = 0; p < BN * BK / 256; p++) {
+        for (int p = 0; p < NB; p++) {
             const int e = threadIdx.x + 256 * p;
             const int nn = TRB ? (e >> 4) : (e & (BN - 1));
             const int kb = TRB ? (e & 15) : (e / BN);
             const int gn = n0 + nn, gkb = k0 + kb;
             T bv = T(0);
             if (gn < N && gkb < K) bv = TRB ? B[(long)gn * K + gkb] : B[(long)gkb * N + gn];
-            Bs[kb][nn] = bv;
+            rb[p] = bv;
+        }
+    };
+    load(0);
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int p = 0; p < NA; p++) {
+            const int e = threadIdx.x + 256 * p;
+            As[buf][e & 15][e >> 4] = ra[p];
+        }
+#pragma unroll
+        for (int p = 0; p < NB; p++) {
+            const int e = threadIdx.x + 256 * p;
+            const int nn = TRB ? (e >> 4) : (e & (BN - 1));
+            const int kb = TRB ? (e & 15) : (e / BN);
+            Bs[buf][kb][nn] = rb[p];
         }
         __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T av[2], bv[4];
 #pragma unroll
-            for (int s = 0; s < 2; s++) av[s] = As[kk + lk][wm + 16 * s + li];
+            for (int s = 0; s < 2; s++) av[s] = As[buf][kk + lk][wm + 16 * s + li];
 #pragma unroll
-            for (int t = 0; t < 4; t++) bv[t] = Bs[kk + lk][wn + 16 * t + li];
+            for (int t = 0; t < 4; t++) bv[t] = Bs[buf][kk + lk][wn + 16 * t + li];
 #pragma unroll
             for (int s = 0; s < 2; s++)
 #pragma unroll
                 for (int t = 0; t < 4; t++) acc[s][t] = Mma<T>::run(av[s], bv[t], acc[s][t]);
         }
-        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int s = 0; s < 2; s++)
@@ -1530,15 +1553,15 @@ __global__ __launch_bounds__(256) void k_sweep_part(Dims d, LineSets<T> sets, co
     const Rng R = L.R;
     const int n0 = R.ke - R.ks + 1;
     const int n1 = DIR == 0 ? R.je - R.js + 1 : R.ie - R.is + 1;
-    const long l = (long)blockIdx.x * LPB + q;
-    const bool live = l < (long)n0 * n1;
-    const int a0 = live ? (int)(l % n0) : 0, a1 = live ? (int)(l / n0) : 0;
+    const int l = blockIdx.x * LPB + q;
+    const bool live = l < n0 * n1;
+    const int a0 = live ? l % n0 : 0, a1 = live ? l / n0 : 0;
     int i, j, k, n;
     long st;
     if (DIR == 0) { k = R.ks + a0; j = R.js + a1; i = R.is; n = R.ie - R.is + 1; st = d.sx; }
     else { k = R.ks + a0; i = R.is + a1; j = R.js; n = R.je - R.js + 1; st = d.sy; }
-    const int x0 = (int)(((long)s * n) / P);
-    const int m = (int)(((long)(s + 1) * n) / P) - x0;
+    const int x0 = (s * n) / P;
+    const int m = ((s + 1) * n) / P - x0;
     const long o0 = IX(d, i, j, k);
     T* __restrict__ S = L.S + o0;
     const bool cst = L.mu == nullptr;
@@ -1672,8 +1695,8 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     __syncthreads();
 
     const bool live = q < nl;
-    const int x0 = (int)(((long)s * n) / P);
-    const int m = (int)(((long)(s + 1) * n) / P) - x0;
+    const int x0 = (s * n) / P;
+    const int m = ((s + 1) * n) / P - x0;
     PChunk<T> ch;
     if (live) {
         const T* rs = tS + q * w + x0;
@@ -1717,6 +1740,103 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     }
 }
 
+
+// Poisson x-lines (phisolver_dct's pll_tridiag, 2055-2100) by the same
+// partition method: line l = (y mode, z mode) of the packed block, rows x
+// nm = ny nz apart, a = c = 1/dx^2, b = -2/dx^2 + lam_y + lam_z, Neumann
+// rows at the inlet and (all modes but the mean) at the outlet; the (0,0)
+// mode keeps its outlet ghost phi = 0.  The (0,0) line is eliminated in FP64
+// whatever the field type (its own LDS rows), as the gauge line of the
+// otherwise singular problem.
+template <typename U, typename T>
+__device__ __forceinline__ void pois_chunk(const T* __restrict__ p, long nm, int x0, int m, int n, U dx2i, U bmid,
+                                           bool mean_mode, PChunk<U>& ch)
+{
+    U v[PM];
+#pragma unroll
+    for (int t = 0; t < PM; t++)
+        if (t < m) v[t] = U(p[(long)(x0 + t) * nm]);
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        if (t < m) {
+            const int x = x0 + t;
+            U a = dx2i, b = bmid, c = dx2i;
+            if (x == 0) { b += a; a = U(0.0); }
+            if (x == n - 1) {
+                if (!mean_mode) b += c;
+                c = U(0.0);
+            }
+            ch.fwd(t, a, b, c, v[t]);
+        }
+    }
+    ch.fold(m);
+}
+
+template <typename U>
+__device__ __forceinline__ void pchunk_put(U* __restrict__ red, int E, int LPB, int q, int s, const PChunk<U>& ch, int m)
+{
+    U la, lc, ld;
+    ch.last(m, la, lc, ld);
+    const int e0 = 2 * s;
+    red[e0 * LPB + q] = ch.A[0];
+    red[(E + e0) * LPB + q] = ch.C[0];
+    red[(2 * E + e0) * LPB + q] = ch.D[0];
+    red[(e0 + 1) * LPB + q] = la;
+    red[(E + e0 + 1) * LPB + q] = lc;
+    red[(2 * E + e0 + 1) * LPB + q] = ld;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pois_part(Dims d, const T* __restrict__ lam_y, const T* __restrict__ lam_z,
+                                                   T* __restrict__ P, const double* __restrict__ lam00)
+{
+    extern __shared__ __align__(8) unsigned char pllc_smem[];
+    const int LPB = blockDim.x, PC = blockDim.y, E = 2 * PC;
+    double* red0 = reinterpret_cast<double*>(pllc_smem);          // [3][E] of the (0,0) line
+    T* red = reinterpret_cast<T*>(pllc_smem + sizeof(double) * 3 * E);   // [3][E][LPB]
+    const int q = threadIdx.x, s = threadIdx.y;
+    const long nm = (long)d.ny * d.nz;
+    const int n = d.nx;
+    const int l = blockIdx.x * LPB + q;
+    const bool live = l < nm;
+    const bool mean = l == 0;
+    const int x0 = (s * n) / PC;
+    const int m = ((s + 1) * n) / PC - x0;
+    T* __restrict__ p = P + (live ? l : 0);
+    PChunk<T> ch;
+    PChunk<double> ch0;
+    if (live) {
+        if (mean) {
+            const double dx2i = 1.0 / (d.dx * d.dx);
+            pois_chunk<double, T>(p, nm, x0, m, n, dx2i, -2.0 * dx2i + lam00[0] + lam00[1], true, ch0);
+            pchunk_put<double>(red0, E, 1, 0, s, ch0, m);
+        } else {
+            const int jy = l / d.nz, kz = l % d.nz;
+            const T dx2i = T(1.0) / (T(d.dx) * T(d.dx));
+            pois_chunk<T, T>(p, nm, x0, m, n, dx2i, T(-2.0) * dx2i + lam_y[jy] + lam_z[kz], false, ch);
+            pchunk_put<T>(red, E, LPB, q, s, ch, m);
+        }
+    }
+    __syncthreads();
+    if (live && s == 0) {
+        if (mean) pchunk_reduced<double>(red0, E, 1, 0);
+        else pchunk_reduced<T>(red, E, LPB, q);
+    }
+    __syncthreads();
+    if (!live) return;
+    if (mean) {
+        const double X0 = red0[2 * E + 2 * s], X1 = red0[2 * E + 2 * s + 1];
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < m)
+                p[(long)(x0 + t) * nm] = T(t == 0 ? X0 : t == m - 1 ? X1 : ch0.D[t] - ch0.A[t] * X0 - ch0.C[t] * X1);
+    } else {
+        const T X0 = red[(2 * E + 2 * s) * LPB + q], X1 = red[(2 * E + 2 * s + 1) * LPB + q];
+#pragma unroll
+        for (int t = 0; t < PM; t++)
+            if (t < m) p[(long)(x0 + t) * nm] = t == 0 ? X0 : t == m - 1 ? X1 : ch.D[t] - ch.A[t] * X0 - ch.C[t] * X1;
+    }
+}
 
 #ifdef PLLC_F64_TU
 PLLC_INST(double)
@@ -1986,7 +2106,15 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div)
     if (from_div)
         PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
     dct2(c, c.P, c.P, false);
-    PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC, c.lam00)));
+    const int PC = c.line_solver == 0 ? part_chunks(c.d.nx, c.d.nx) : 0;
+    if (PC > 0) {
+        const int LPB = std::min(64, std::max(1, 256 / PC));
+        const size_t sm = sizeof(double) * 3 * 2 * PC + sizeof(T) * 6 * PC * LPB;
+        PLLC_L(c, "pois_x", (k_pois_part<T><<<(unsigned)((nm + LPB - 1) / LPB), dim3(LPB, PC, 1), sm, c.st>>>(
+                                c.d, c.lam_y, c.lam_z, c.P, c.lam00)));
+    } else {
+        PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC, c.lam00)));
+    }
     dct2(c, c.P, c.P, true);
     PLLC_L(c, "phi_scatter", (k_phi_scatter<T><<<grd, blk, 0, c.st>>>(c.d, c.P, c.f[PHI])));
 }
@@ -2007,6 +2135,8 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
                                                                    c.f[W_NEW], c.scal, T(g), T(r))));
     PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
     PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
+    PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,
+                                                                                          slot)));
 }
 
 template <typename T>
