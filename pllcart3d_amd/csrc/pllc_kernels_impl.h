@@ -1542,7 +1542,7 @@ __device__ __forceinline__ void prow(const LineSet<T>& L, int x, int n, T fac, T
 // x- and y-lines (rows strided by sx / sy): block (LPB lines along k,
 // P chunks); a wavefront reads one row of consecutive k, coalesced
 template <typename T, int DIR>
-__global__ __launch_bounds__(256) void k_sweep_part(Dims d, LineSets<T> sets, const double* __restrict__ scal,
+__global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, const double* __restrict__ scal,
                                                     int slot, T g, T r)
 {
     extern __shared__ unsigned char pllc_smem[];
@@ -1686,11 +1686,24 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     const T del = L.del, h = L.h;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
 
-    for (int lq = 0; lq < nl; lq++) {
-        const long gb = base + (long)lq * d.sy;
-        for (int x = tid; x < n; x += NT) tS[lq * w + x] = L.S[gb + x];
-        if (!cst)
-            for (int x = tid; x <= n; x += NT) tM[lq * w + x] = mu[gb + x];
+    // loads of 8 lines are issued before their LDS stores
+    for (int x = tid; x <= n; x += NT) {
+        for (int l0 = 0; l0 < nl; l0 += 8) {
+            T vs[8], vm[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int lq = l0 + u;
+                const long gb = base + (long)lq * d.sy + x;
+                if (lq < nl && x < n) vs[u] = L.S[gb];
+                if (lq < nl && !cst) vm[u] = mu[gb];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int lq = l0 + u;
+                if (lq < nl && x < n) tS[lq * w + x] = vs[u];
+                if (lq < nl && !cst) tM[lq * w + x] = vm[u];
+            }
+        }
     }
     __syncthreads();
 
@@ -1734,12 +1747,18 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
             if (t < m) ws[t] = t == 0 ? X0 : t == m - 1 ? X1 : ch.D[t] - ch.A[t] * X0 - ch.C[t] * X1;
     }
     __syncthreads();
-    for (int lq = 0; lq < nl; lq++) {
-        const long gb = base + (long)lq * d.sy;
-        for (int x = tid; x < n; x += NT) L.S[gb + x] = tS[lq * w + x];
+    for (int x = tid; x < n; x += NT) {
+        for (int l0 = 0; l0 < nl; l0 += 8) {
+            T vs[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (l0 + u < nl) vs[u] = tS[(l0 + u) * w + x];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (l0 + u < nl) L.S[base + (long)(l0 + u) * d.sy + x] = vs[u];
+        }
     }
 }
-
 
 // Poisson x-lines (phisolver_dct's pll_tridiag, 2055-2100) by the same
 // partition method: line l = (y mode, z mode) of the packed block, rows x
@@ -1952,7 +1971,9 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
         }
         const int P = c.line_solver == 0 ? part_chunks(nmin, nmax) : 0;
         if (P > 0 && DIR != 2) {
-            const int LPB = std::min(64, std::max(1, 256 / P));
+            // wide rows per wavefront: rows start 8-byte aligned only, so
+            // longer contiguous runs fetch fewer partial cache lines
+            const int LPB = std::min(sizeof(T) == 8 ? 32 : 64, std::max(1, 512 / P));
             const dim3 blk(LPB, P, 1), grd((unsigned)((lines + LPB - 1) / LPB), nset, 1);
             const size_t sm = (size_t)6 * P * LPB * sizeof(T);
             if (DIR == 0)
