@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 TOUCHES = {
     "rhs_c": 7, "rhs_u": 8, "rhs_v": 8, "rhs_w": 8,
     "sweep_y": None, "sweep_z": None, "sweep_x": None,   # per-field, see below
-    "div": 4, "velo_corr": 7, "pg": 8, "cfl": 1.5, "visc": 5,
+    "div": 4, "velo_corr": 7, "pg": 8, "corr_pg": 14, "cfl": 1.5, "visc": 5,
 }
 # sweeps run for c, u, v, w in that order each substage: y/z (2,3,3,3), x (3,4,4,4)
 SWEEP_TOUCHES = {"sweep_y": 11 / 4.0, "sweep_z": 11 / 4.0, "sweep_x": 15 / 4.0}
@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--ensemble-members", type=int, default=64)
     return ap.parse_args()
 
 
@@ -120,6 +122,25 @@ def hbm_table(kernels, cells, bytes_per_value):
     return out
 
 
+def ensemble_line(a):
+    """The batched ensemble (64 members of three grid sizes, random R and Pe)
+    in the mixed precision: aggregate interior cell-steps per second."""
+    import pllcart3d_amd as P
+    from pllcart3d_amd.configs import ensemble_members
+    mem = ensemble_members(a.ensemble_members)
+    ens = P.Ensemble([m[0] for m in mem], "mixed")
+    for (prm, rec, seed, amp), sim in zip(mem, ens.members):
+        sim.init(P.ic_fill(prm, rec, seed, amp))
+    cells = sum(m[0].nx * m[0].ny * m[0].nz for m in mem)
+    steps = max(2, min(a.steps, 10))
+    ms = ens.time_steps(2, steps)
+    ens.close()
+    return {"value": cells * steps / (ms * 1e-3), "unit": "cell-steps/s", "members": len(mem),
+            "cells": cells, "steps": steps, "ms_per_step": ms / steps, "dtype": "f32/f64",
+            "workload": "ensemble: %d members from {64x32x32, 128x48x48, 192x64x64}, R~U[1,4], "
+                        "Pe log-uniform [200,4000], mixed precision" % len(mem)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -156,6 +177,9 @@ def main():
         kernels = kernel_table(plain, ksteps)
         kernels_m = kernel_table(mixed, ksteps)
         roof = roofline(kernels, cells, 8.0, prm)
+    ens = None
+    if not a.no_ensemble:
+        ens = ensemble_line(a)
     step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
     cpu = None
     if not a.no_cpu:
@@ -176,6 +200,7 @@ def main():
                   "roofline": roofline(kernels_m, cells, 4.0, prm) if kernels_m else None,
                   "kernel_gbs": hbm_table(kernels_m, cells, 4.0) if kernels_m else None},
         "kernel_gbs": hbm_table(kernels, cells, 8.0) if kernels else None,
+        "ensemble": ens,
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
         "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},
     }

@@ -265,6 +265,18 @@ double pllc_sim_op(pllc_sim *s, int op, double dt, int rk, double vbulk);
    vbulk and tval; tval/dt (nullable) receive the values after the last step */
 int pllc_sim_run(pllc_sim *s, int nsteps, double *tval, double *dt);
 double pllc_sim_timestep(const pllc_sim *s);
+/* Ensembles (no counterpart in the reference): n independent members with
+ * their own parameters (p[0..n-1]) and precision, advanced together; each
+ * member is a pllc_sim (upload, download, op, ...) owned by the ensemble. */
+typedef struct pllc_ensemble pllc_ensemble;
+pllc_ensemble *pllc_ensemble_create(const pllc_params *p, int n, int precision);
+void pllc_ensemble_destroy(pllc_ensemble *e);
+int pllc_ensemble_size(const pllc_ensemble *e);
+pllc_sim *pllc_ensemble_member(pllc_ensemble *e, int i);
+/* nsteps fused steps of every member */
+int pllc_ensemble_run(pllc_ensemble *e, int nsteps);
+/* warmup steps, then nsteps timed with HIP events across all members */
+int pllc_ensemble_time_steps(pllc_ensemble *e, int warmup, int nsteps, double *ms);
 /* the wave perturbation (shift_conc) on c_old with Ampli, Lamb, Subharm
  * of param.input */
 int pllc_sim_shift_conc(pllc_sim *s, double Ampli, double Lamb, int Subharm);

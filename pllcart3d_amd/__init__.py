@@ -13,7 +13,7 @@ from ._lib import (FIELDS, FIELD_ID, IC_ERF, IC_MULTIMODE, IC_NOISE, OP_ID, OPS,
                    last_error, load)
 from .configs import CONFIGS, make_params
 
-__all__ = ["Sim", "Params", "make_params", "CONFIGS", "ic_fill", "FIELDS", "FIELD_ID", "OPS",
+__all__ = ["Sim", "Ensemble", "Params", "make_params", "CONFIGS", "ic_fill", "FIELDS", "FIELD_ID", "OPS",
            "OP_ID", "IC_ERF", "IC_NOISE", "IC_MULTIMODE", "load", "last_error"]
 
 
@@ -121,3 +121,58 @@ class Sim:
         self._chk(self.L.pllc_sim_time_kernel(self.h, name.encode(), nsteps, C.byref(avg), C.byref(n)),
                   "pllc_sim_time_kernel")
         return avg.value, n.value
+
+
+class _Member(Sim):
+    """A member of an Ensemble: the ensemble owns (and frees) the handle."""
+
+    def __init__(self, ens, h, params, precision):
+        self.L = ens.L
+        self.params = params
+        self.precision = precision
+        self.h = h
+        self._ens = ens
+        d = (C.c_int * 3)()
+        self.n = self.L.pllc_sim_dims(self.h, d)
+        self.shape = (d[0], d[1], d[2])
+
+    def close(self):
+        self.h = None
+
+
+class Ensemble:
+    """Independent members (own grid, Vratio, Pe, initial state) advanced
+    together: each member replays its own captured two-step graph on its own
+    stream, launched round-robin so the members' kernels overlap."""
+
+    def __init__(self, params_list, precision="plain"):
+        self.L = load()
+        arr = (Params * len(params_list))(*params_list)
+        self.h = self.L.pllc_ensemble_create(arr, len(params_list), PRECISION[precision])
+        if not self.h:
+            raise RuntimeError("pllc_ensemble_create: " + last_error())
+        self.members = [_Member(self, self.L.pllc_ensemble_member(self.h, i), p, precision)
+                        for i, p in enumerate(params_list)]
+
+    def run(self, nsteps):
+        if self.L.pllc_ensemble_run(self.h, nsteps) != 0:
+            raise RuntimeError("pllc_ensemble_run: " + last_error())
+
+    def time_steps(self, warmup, nsteps):
+        ms = C.c_double(0.0)
+        if self.L.pllc_ensemble_time_steps(self.h, warmup, nsteps, C.byref(ms)) != 0:
+            raise RuntimeError("pllc_ensemble_time_steps: " + last_error())
+        return ms.value
+
+    def close(self):
+        if self.h:
+            for m in self.members:
+                m.h = None
+            self.L.pllc_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

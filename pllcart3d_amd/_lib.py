@@ -68,6 +68,14 @@ def load():
     L.pllc_sim_timestep.restype = C.c_double
     L.pllc_sim_timestep.argtypes = [vp]
     L.pllc_sim_shift_conc.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.pllc_ensemble_create.restype = vp
+    L.pllc_ensemble_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int]
+    L.pllc_ensemble_destroy.argtypes = [vp]
+    L.pllc_ensemble_size.argtypes = [vp]
+    L.pllc_ensemble_member.restype = vp
+    L.pllc_ensemble_member.argtypes = [vp, C.c_int]
+    L.pllc_ensemble_run.argtypes = [vp, C.c_int]
+    L.pllc_ensemble_time_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.pllc_sim_slope_parameter.restype = C.c_double
     L.pllc_sim_slope_parameter.argtypes = [vp]
     L.pllc_sim_synchronize.argtypes = [vp]

@@ -31,3 +31,22 @@ def make_params(nx, ny, nz, Vratio=2.0, Pe=500.0, Inf_thick=0.05, Re=1.0, Inf_po
 def config_params(name):
     c = CONFIGS[name]
     return make_params(c["nx"], c["ny"], c["nz"], Vratio=c["Vratio"], Pe=c["Pe"], Inf_thick=c["Inf_thick"])
+
+
+# the batched ensemble: members drawn with numpy's PCG64 from seed 1000 + m:
+# grid uniform from three shapes, R ~ U[1, 4], Pe log-uniform in [200, 4000],
+# a +-0.01 noise front (recipe NOISE, seed 1000 + m); 400 steps each
+ENSEMBLE_SHAPES = [(64, 32, 32), (128, 48, 48), (192, 64, 64)]
+
+
+def ensemble_members(n=64, seed0=1000):
+    """[(params, ic_recipe, seed, amp)] of the first n members."""
+    import numpy as np
+    out = []
+    for m in range(n):
+        rng = np.random.default_rng(seed0 + m)
+        nx, ny, nz = ENSEMBLE_SHAPES[int(rng.integers(0, len(ENSEMBLE_SHAPES)))]
+        R = float(rng.uniform(1.0, 4.0))
+        Pe = float(np.exp(rng.uniform(np.log(200.0), np.log(4000.0))))
+        out.append((make_params(nx, ny, nz, Vratio=R, Pe=Pe, Inf_thick=0.05), IC_NOISE, seed0 + m, 0.01))
+    return out

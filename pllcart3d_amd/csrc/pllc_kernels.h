@@ -78,6 +78,9 @@ struct Ops {
     static void cfl(Ctx<T>& c);
     static void bulk(Ctx<T>& c, int slot, double alpha, bool advance_t);
     static void copy(Ctx<T>& c, const T* a, T* b, long n);
+    // velocity correction, outflow deficit and the pressure-gradient update
+    // of all three components (mu_new) in one pass over the cells
+    static void correction_and_gradient(Ctx<T>& c, double g, double r, int slot);
     // shift_conc on field q: every (j,k) line re-sampled at Xc - shift[k]
     // (x: the NX cell centres, shift: NZ values, both on the device)
     static void shift_conc(Ctx<T>& c, T* q, const double* x, const double* shift);

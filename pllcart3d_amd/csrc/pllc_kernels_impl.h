@@ -868,6 +868,45 @@ __device__ __forceinline__ void pg_update(const Dims& d, const Phys& ph, const V
     pg[o] = pg[o] + g0 - fac * (pyy + pxx + pzz);
 }
 
+// velo_correction (2144-2169) and pressure_gradient (2964-3044) of one
+// substage in one pass: both use the same phi gradient g0 at a velocity
+// point, and pg reads neither u, v nor w, so updating it here, before the
+// outflow deficit and the ghost copies, gives the reference's values.
+template <typename T, int F>
+__device__ __forceinline__ void corr_pg_update(const Dims& d, const Phys& ph, const Visc4<T>& mu,
+                                               const T* __restrict__ phi, T* __restrict__ q, T* __restrict__ pg,
+                                               long o, T fac, T facv)
+{
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
+    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
+    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
+    const T g0 = pgrad<T, F>(d, phi, o);
+    q[o] = q[o] - facv * g0;
+    const T pxx = (ax) * pgrad<T, F>(d, phi, o - d.sx) + (bx) * g0 + (cx) * pgrad<T, F>(d, phi, o + d.sx);
+    const T pyy = (ay) * pgrad<T, F>(d, phi, o - d.sy) + (by) * g0 + (cy) * pgrad<T, F>(d, phi, o + d.sy);
+    const T pzz = (az) * pgrad<T, F>(d, phi, o - 1) + (bz) * g0 + (cz) * pgrad<T, F>(d, phi, o + 1);
+    pg[o] = pg[o] + g0 - fac * (pyy + pxx + pzz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
+                                                 T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                                                 T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
+                                                 const double* __restrict__ scal, T g, T r)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int i = blockIdx.z + 3;
+    if (k > d.nz || j > d.ny) return;
+    const long o = IX(d, i, j, k);
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+    const T facv = T(scal[S_DT]) * (g + r) / T(ph.Re);
+    if (j >= 2) corr_pg_update<T, FV>(d, ph, mu, phi, v, pgv, o, fac, facv);
+    if (k >= 2) corr_pg_update<T, FW>(d, ph, mu, phi, w, pgw, o, fac, facv);
+    if (i >= 4) corr_pg_update<T, FU>(d, ph, mu, phi, u, pgu, o, fac, facv);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
                                             T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
@@ -1872,7 +1911,7 @@ namespace pllc {
 #ifdef PLLC_F64_TU
 const char* kernel_class_names =
     "stale;rhs_c;rhs_u;rhs_v;rhs_w;sweep_y;sweep_z;sweep_x;visc;velo_ghost;div;dct;pois_x;phi_scatter;"
-    "phi_ghost;velo_corr;flux;pg;cfl;bulk;copy";
+    "phi_ghost;velo_corr;flux;pg;corr_pg;cfl;bulk;copy";
 #endif
 
 // launch `expr` inside the timing hook of kernel class `name`
@@ -2154,6 +2193,20 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "velo_corr", (k_velo_corr<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[PHI], c.f[U_NEW], c.f[V_NEW],
                                                                    c.f[W_NEW], c.scal, T(g), T(r))));
+    PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
+    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
+    PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,
+                                                                                          slot)));
+}
+
+template <typename T>
+void Ops<T>::correction_and_gradient(Ctx<T>& c, double g, double r, int slot)
+{
+    dim3 blk;
+    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    PLLC_L(c, "corr_pg", (k_corr_pg<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_NEW], c.f[V_NEW],
+                                                               c.f[W_NEW], c.f[U_PG], c.f[V_PG], c.f[W_PG], c.scal,
+                                                               T(g), T(r))));
     PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
     PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
     PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,

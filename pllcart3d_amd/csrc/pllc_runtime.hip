@@ -679,9 +679,8 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::divergence(c, g, r, false, true);
         Ops<T>::poisson(c, g, r, false);
         Ops<T>::phi_ghost(c);
-        Ops<T>::velo_correction(c, g, r, rk);
+        Ops<T>::correction_and_gradient(c, g, r, rk);   // velo_correction + pressure_gradient
         Ops<T>::velo_ghost(c);
-        Ops<T>::pressure_gradient(c, g, r, 7);
         if (s->copy_next) {
             update_old_copy<T>(s, c);
             s->copy_next = false;
@@ -786,6 +785,98 @@ extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* 
     if (finish(s, "pllc_sim_time_steps")) return -1;
     float f = 0.f;
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventElapsedTime(&f, s->e0, s->e1))) return -1;
+    if (ms) *ms = f;
+    return 0;
+}
+
+// ------------------------------------------------------------ ensembles
+//
+// Independent realisations (own grid, viscosity ratio, Peclet number, seed)
+// advanced together: every member keeps its own stream and captured
+// two-step graph, and the members' graphs are launched round-robin, pair of
+// steps by pair of steps, so the device runs several members' kernels at
+// once (small members do not fill the chip on their own).
+struct pllc_ensemble {
+    std::vector<pllc_sim*> m;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> done;
+};
+
+extern "C" void pllc_ensemble_destroy(pllc_ensemble* e)
+{
+    if (!e) return;
+    for (pllc_sim* s : e->m) pllc_sim_destroy(s);
+    for (hipEvent_t v : e->done) (void)hipEventDestroy(v);
+    if (e->e0) (void)hipEventDestroy(e->e0);
+    if (e->e1) (void)hipEventDestroy(e->e1);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
+extern "C" pllc_ensemble* pllc_ensemble_create(const pllc_params* p, int n, int precision)
+{
+    const char* W = "pllc_ensemble_create";
+    if (!p || n < 1) { pllc_set_error(W, "no members"); return nullptr; }
+    pllc_ensemble* e = new pllc_ensemble();
+    bool ok = pllc_hip_ok(W, hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking)) &&
+              pllc_hip_ok(W, hipEventCreate(&e->e0)) && pllc_hip_ok(W, hipEventCreate(&e->e1));
+    for (int i = 0; ok && i < n; i++) {
+        pllc_sim* s = pllc_sim_create2(&p[i], precision);
+        hipEvent_t v = nullptr;
+        ok = s != nullptr && pllc_hip_ok(W, hipEventCreateWithFlags(&v, hipEventDisableTiming));
+        if (s) e->m.push_back(s);
+        if (v) e->done.push_back(v);
+    }
+    if (!ok) { pllc_ensemble_destroy(e); return nullptr; }
+    return e;
+}
+
+extern "C" int pllc_ensemble_size(const pllc_ensemble* e) { return (int)e->m.size(); }
+
+extern "C" pllc_sim* pllc_ensemble_member(pllc_ensemble* e, int i)
+{
+    if (i < 0 || i >= (int)e->m.size()) { pllc_set_error("pllc_ensemble_member", "no such member"); return nullptr; }
+    return e->m[i];
+}
+
+static void ensemble_steps(pllc_ensemble* e, int nsteps)
+{
+    for (int done = 0; done < nsteps;) {
+        const int k = nsteps - done >= 2 ? 2 : 1;
+        for (pllc_sim* s : e->m) steps(s, k);
+        done += k;
+    }
+}
+
+extern "C" int pllc_ensemble_run(pllc_ensemble* e, int nsteps)
+{
+    ensemble_steps(e, nsteps);
+    for (pllc_sim* s : e->m)
+        if (finish(s, "pllc_ensemble_run")) return -1;
+    return 0;
+}
+
+// HIP-event time of nsteps steps of every member: the member streams start
+// after an event on the ensemble stream, which waits for all of them
+extern "C" int pllc_ensemble_time_steps(pllc_ensemble* e, int warmup, int nsteps, double* ms)
+{
+    const char* W = "pllc_ensemble_time_steps";
+    ensemble_steps(e, warmup);
+    for (pllc_sim* s : e->m)
+        if (finish(s, W)) return -1;
+    if (!pllc_hip_ok(W, hipEventRecord(e->e0, e->st))) return -1;
+    for (pllc_sim* s : e->m)
+        if (!pllc_hip_ok(W, hipStreamWaitEvent(s->st, e->e0, 0))) return -1;
+    ensemble_steps(e, nsteps);
+    for (size_t i = 0; i < e->m.size(); i++) {
+        if (!pllc_hip_ok(W, hipEventRecord(e->done[i], e->m[i]->st))) return -1;
+        if (!pllc_hip_ok(W, hipStreamWaitEvent(e->st, e->done[i], 0))) return -1;
+    }
+    if (!pllc_hip_ok(W, hipEventRecord(e->e1, e->st))) return -1;
+    if (!pllc_hip_ok(W, hipEventSynchronize(e->e1))) return -1;
+    float f = 0.f;
+    if (!pllc_hip_ok(W, hipEventElapsedTime(&f, e->e0, e->e1))) return -1;
     if (ms) *ms = f;
     return 0;
 }
