@@ -154,6 +154,14 @@ def main():
     cells = prm.nx * prm.ny * prm.nz
     c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
 
+    # the sweeps' block shapes are tuned on throwaway simulations first
+    tuned = {}
+    for prec in ("plain", "mixed"):
+        t = P.Sim(prm, prec)
+        t.init(c0)
+        tuned[prec] = t.autotune(3)
+        t.close()
+
     plain = P.Sim(prm, "plain")
     plain.init(c0)
     ms = plain.time_steps(a.warmup, a.steps)
@@ -201,6 +209,7 @@ def main():
                   "kernel_gbs": hbm_table(kernels_m, cells, 4.0) if kernels_m else None},
         "kernel_gbs": hbm_table(kernels, cells, 8.0) if kernels else None,
         "ensemble": ens,
+        "sweep_lines_per_block": {k: {"x": v[0], "y": v[1], "z": v[2]} for k, v in tuned.items()},
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
         "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},
     }

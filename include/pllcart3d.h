@@ -265,6 +265,12 @@ double pllc_sim_op(pllc_sim *s, int op, double dt, int rk, double vbulk);
    vbulk and tval; tval/dt (nullable) receive the values after the last step */
 int pllc_sim_run(pllc_sim *s, int nsteps, double *tval, double *dt);
 double pllc_sim_timestep(const pllc_sim *s);
+/* Autotuner (no counterpart in the reference): times nsteps fused steps per
+ * candidate lines-per-block of the x, y and z sweeps (advancing s), keeps the
+ * fastest (lpb_out[3], nullable) and applies it to simulations of the same
+ * precision and grid created afterwards. */
+int pllc_sim_autotune(pllc_sim *s, int nsteps, int *lpb_out);
+
 /* Ensembles (no counterpart in the reference): n independent members with
  * their own parameters (p[0..n-1]) and precision, advanced together; each
  * member is a pllc_sim (upload, download, op, ...) owned by the ensemble. */

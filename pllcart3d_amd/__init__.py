@@ -98,6 +98,13 @@ class Sim:
     def timestep(self):
         return self.L.pllc_sim_timestep(self.h)
 
+    def autotune(self, nsteps=3):
+        """Pick the sweeps' lines per block by timing steps (advances the
+        state); later simulations of this grid and precision use the result."""
+        out = (C.c_int * 3)()
+        self._chk(self.L.pllc_sim_autotune(self.h, nsteps, out), "pllc_sim_autotune")
+        return tuple(out)
+
     def synchronize(self):
         self._chk(self.L.pllc_sim_synchronize(self.h), "pllc_sim_synchronize")
 

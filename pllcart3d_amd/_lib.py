@@ -68,6 +68,7 @@ def load():
     L.pllc_sim_timestep.restype = C.c_double
     L.pllc_sim_timestep.argtypes = [vp]
     L.pllc_sim_shift_conc.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.pllc_sim_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.pllc_ensemble_create.restype = vp
     L.pllc_ensemble_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int]
     L.pllc_ensemble_destroy.argtypes = [vp]

@@ -51,6 +51,9 @@ struct Ctx {
     // ADI line solver: 0 partitioned chunks (k_sweep_part*), 1 one thread
     // per line with the reference's Thomas arithmetic (k_sweep*)
     int line_solver = 0;
+    // lines per block of the partitioned sweeps along x, y, z (0: the
+    // built-in choice); set by pllc_sim_autotune
+    int lpb[3] = {0, 0, 0};
     // optional per-launch hook (phase 0 before, 1 after a launch of the
     // named kernel class), used for HIP-event timing of one kernel class
     void (*hook)(void* user, const char* name, int phase) = nullptr;

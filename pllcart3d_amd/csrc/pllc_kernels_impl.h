@@ -2012,7 +2012,8 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
         if (P > 0 && DIR != 2) {
             // wide rows per wavefront: rows start 8-byte aligned only, so
             // longer contiguous runs fetch fewer partial cache lines
-            const int LPB = std::min(sizeof(T) == 8 ? 32 : 64, std::max(1, 512 / P));
+            int LPB = std::min(sizeof(T) == 8 ? 32 : 64, std::max(1, 512 / P));
+            if (c.lpb[DIR] > 0) LPB = std::min(c.lpb[DIR], std::max(1, 512 / P));   // tuned
             const dim3 blk(LPB, P, 1), grd((unsigned)((lines + LPB - 1) / LPB), nset, 1);
             const size_t sm = (size_t)6 * P * LPB * sizeof(T);
             if (DIR == 0)
@@ -2032,6 +2033,8 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
             const int w = (nmax + 1) | 1;
             int LPB = 32;
             while (LPB > 1 && ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 40960) LPB /= 2;
+            if (c.lpb[2] > 0) LPB = c.lpb[2];   // tuned
+            while (LPB > 1 && ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 65536) LPB /= 2;
             while (LPB > 1 && LPB * P > 256) LPB /= 2;
             const size_t sm = ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T);
             const dim3 blk(LPB, P, 1), gz((nj + LPB - 1) / LPB, ni, nset);

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -332,6 +335,8 @@ struct pllc_sim {
     std::vector<void*> gtab;
 };
 
+static void apply_tuned(pllc_sim* s);
+
 static void graph_drop(pllc_sim* s)
 {
     if (s->gx) (void)hipGraphExecDestroy(s->gx);
@@ -398,6 +403,7 @@ extern "C" pllc_sim* pllc_sim_create2(const pllc_params* p, int precision)
     s->Timestep = p->Timestep;
     const char* gr = getenv("PLLC_GRAPH");
     s->use_graph = !(gr && strcmp(gr, "0") == 0);
+    apply_tuned(s);
     return s;
 }
 
@@ -786,6 +792,65 @@ extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* 
     float f = 0.f;
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventElapsedTime(&f, s->e0, s->e1))) return -1;
     if (ms) *ms = f;
+    return 0;
+}
+
+// ------------------------------------------------------------ autotuner
+//
+// The partitioned sweeps' lines per block (per direction) are picked by
+// timing whole fused steps of the simulation itself: one direction at a time,
+// each candidate width for `nsteps` steps (HIP events), the fastest kept.
+// The result is remembered per (precision, grid) and applied to every
+// simulation created afterwards with that shape.  Tuning advances the state.
+namespace {
+struct TuneKey {
+    int prec, nx, ny, nz;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(prec, nx, ny, nz) < std::tie(o.prec, o.nx, o.ny, o.nz);
+    }
+};
+std::map<TuneKey, std::array<int, 3>>& tune_cache()
+{
+    static std::map<TuneKey, std::array<int, 3>> m;
+    return m;
+}
+}  // namespace
+
+static void set_lpb(pllc_sim* s, const int* lpb)
+{
+    for (int d = 0; d < 3; d++) { s->cd.lpb[d] = lpb[d]; s->cf.lpb[d] = lpb[d]; }
+    graph_drop(s);   // launch shapes are part of the captured graph
+}
+
+static void apply_tuned(pllc_sim* s)
+{
+    auto it = tune_cache().find(TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz});
+    if (it != tune_cache().end()) set_lpb(s, it->second.data());
+}
+
+extern "C" int pllc_sim_autotune(pllc_sim* s, int nsteps, int* lpb_out)
+{
+    static const int cand[] = {8, 16, 32, 64};
+    int best[3] = {0, 0, 0};
+    set_lpb(s, best);
+    for (int d = 0; d < 3; d++) {
+        double tbest = 1e30;
+        int lbest = 0;
+        for (int L : cand) {
+            int trial[3] = {best[0], best[1], best[2]};
+            trial[d] = L;
+            set_lpb(s, trial);
+            double ms = 0.0;
+            if (pllc_sim_time_steps(s, 2, nsteps, &ms)) return -1;
+            if (ms < tbest) { tbest = ms; lbest = L; }
+        }
+        best[d] = lbest;
+    }
+    set_lpb(s, best);
+    tune_cache()[TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz}] = {best[0], best[1], best[2]};
+    if (lpb_out)
+        for (int d = 0; d < 3; d++) lpb_out[d] = best[d];
     return 0;
 }
 

@@ -213,3 +213,19 @@ def test_ensemble_members_equal_standalone_runs(precision):
             assert np.array_equal(m.download(f), s.download(f)), (precision, i, f)
     print("[ensemble] %s 3 members x 5 steps equal" % precision, flush=True)
     ens.close()
+
+
+def test_autotuned_shapes_track_oracle(oracle_mod):
+    """The autotuner picks the sweeps' lines per block by timing; simulations
+    created afterwards with those shapes still follow the oracle."""
+    shape = dict(nx=64, ny=40, nz=36)
+    prm, _, tmp = _pair(oracle_mod, **shape)
+    lpb = tmp.autotune(1)
+    print("[autotune] lines per block x/y/z", lpb, flush=True)
+    assert all(v in (8, 16, 32, 64) for v in lpb)
+    tmp.close()
+    prm, orc, sim = _pair(oracle_mod, **shape)
+    for n in range(2):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "tuned step %d" % (n + 1))
