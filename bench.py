@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 plain/mixed line")
     ap.add_argument("--ensemble-members", type=int, default=64)
     return ap.parse_args()
 
@@ -141,6 +142,37 @@ def ensemble_line(a):
                         "Pe log-uniform [200,4000], mixed precision" % len(mem)}
 
 
+def large_line(a):
+    """512x256x256 (33.5 M cells, Pe = 4000): plain FP64 and mixed from the
+    same state, and the mixed run's relative L2 error vs the FP64 run."""
+    import numpy as np
+    import pllcart3d_amd as P
+    from pllcart3d_amd.configs import CONFIGS, config_params
+    prm = config_params("cfg3")
+    cfg = CONFIGS["cfg3"]
+    c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
+    cells = prm.nx * prm.ny * prm.nz
+    steps = max(2, min(a.steps, 6))
+    out, sims = {}, {}
+    for prec in ("plain", "mixed"):
+        s = P.Sim(prm, prec)
+        s.init(c0)
+        ms = s.time_steps(2, steps)
+        sims[prec] = s
+        out[prec] = {"value": cells * steps / (ms * 1e-3), "ms_per_step": ms / steps}
+    err = {}
+    for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI"):
+        ref = sims["plain"].download(f)
+        err[f] = float(np.linalg.norm(sims["mixed"].download(f) - ref) / np.linalg.norm(ref))
+    for s in sims.values():
+        s.close()
+    out["workload"] = "cfg3: 512x256x256 box 4x1x1, R=3 Pe=4000, 16x16-mode front"
+    out["steps"] = steps
+    out["speedup_mixed_vs_plain"] = out["plain"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    out["mixed_rel_l2_vs_fp64_after_%d_steps" % (steps + 2)] = err
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -188,6 +220,9 @@ def main():
     ens = None
     if not a.no_ensemble:
         ens = ensemble_line(a)
+    large = None
+    if not a.no_large:
+        large = large_line(a)
     step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
     cpu = None
     if not a.no_cpu:
@@ -209,6 +244,7 @@ def main():
                   "kernel_gbs": hbm_table(kernels_m, cells, 4.0) if kernels_m else None},
         "kernel_gbs": hbm_table(kernels, cells, 8.0) if kernels else None,
         "ensemble": ens,
+        "large": large,
         "sweep_lines_per_block": {k: {"x": v[0], "y": v[1], "z": v[2]} for k, v in tuned.items()},
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
         "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},
