@@ -120,6 +120,7 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
         D.NX = g->nx_max; D.NY = g->ny_max; D.NZ = g->nz_max;
         D.sy = D.NZ; D.sx = (long)D.NY * D.NZ;
         D.dx = g->dx; D.dy = g->dy; D.dz = g->dz;
+        pllc::pllc_dims_recip(D);
         const size_t n = (size_t)D.NX * D.NY * D.NZ;
         for (int f : {pllc::F_SRC, pllc::F_CC}) {
             void* p = nullptr;
@@ -131,6 +132,7 @@ pllc::Ctx<double>* ctx_for(Grid* g, const char* where)
         d.nx = g->nx_cell; d.ny = g->ny_cell; d.nz = g->nz_cell;
     }
     d.c.d.dx = g->dx; d.c.d.dy = g->dy; d.c.d.dz = g->dz;
+    pllc::pllc_dims_recip(d.c.d);
     phys_from_globals(d.c.ph);
     pllc_phys_consts(d.c.ph, d.c.d);
     return &d.c;

@@ -20,7 +20,35 @@ struct Dims {
     int NX, NY, NZ;   // with ghosts
     long sx, sy;      // strides of i and j (k stride is 1)
     double dx, dy, dz;
+    double rdx, rdy, rdz;   // 1/dx, 1/dy, 1/dz rounded once on the host (divh)
 };
+
+// a / h for a divisor fixed on the host, with rh = 1/h correctly rounded
+// there.  FP64: q = a*rh and one fused correction r = a - q*h (exact),
+// q + r*rh, the correctly rounded quotient (Markstein's theorem for a
+// correctly rounded reciprocal), i.e. the bits of the IEEE division a/h in
+// three operations.  FP32 keeps the plain division.
+template <typename T>
+__device__ __forceinline__ T divh(T a, T h, double rh);
+template <>
+__device__ __forceinline__ double divh<double>(double a, double h, double rh)
+{
+    const double q = a * rh;
+    const double r = fma(-q, h, a);
+    return fma(r, rh, q);
+}
+template <>
+__device__ __forceinline__ float divh<float>(float a, float h, double)
+{
+    return a / h;
+}
+
+__host__ inline void pllc_dims_recip(Dims& d)
+{
+    d.rdx = 1.0 / d.dx;
+    d.rdy = 1.0 / d.dy;
+    d.rdz = 1.0 / d.dz;
+}
 
 // a field's unknown range (simulation_setup, pllcart3d.c:405-431)
 struct Rng { int is, ie, js, je, ks, ke; };
@@ -42,6 +70,7 @@ struct Phys {
     double cdel[4][3];
     double edel[3][2];
     double fre[2];
+    double rRe;   // 1/Re (divh)
 };
 
 // fills Phys::cdel/edel/fre from Re, Pe, Fx, Fy and the spacings
@@ -60,6 +89,7 @@ __host__ inline void pllc_phys_consts(Phys& ph, const Dims& d)
     ph.edel[2][1] = 1.0 / (ph.Re * d.dy * d.dz);
     ph.fre[0] = ph.Fx / ph.Re;
     ph.fre[1] = ph.Fy / ph.Re;
+    ph.rRe = 1.0 / ph.Re;
 }
 
 template <typename T>
@@ -140,20 +170,20 @@ __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
 
 // first divided difference D1[m] of a line p[m*st] (e.g. pllcart3d.c:1554)
 template <typename T>
-__device__ __forceinline__ T dd1(const T* __restrict__ p, long st, int m, T h)
+__device__ __forceinline__ T dd1(const T* __restrict__ p, long st, int m, T h, double rh)
 {
-    return (p[(long)(m + 1) * st] - p[(long)m * st]) / h;
+    return divh<T>(p[(long)(m + 1) * st] - p[(long)m * st], h, rh);
 }
 
 // D2[m] over the range [ms, me] a pass writes (1557-1559); an index one
 // outside reads the scratch value an earlier pass left: lo at ms-1, hi at
 // me+1 (computed once per operator call by k_stale).
 template <typename T>
-__device__ __forceinline__ T dd2(const T* __restrict__ p, long st, int m, int ms, int me, T h, T lo, T hi)
+__device__ __forceinline__ T dd2(const T* __restrict__ p, long st, int m, int ms, int me, T h, double rh, T lo, T hi)
 {
     if (m < ms) return lo;
     if (m > me) return hi;
-    return (dd1(p, st, m, h) - dd1(p, st, m - 1, h)) / (T(2.0) * h);
+    return divh<T>(dd1(p, st, m, h, rh) - dd1(p, st, m - 1, h, rh), T(2.0) * h, 0.5 * rh);
 }
 
 // Upwind derivative at point m of a line: ENO2 with the minabs second
@@ -161,18 +191,18 @@ __device__ __forceinline__ T dd2(const T* __restrict__ p, long st, int m, int ms
 // elsewhere (pllcart3d.c:1566-1574 and the matching blocks of every pass).
 template <typename T>
 __device__ __forceinline__ T upwind(const T* __restrict__ p, long st, int m, int ms, int me, int sign,
-                                    T h, T lo, T hi)
+                                    T h, double rh, T lo, T hi)
 {
     if ((m <= ms + 1) || (m >= me - 1)) {
         const int kk = (sign == 1) ? m - 1 : m;
-        const T a = dd2(p, st, kk, ms, me, h, lo, hi);
-        const T b = dd2(p, st, kk + 1, ms, me, h, lo, hi);
+        const T a = dd2(p, st, kk, ms, me, h, rh, lo, hi);
+        const T b = dd2(p, st, kk + 1, ms, me, h, rh, lo, hi);
         const T d2 = (fabs(a) <= fabs(b)) ? a : b;
-        return dd1(p, st, kk, h) + d2 * h * T(sign);
+        return dd1(p, st, kk, h, rh) + d2 * h * T(sign);
     }
     const int kk = m - 3 - (sign == 1 ? 0 : -5);   // (int)(2.5*(sign-1))
-    return weno5(dd1(p, st, kk, h), dd1(p, st, kk + sign, h), dd1(p, st, kk + 2 * sign, h),
-                 dd1(p, st, kk + 3 * sign, h), dd1(p, st, kk + 4 * sign, h));
+    return weno5(dd1(p, st, kk, h, rh), dd1(p, st, kk + sign, h, rh), dd1(p, st, kk + 2 * sign, h, rh),
+                 dd1(p, st, kk + 3 * sign, h, rh), dd1(p, st, kk + 4 * sign, h, rh));
 }
 
 // The six first differences of a line around point m: D[q] = D1 at face
@@ -180,19 +210,20 @@ __device__ __forceinline__ T upwind(const T* __restrict__ p, long st, int m, int
 // Every stencil below reads them at fixed offsets, so each address is a
 // uniform offset from one per-field pointer.
 template <typename T>
-__device__ __forceinline__ void line_d1(const T* __restrict__ pc, long st, T h, T (&D)[6])
+__device__ __forceinline__ void line_d1(const T* __restrict__ pc, long st, T h, double rh, T (&D)[6])
 {
     T v[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) v[q] = pc[(long)(q - 3) * st];
 #pragma unroll
-    for (int q = 0; q < 6; q++) D[q] = (v[q + 1] - v[q]) / h;
+    for (int q = 0; q < 6; q++) D[q] = divh<T>(v[q + 1] - v[q], h, rh);
 }
 
 // upwind() on the differences of line_d1: the same ENO2 / HJ-WENO5 choice
 // and operands (1566-1574), with the faces picked by the upwind side
 template <typename T>
-__device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, int sign, T h, T lo, T hi)
+__device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, int sign, T h, double rh, T lo,
+                                      T hi)
 {
     const bool up = sign == 1;
     if ((m <= ms + 1) || (m >= me - 1)) {
@@ -200,8 +231,8 @@ __device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, in
         const T d1k = up ? D[2] : D[3];
         const T d1m = up ? D[1] : D[2];
         const T d1p = up ? D[3] : D[4];
-        const T a = (kk < ms) ? lo : (kk > me) ? hi : (d1k - d1m) / (T(2.0) * h);
-        const T b = (kk + 1 < ms) ? lo : (kk + 1 > me) ? hi : (d1p - d1k) / (T(2.0) * h);
+        const T a = (kk < ms) ? lo : (kk > me) ? hi : divh<T>(d1k - d1m, T(2.0) * h, 0.5 * rh);
+        const T b = (kk + 1 < ms) ? lo : (kk + 1 > me) ? hi : divh<T>(d1p - d1k, T(2.0) * h, 0.5 * rh);
         const T d2 = (fabs(a) <= fabs(b)) ? a : b;
         return d1k + d2 * h * T(sign);
     }
@@ -221,6 +252,7 @@ template <typename T, int F, int D>
 __device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, long o, T& a, T& b, T& c)
 {
     const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
+    const double rh = D == 0 ? d.rdx : D == 1 ? d.rdy : d.rdz;
     const T del = T(ph.cdel[F][D]);
     const long sx = d.sx, sy = d.sy;
     if (F == FC) {
@@ -229,22 +261,22 @@ __device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4
         return;
     }
     if (F == FU) {
-        if (D == 0) { a = del * mu.cell[o - sx] / h; c = del * mu.cell[o] / h; b = -(a + c); }
-        else if (D == 1) { a = del * mu.xy[o] / h; c = del * mu.xy[o + sy] / h; b = -(a + c); }
-        else { a = del * mu.xz[o] / h; c = del * mu.xz[o + 1] / h; b = -a - c; }
+        if (D == 0) { a = divh<T>(del * mu.cell[o - sx], h, rh); c = divh<T>(del * mu.cell[o], h, rh); b = -(a + c); }
+        else if (D == 1) { a = divh<T>(del * mu.xy[o], h, rh); c = divh<T>(del * mu.xy[o + sy], h, rh); b = -(a + c); }
+        else { a = divh<T>(del * mu.xz[o], h, rh); c = divh<T>(del * mu.xz[o + 1], h, rh); b = -a - c; }
         return;
     }
     if (F == FV) {
-        if (D == 0) { a = del * mu.xy[o] / h; c = del * mu.xy[o + sx] / h; }
-        else if (D == 1) { a = del * mu.cell[o - sy] / h; c = del * mu.cell[o] / h; }
-        else { a = del * mu.yz[o] / h; c = del * mu.yz[o + 1] / h; }
+        if (D == 0) { a = divh<T>(del * mu.xy[o], h, rh); c = divh<T>(del * mu.xy[o + sx], h, rh); }
+        else if (D == 1) { a = divh<T>(del * mu.cell[o - sy], h, rh); c = divh<T>(del * mu.cell[o], h, rh); }
+        else { a = divh<T>(del * mu.yz[o], h, rh); c = divh<T>(del * mu.yz[o + 1], h, rh); }
         b = -a - c;
         return;
     }
     // FW
-    if (D == 0) { a = del * mu.xz[o] / h; c = del * mu.xz[o + sx] / h; }
-    else if (D == 1) { a = del * mu.yz[o] / h; c = del * mu.yz[o + sy] / h; }
-    else { a = del * mu.cell[o - 1] / h; c = del * mu.cell[o] / h; }
+    if (D == 0) { a = divh<T>(del * mu.xz[o], h, rh); c = divh<T>(del * mu.xz[o + sx], h, rh); }
+    else if (D == 1) { a = divh<T>(del * mu.yz[o], h, rh); c = divh<T>(del * mu.yz[o + sy], h, rh); }
+    else { a = divh<T>(del * mu.cell[o - 1], h, rh); c = divh<T>(del * mu.cell[o], h, rh); }
     b = -a - c;
 }
 

@@ -41,11 +41,14 @@ __global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ 
     T v = T(0.0);
     if (w < 2) {
         const int m = w == 0 ? R.js - 1 : R.je + 1;
-        if (m >= R.is && m <= R.ie) v = (dd1(px, d.sx, m, hx) - dd1(px, d.sx, m - 1, hx)) / (T(2.0) * hx);
+        if (m >= R.is && m <= R.ie)
+            v = divh<T>(dd1(px, d.sx, m, hx, d.rdx) - dd1(px, d.sx, m - 1, hx, d.rdx), T(2.0) * hx, 0.5 * d.rdx);
     } else {
         const int m = w == 2 ? R.ks - 1 : R.ke + 1;
-        if (m >= R.js && m <= R.je) v = (dd1(py, d.sy, m, hy) - dd1(py, d.sy, m - 1, hy)) / (T(2.0) * hy);
-        else if (m >= R.is && m <= R.ie) v = (dd1(px, d.sx, m, hx) - dd1(px, d.sx, m - 1, hx)) / (T(2.0) * hx);
+        if (m >= R.js && m <= R.je)
+            v = divh<T>(dd1(py, d.sy, m, hy, d.rdy) - dd1(py, d.sy, m - 1, hy, d.rdy), T(2.0) * hy, 0.5 * d.rdy);
+        else if (m >= R.is && m <= R.ie)
+            v = divh<T>(dd1(px, d.sx, m, hx, d.rdx) - dd1(px, d.sx, m - 1, hx, d.rdx), T(2.0) * hx, 0.5 * d.rdx);
     }
     stale[t] = v;
 }
@@ -71,16 +74,16 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     const T* __restrict__ cc = cold + o;
     T D[6];
     const T uval = T(0.5) * (un[o + d.sx] + un[o]);
-    line_d1(cc, d.sx, T(d.dx), D);
-    const T cxd = upwind_d(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    line_d1(cc, d.sx, T(d.dx), d.rdx, D);
+    const T cxd = upwind_d(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
     const T convx = uval * cxd;
     const T vval = T(0.5) * (vn[o + d.sy] + vn[o]);
-    line_d1(cc, d.sy, T(d.dy), D);
-    const T cyd = upwind_d(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    line_d1(cc, d.sy, T(d.dy), d.rdy, D);
+    const T cyd = upwind_d(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
     const T convy = vval * cyd;
     const T wval = T(0.5) * (wn[o + 1] + wn[o]);
-    line_d1(cc, 1, T(d.dz), D);
-    const T czd = upwind_d(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    line_d1(cc, 1, T(d.dz), d.rdz, D);
+    const T czd = upwind_d(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
     const T nval = -(wval * czd + convx + convy);
     N[o] = nval;
 
@@ -177,16 +180,16 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     const T* __restrict__ qc = q + o;
     T D[6];
     const T vx = mom_vel<T, F, 0>(d, u, v, w, o);
-    line_d1(qc, d.sx, T(d.dx), D);
-    const T dxd = upwind_d(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), T(0.0), T(0.0));
+    line_d1(qc, d.sx, T(d.dx), d.rdx, D);
+    const T dxd = upwind_d(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
     const T convx = vx * dxd;
     const T vy = mom_vel<T, F, 1>(d, u, v, w, o);
-    line_d1(qc, d.sy, T(d.dy), D);
-    const T dyd = upwind_d(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), stale[0], stale[1]);
+    line_d1(qc, d.sy, T(d.dy), d.rdy, D);
+    const T dyd = upwind_d(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
     const T convy = vy * dyd;
     const T vz = mom_vel<T, F, 2>(d, u, v, w, o);
-    line_d1(qc, 1, T(d.dz), D);
-    const T dzd = upwind_d(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), stale[2], stale[3]);
+    line_d1(qc, 1, T(d.dz), d.rdz, D);
+    const T dzd = upwind_d(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
     const T ct = vz * dzd + convx + convy;
     const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, o, ct);
     N[o] = nval;
@@ -204,12 +207,12 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     T bodyforce;
     if (F == FU) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sx]);
-        bodyforce = alpha * (-T(ph.fre[0]) * cval - (pg[o] / Re));
+        bodyforce = alpha * (-T(ph.fre[0]) * cval - divh<T>(pg[o], Re, ph.rRe));
     } else if (F == FV) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sy]);
-        bodyforce = alpha * (-T(ph.fre[1]) * cval - (pg[o] / Re));
+        bodyforce = alpha * (-T(ph.fre[1]) * cval - divh<T>(pg[o], Re, ph.rRe));
     } else {
-        bodyforce = alpha * (-(pg[o] / Re));
+        bodyforce = alpha * (-divh<T>(pg[o], Re, ph.rRe));
     }
     S[o] = dt * (conv + visc + bodyforce);
 }
@@ -240,6 +243,7 @@ struct LineSet {
     const T* mu;          // viscosity of the lower coefficient of row 0 + x*stride, or nullptr
     long mu_off;          // offset of that value from the row's own point
     T del, h;             // coefficient = del*mu/h (A_*mom 1453-1532)
+    double rh;            // 1/h rounded on the host (divh)
     T ca, cb, cc;         // constant coefficients when mu == nullptr (A_conc)
     int bc0, bc1;         // boundary kinds of the first and last row (x: first only)
     Rng R;
@@ -281,6 +285,7 @@ __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const d
     const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + o0 + L.mu_off;
     const T del = L.del, h = L.h;
+    const double rh = L.rh;
     const T dt = T(scal[S_DT]);
     const T fac = T(0.5) * dt * (g + r);
     const T zero = T(0.0);
@@ -290,14 +295,14 @@ __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const d
         // explicit outflow face iend+1 (677-683 and matching lines)
         const long ob = o0 + (long)n * st;
         const T vbulk = T(scal[S_RK + 4 * slot]);
-        const T nb = -vbulk * (L.qold[ob] - L.qold[ob - st]) / T(d.dx);
+        const T nb = divh<T>(-vbulk * (L.qold[ob] - L.qold[ob - st]), T(d.dx), d.rdx);
         L.N[ob] = nb;
         bcor = dt * (g * nb + r * L.Nm1[ob]);
         L.qnew[ob] = L.qold[ob] + bcor;
     }
 
     // ---- forward elimination
-    T araw = cst ? L.ca : del * mu[0] / h;
+    T araw = cst ? L.ca : divh<T>(del * mu[0], h, rh);
     T pc = T(0.0), pr = T(0.0);
     T cs[LB], cm[LB], ns[LB], nm[LB];
 #pragma unroll
@@ -314,7 +319,7 @@ __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const d
         for (int t = 0; t < LB; t++) {
             const int x = x0 + t;
             if (x < n) {
-                const T craw = cst ? L.cc : del * cm[t] / h;
+                const T craw = cst ? L.cc : divh<T>(del * cm[t], h, rh);
                 const T braw = cst ? L.cb : -(araw + craw);
                 const T a = -fac * araw;
                 T b = T(1.0) - fac * braw;
@@ -435,6 +440,7 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
     const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
     const T del = L.del, h = L.h;
+    const double rh = L.rh;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
     // cooperative chunk copy: element e of the chunk = (line e / ZC, row e % ZC)
     constexpr int PER = ZL * ZC / ZL;   // elements per thread per chunk
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
     };
 
     // ---- forward elimination
-    T araw = cst ? L.ca : (mine ? del * mu[base + (long)t * d.sy] / h : T(0.0));
+    T araw = cst ? L.ca : (mine ? divh<T>(del * mu[base + (long)t * d.sy], h, rh) : T(0.0));
     T pc = T(0.0), pr = T(0.0);
     gload(0);
     for (int k0 = 0; k0 < n; k0 += ZC) {
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
             const int m = min(ZC, n - k0);
             for (int kk = 0; kk < m; kk++) {
                 const int x = k0 + kk;
-                const T craw = cst ? L.cc : del * sA[t][kk] / h;
+                const T craw = cst ? L.cc : divh<T>(del * sA[t][kk], h, rh);
                 const T braw = cst ? L.cb : -(araw + craw);
                 const T a = -fac * araw;
                 T b = T(1.0) - fac * braw;
@@ -634,9 +640,9 @@ __global__ __launch_bounds__(256) void k_div(Dims d, Phys ph, const T* __restric
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const long o = IX(d, i, j, k);
-    const T dvx = (u[o + d.sx] - u[o]) / T(d.dx);
-    const T dvy = (v[o + d.sy] - v[o]) / T(d.dy);
-    const T dvz = (w[o + 1] - w[o]) / T(d.dz);
+    const T dvx = divh<T>(u[o + d.sx] - u[o], T(d.dx), d.rdx);
+    const T dvy = divh<T>(v[o + d.sy] - v[o], T(d.dy), d.rdy);
+    const T dvz = divh<T>(w[o + 1] - w[o], T(d.dz), d.rdz);
     const T dv = (dvx + dvy + dvz);
     if (div) div[o] = dv;
     if (P) {
@@ -771,15 +777,15 @@ __global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __r
     const long o = IX(d, i, j, k);
     const T fac = T(scal[S_DT]) * (g + r) / T(ph.Re);
     if (j >= 2) {
-        const T dp = (phi[o] - phi[o - d.sy]) / T(d.dy);
+        const T dp = divh<T>(phi[o] - phi[o - d.sy], T(d.dy), d.rdy);
         v[o] = v[o] - fac * dp;
     }
     if (k >= 2) {
-        const T dp = (phi[o] - phi[o - 1]) / T(d.dz);
+        const T dp = divh<T>(phi[o] - phi[o - 1], T(d.dz), d.rdz);
         w[o] = w[o] - fac * dp;
     }
     if (i >= 4) {
-        const T dp = (phi[o] - phi[o - d.sx]) / T(d.dx);
+        const T dp = divh<T>(phi[o] - phi[o - d.sx], T(d.dx), d.rdx);
         u[o] = u[o] - fac * dp;
     }
 }
@@ -847,9 +853,9 @@ __global__ void k_add_deficit(Dims d, T* __restrict__ u, const double* __restric
 template <typename T, int F>
 __device__ __forceinline__ T pgrad(const Dims& d, const T* __restrict__ phi, long o)
 {
-    if (F == FU) return (phi[o] - phi[o - d.sx]) / T(d.dx);
-    if (F == FV) return (phi[o] - phi[o - d.sy]) / T(d.dy);
-    return (phi[o] - phi[o - 1]) / T(d.dz);
+    if (F == FU) return divh<T>(phi[o] - phi[o - d.sx], T(d.dx), d.rdx);
+    if (F == FV) return divh<T>(phi[o] - phi[o - d.sy], T(d.dy), d.rdy);
+    return divh<T>(phi[o] - phi[o - 1], T(d.dz), d.rdz);
 }
 
 template <typename T, int F>
@@ -1606,6 +1612,7 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
     const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + o0 + L.mu_off;
     const T del = L.del, h = L.h;
+    const double rh = L.rh;
     const T dt = T(scal[S_DT]);
     const T fac = T(0.5) * dt * (g + r);
 
@@ -1617,7 +1624,7 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
             // explicit outflow face iend+1 (677-683 and matching lines)
             const long ob = o0 + (long)n * st;
             const T vbulk = T(scal[S_RK + 4 * slot]);
-            const T nb = -vbulk * (L.qold[ob] - L.qold[ob - st]) / T(d.dx);
+            const T nb = divh<T>(-vbulk * (L.qold[ob] - L.qold[ob - st]), T(d.dx), d.rdx);
             L.N[ob] = nb;
             bcor = dt * (g * nb + r * L.Nm1[ob]);
             L.qnew[ob] = L.qold[ob] + bcor;
@@ -1631,11 +1638,11 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
             for (int t = 0; t <= PM; t++)
                 if (t <= m) mv[t] = mu[(long)(x0 + t) * st];
         }
-        T araw = cst ? L.ca : del * mv[0] / h;
+        T araw = cst ? L.ca : divh<T>(del * mv[0], h, rh);
 #pragma unroll
         for (int t = 0; t < PM; t++) {
             if (t < m) {
-                const T craw = cst ? L.cc : del * mv[t + 1] / h;
+                const T craw = cst ? L.cc : divh<T>(del * mv[t + 1], h, rh);
                 T a, b, c, rhs = sv[t];
                 prow<T, DIR>(L, x0 + t, n, fac, araw, craw, bcor, a, b, c, rhs);
                 ch.fwd(t, a, b, c, rhs);
@@ -1723,6 +1730,7 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
     const T del = L.del, h = L.h;
+    const double rh = L.rh;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
 
     // loads of 8 lines are issued before their LDS stores
@@ -1753,11 +1761,11 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     if (live) {
         const T* rs = tS + q * w + x0;
         const T* rm = tM + q * w + x0;
-        T araw = cst ? L.ca : del * rm[0] / h;
+        T araw = cst ? L.ca : divh<T>(del * rm[0], h, rh);
 #pragma unroll
         for (int t = 0; t < PM; t++) {
             if (t < m) {
-                const T craw = cst ? L.cc : del * rm[t + 1] / h;
+                const T craw = cst ? L.cc : divh<T>(del * rm[t + 1], h, rh);
                 T a, b, c, rhs = rs[t];
                 prow<T, 2>(L, x0 + t, n, fac, araw, craw, T(0.0), a, b, c, rhs);
                 ch.fwd(t, a, b, c, rhs);
@@ -1951,6 +1959,7 @@ static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T
     L.S = S; L.CC = CC; L.R = field_rng(c.d, F);
     const T h = DIR == 0 ? T(c.d.dx) : DIR == 1 ? T(c.d.dy) : T(c.d.dz);
     L.h = h;
+    L.rh = DIR == 0 ? c.d.rdx : DIR == 1 ? c.d.rdy : c.d.rdz;
     if (F == FC) {
         const T del = T(1.0) / (T(c.ph.Pe) * h * h);
         L.mu = nullptr; L.ca = del; L.cc = del;

@@ -358,6 +358,7 @@ void pllc_fill_dims(const pllc_params* p, Dims& d, Phys& ph)
     d.dx = (p->X2 - p->X1) / (double)p->nx;
     d.dy = (p->Y2 - p->Y1) / (double)p->ny;
     d.dz = (p->Z2 - p->Z1) / (double)p->nz;
+    pllc_dims_recip(d);
     ph.Re = p->Re; ph.Pe = p->Pe; ph.M = p->Vratio; ph.Vnorm = 0.0;
     ph.Fx = p->Fx; ph.Fy = p->Fy; ph.CFLmax = p->CFLmax; ph.Timestep = p->Timestep;
     ph.Y1 = p->Y1; ph.Y2 = p->Y2; ph.Z1 = p->Z1; ph.Z2 = p->Z2;
