@@ -29,8 +29,16 @@ TOUCHES = {
     "sweep_y": None, "sweep_z": None, "sweep_x": None,   # per-field, see below
     "div": 4, "velo_corr": 7, "pg": 8, "corr_pg": 14, "cfl": 1.5, "visc": 5,
 }
-# sweeps run for c, u, v, w in that order each substage: y/z (2,3,3,3), x (3,4,4,4)
-SWEEP_TOUCHES = {"sweep_y": 11 / 4.0, "sweep_z": 11 / 4.0, "sweep_x": 15 / 4.0}
+# sweeps per substage: c then u, v, w (one launch per direction for c and one
+# for the three velocity components): y/z touch 2 + 3 + 3 + 3 fields, x
+# 3 + 4 + 4 + 4; per launch that is the substage total over the launches
+SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
+
+
+def touches_per_launch(k, launches_per_step):
+    if k in SWEEP_TOUCHES_PER_SUBSTAGE:
+        return SWEEP_TOUCHES_PER_SUBSTAGE[k] * 3.0 / launches_per_step
+    return TOUCHES.get(k)
 
 
 def parse():
@@ -83,6 +91,18 @@ def kernel_table(sim, ksteps):
 PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
 
 
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated by the copy
+# kernel) of the 256x128x128 workload, from profiles/step_fp64_256x128x128.md
+# and profiles/step_mixed_256x128x128.md (MB, per kernel class; the DCT is
+# the mean of its two contraction kernels)
+MEASURED_TRAFFIC_MB = {
+    8.0: {"sweep_z": 188.2, "sweep_x": 352.6, "sweep_y": 253.9, "dct": 84.8, "corr_pg": 758.1,
+          "rhs_c": 356.9, "rhs_u": 536.1, "rhs_v": 552.9, "rhs_w": 503.7, "div": 170.0, "visc": 250.2},
+    4.0: {"sweep_z": 94.1, "sweep_x": 226.8, "sweep_y": 129.8, "dct": 42.5, "corr_pg": 433.8,
+          "rhs_c": 208.6, "rhs_u": 315.3, "rhs_v": 326.3, "rhs_w": 295.9, "div": 98.6, "visc": 145.9},
+}
+
+
 def dct_flops_per_launch(prm):
     """Dense REDFT contractions: the four launches of a substage are two
     along z (M = nx ny, N = K = nz) and two along y (nx batches of
@@ -91,8 +111,10 @@ def dct_flops_per_launch(prm):
     return (2.0 * nx * ny * nz * nz + 2.0 * nx * ny * ny * nz) / 2.0
 
 
-def roofline(kernels, cells, bytes_per_value, prm, traffic=None):
+def roofline(kernels, cells, bytes_per_value, prm):
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    tmb = MEASURED_TRAFFIC_MB.get(bytes_per_value, {}).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
+    traffic = {dom: tmb * 1e6} if tmb else None
     avg_s = kernels[dom]["avg_ms"] * 1e-3
     if dom == "dct":
         fl = dct_flops_per_launch(prm)
@@ -101,7 +123,7 @@ def roofline(kernels, cells, bytes_per_value, prm, traffic=None):
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": (traffic or {}).get(dom), "kernel": dom,
                 "algorithmic_flops_per_launch": fl, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
-    t = TOUCHES.get(dom) or SWEEP_TOUCHES.get(dom)
+    t = touches_per_launch(dom, kernels[dom]["launches_per_step"])
     if t is None:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom}
@@ -116,7 +138,7 @@ def hbm_table(kernels, cells, bytes_per_value):
     """Achieved algorithmic GB/s of every kernel class with an AB model."""
     out = {}
     for k, v in kernels.items():
-        t = TOUCHES.get(k) or SWEEP_TOUCHES.get(k)
+        t = touches_per_launch(k, v["launches_per_step"])
         if t is None or not v["avg_ms"]:
             continue
         out[k] = round(t * cells * bytes_per_value / (v["avg_ms"] * 1e-3) / 1e9, 1)
