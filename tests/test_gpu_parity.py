@@ -229,3 +229,16 @@ def test_autotuned_shapes_track_oracle(oracle_mod):
         orc.run(1)
         sim.run(1)
         _compare_step(orc, sim, "tuned step %d" % (n + 1))
+
+
+@pytest.mark.parametrize("shape", [(37, 9, 13), (16, 3, 2), (20, 5, 33)])
+def test_ragged_and_thin_grids_track_oracle(oracle_mod, shape):
+    """Sizes that are not multiples of any tile, lines too short for the
+    partitioned solver (nz = 2: the w lines have one row, so those sweeps
+    fall back to one thread per line), odd ny."""
+    nx, ny, nz = shape
+    prm, orc, sim = _pair(oracle_mod, nx=nx, ny=ny, nz=nz)
+    for n in range(3):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "%s step %d" % ("x".join(map(str, shape)), n + 1))
