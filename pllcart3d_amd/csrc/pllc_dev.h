@@ -27,7 +27,7 @@ struct Dims {
 // there.  FP64: q = a*rh and one fused correction r = a - q*h (exact),
 // q + r*rh, the correctly rounded quotient (Markstein's theorem for a
 // correctly rounded reciprocal), i.e. the bits of the IEEE division a/h in
-// three operations.  FP32 keeps the plain division.
+// three operations.  FP32 (the mixed path): a * rh, within one ulp.
 template <typename T>
 __device__ __forceinline__ T divh(T a, T h, double rh);
 template <>
@@ -38,10 +38,29 @@ __device__ __forceinline__ double divh<double>(double a, double h, double rh)
     return fma(r, rh, q);
 }
 template <>
-__device__ __forceinline__ float divh<float>(float a, float h, double)
+__device__ __forceinline__ float divh<float>(float a, float, double rh)
 {
-    return a / h;
+    return a * float(rh);
 }
+
+// 1/b and a/b on data: the IEEE quotient in FP64; in FP32 the hardware
+// reciprocal (v_rcp_f32, 1 ulp) and one multiply.  The FP32 callers' divisors
+// are normal numbers (line pivots near 1, WENO denominators >= 1e-36).
+template <typename T>
+__device__ __forceinline__ T recip(T b) { return T(1.0) / b; }
+template <>
+__device__ __forceinline__ float recip<float>(float b) { return __builtin_amdgcn_rcpf(b); }
+template <typename T>
+__device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// exp and sin of the viscosity laws: libm in FP64, the hardware exp2/sin
+// forms in FP32 (relative error ~1e-7 for the arguments here, |x| < 10)
+__device__ __forceinline__ double vexp(double x) { return exp(x); }
+__device__ __forceinline__ float vexp(float x) { return __expf(x); }
+__device__ __forceinline__ double vsin(double x) { return sin(x); }
+__device__ __forceinline__ float vsin(float x) { return __sinf(x); }
 
 __host__ inline void pllc_dims_recip(Dims& d)
 {
@@ -97,11 +116,11 @@ __device__ __forceinline__ T visc_law(const Phys& ph, T cavg)
 {
     if (ph.nmvp) {
         const T a = T(ph.a);
-        const T eta = ((T(1) + a) * cavg) / (T(1) + a * cavg);
+        const T eta = fdiv<T>((T(1) + a) * cavg, T(1) + a * cavg);
         const T psi = T(ph.psi1) * (T(1) - eta) + T(ph.psi0) * eta;
-        return T(ph.mum) * sin(psi);
+        return T(ph.mum) * vsin(psi);
     }
-    return exp(T(ph.M) * (cavg - T(ph.Vnorm)));
+    return vexp(T(ph.M) * (cavg - T(ph.Vnorm)));
 }
 
 // field kinds (advected quantity of an H operator / ADI update)
@@ -161,11 +180,11 @@ __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
     t2 = T(3.0) * v2 - T(4.0) * v3 + v4;
     T s3 = (con2 * t1 * t1) + (con3 * t2 * t2);
     T epi = T(1.0e-6) * maxd + Tr<T>::tiny;
-    T a1 = T(0.1) / ((s1 + epi) * (s1 + epi));
-    T a2 = T(0.6) / ((s2 + epi) * (s2 + epi));
-    T a3 = T(0.3) / ((s3 + epi) * (s3 + epi));
+    T a1 = fdiv<T>(T(0.1), (s1 + epi) * (s1 + epi));
+    T a2 = fdiv<T>(T(0.6), (s2 + epi) * (s2 + epi));
+    T a3 = fdiv<T>(T(0.3), (s3 + epi) * (s3 + epi));
     T at = a1 + a2 + a3;
-    return (a1 / at) * c1 + (a2 / at) * c2 + (a3 / at) * c3;
+    return fdiv<T>(a1, at) * c1 + fdiv<T>(a2, at) * c2 + fdiv<T>(a3, at) * c3;
 }
 
 // first divided difference D1[m] of a line p[m*st] (e.g. pllcart3d.c:1554)

@@ -55,6 +55,23 @@ __global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ 
 
 // ------------------------------------------------- explicit terms + source
 
+// XCD-aware tile order of the cell grids (blocks of 64 k x 4 j, grid (k
+// tiles, j tiles, planes)).  Workgroups go to the 8 XCDs round-robin by
+// linear id and each XCD has its own L2.  In dispatch order a plane's
+// consecutive j tiles land on different XCDs, so every XCD fetches the
+// y-stencil halo rows of its tiles itself.  This hands XCD x the x-th eighth
+// of each plane's tiles in (k, j) order instead, a band of consecutive j
+// rows, whose halo rows its L2 shares; planes keep their dispatch order, so
+// the x-stencil planes stay in the same L2 as before.  A bijection of the
+// plane's tiles (the identity when their count is not a multiple of 8).
+__device__ __forceinline__ int2 xcd_tile()
+{
+    const int gx = gridDim.x, nt = gx * gridDim.y;
+    int r = blockIdx.x + gx * blockIdx.y;
+    if ((nt & 7) == 0) r = (r & 7) * (nt >> 3) + (r >> 3);
+    return make_int2(r % gx, r / gx);
+}
+
 // concentration's explicit part: Hc_op (1539-1644) and the source loop
 // (573-590).  One thread per interior cell.
 template <typename T>
@@ -63,8 +80,9 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
                                                const T* __restrict__ Nm1, T* __restrict__ N, T* __restrict__ S,
                                                const T* __restrict__ stale, const double* __restrict__ scal, T g, T r)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const Rng R = field_rng(d, FC);
@@ -168,8 +186,9 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
                                                  const double* __restrict__ scal, T g, T r)
 {
     const Rng R = field_rng(d, F);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + R.ks;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + R.js;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + R.ks;
+    const int j = bt.y * blockDim.y + threadIdx.y + R.js;
     const int i = blockIdx.z + R.is;
     if (k > R.ke || j > R.je) return;
     const T* __restrict__ q = F == FU ? u : F == FV ? v : w;
@@ -635,8 +654,9 @@ __global__ __launch_bounds__(256) void k_div(Dims d, Phys ph, const T* __restric
                                              const T* __restrict__ w, T* __restrict__ div, T* __restrict__ P,
                                              const double* __restrict__ scal, T g, T r)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const long o = IX(d, i, j, k);
@@ -656,8 +676,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_pack_div(Dims d, Phys ph, const T* __restrict__ div, T* __restrict__ P,
                                                   const double* __restrict__ scal, T g, T r)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const T fac = T(ph.Re) / (T(scal[S_DT]) * (g + r));
@@ -755,12 +776,13 @@ __global__ __launch_bounds__(256) void k_pois_x(Dims d, const T* __restrict__ la
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi_scatter(Dims d, const T* __restrict__ H, T* __restrict__ phi)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const T norm = T(4 * d.ny * d.nz);
-    phi[IX(d, i, j, k)] = H[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)] / norm;
+    phi[IX(d, i, j, k)] = fdiv<T>(H[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)], norm);
 }
 
 // velo_correction (2144-2169): u -= dt*alpha/Re * grad(phi) on each
@@ -770,8 +792,9 @@ __global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __r
                                                    T* __restrict__ v, T* __restrict__ w,
                                                    const double* __restrict__ scal, T g, T r)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const long o = IX(d, i, j, k);
@@ -901,8 +924,9 @@ __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, c
                                                  T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
                                                  const double* __restrict__ scal, T g, T r)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const long o = IX(d, i, j, k);
@@ -918,8 +942,9 @@ __global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const 
                                             T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
                                             const double* __restrict__ scal, T g, T r, int mask)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
@@ -1475,6 +1500,18 @@ __global__ void k_spline_lines(int N, long nlines, int nk, long sj, long sk, lon
     for (int ii = 0; ii < N; ii++) Y[(long)ii * rs] = C[(long)ii * rs];
 }
 
+// XCD-aware order of the line blocks: with the block count of the (x, y)
+// grid a multiple of 8, XCD x takes the x-th eighth of the blocks in (x, y)
+// order instead of every eighth block, so blocks whose rows share a cache
+// line at their edges (rows start at any 8-byte offset) run in one L2.
+__device__ __forceinline__ int2 xcd_block_xy()
+{
+    const int gx = gridDim.x, nt = gx * gridDim.y;
+    int r = blockIdx.x + gx * blockIdx.y;
+    if ((nt & 7) == 0) r = (r & 7) * (nt >> 3) + (r >> 3);
+    return make_int2(r % gx, r / gx);
+}
+
 // ------------------------------------------------ partitioned line solve
 //
 // The same systems as k_sweep, cut into P chunks of at most PM rows per
@@ -1502,10 +1539,10 @@ struct PChunk {
     __device__ __forceinline__ void fwd(int t, T a, T b, T c, T d)
     {
         if (t < 2) {
-            const T rb = T(1.0) / b;
+            const T rb = recip<T>(b);
             D[t] = d * rb; A[t] = a * rb; C[t] = c * rb;
         } else {
-            const T rb = T(1.0) / (b - a * C[t - 1]);
+            const T rb = recip<T>(b - a * C[t - 1]);
             D[t] = rb * (d - a * D[t - 1]);
             A[t] = -rb * a * A[t - 1];
             C[t] = rb * c;
@@ -1522,7 +1559,7 @@ struct PChunk {
                 C[t] = -C[t] * C[t + 1];
             }
         }
-        const T rb = T(1.0) / (T(1.0) - C[0] * A[1]);
+        const T rb = recip<T>(T(1.0) - C[0] * A[1]);
         D[0] = rb * (D[0] - C[0] * D[1]);
         A[0] = rb * A[0];
         C[0] = -rb * C[0] * C[1];
@@ -1550,7 +1587,7 @@ __device__ __forceinline__ void pchunk_reduced(T* __restrict__ red, int E, int L
     for (int e = 1; e < E; e++) {
         const int o = e * LPB + q;
         const T a = rA[o];
-        const T rb = T(1.0) / (T(1.0) - a * cp);
+        const T rb = recip<T>(T(1.0) - a * cp);
         cp = rC[o] * rb;
         dp = (rD[o] - a * dp) * rb;
         rC[o] = cp;
@@ -1598,7 +1635,9 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
     const Rng R = L.R;
     const int n0 = R.ke - R.ks + 1;
     const int n1 = DIR == 0 ? R.je - R.js + 1 : R.ie - R.is + 1;
-    const int l = blockIdx.x * LPB + q;
+    // blocks of one line set: grid x is a multiple of 8 (XCD order within the set)
+    const int bx = (gridDim.x & 7) == 0 ? (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int l = bx * LPB + q;
     const bool live = l < n0 * n1;
     const int a0 = live ? l % n0 : 0, a1 = live ? l / n0 : 0;
     int i, j, k, n;
@@ -1718,15 +1757,16 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     const LineSet<T>& L = sets.s[blockIdx.z];
     const Rng R = L.R;
     const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
-    const int i = R.is + blockIdx.y;
-    const int nl = (i <= R.ie) ? min(LPB, nj - (int)(blockIdx.x * LPB)) : 0;
+    const int2 bt = xcd_block_xy();   // (j block, plane)
+    const int i = R.is + bt.y;
+    const int nl = (i <= R.ie) ? min(LPB, nj - (int)(bt.x * LPB)) : 0;
     if (nl <= 0) return;   // whole block
     // tiles tS[LPB][w], tM[LPB][w] (w >= n + 1: rows and the mu row above
     // the last), then the reduced rows
     T* tS = reinterpret_cast<T*>(pllc_smem);
     T* tM = tS + LPB * w;
     T* red = tM + LPB * w;
-    const long base = IX(d, i, R.js + blockIdx.x * LPB, R.ks);
+    const long base = IX(d, i, R.js + bt.x * LPB, R.ks);
     const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
     const T del = L.del, h = L.h;
@@ -2023,7 +2063,9 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
             // longer contiguous runs fetch fewer partial cache lines
             int LPB = std::min(sizeof(T) == 8 ? 32 : 64, std::max(1, 512 / P));
             if (c.lpb[DIR] > 0) LPB = std::min(c.lpb[DIR], std::max(1, 512 / P));   // tuned
-            const dim3 blk(LPB, P, 1), grd((unsigned)((lines + LPB - 1) / LPB), nset, 1);
+            // block count rounded up to a multiple of 8 (XCD order); the
+            // blocks past the last line do nothing
+            const dim3 blk(LPB, P, 1), grd((unsigned)(((lines + LPB - 1) / LPB + 7) / 8 * 8), nset, 1);
             const size_t sm = (size_t)6 * P * LPB * sizeof(T);
             if (DIR == 0)
                 PLLC_L(c, nm[0], (k_sweep_part<T, 0><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r)));
