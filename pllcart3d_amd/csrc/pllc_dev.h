@@ -55,6 +55,25 @@ __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
 template <>
 __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// 1/b for the pivots of the partitioned line solves, whose elimination order
+// already differs from the reference's Thomas by rounding: in FP64 the
+// hardware estimate (v_rcp_f64, about 2^-29 relative) refined by two Newton
+// steps, within an ulp or so of 1/b, in 5 dependent operations instead of
+// the ~10 of the IEEE division; in FP32 v_rcp_f32.  The pivots (near 1 for
+// the ADI lines, ~1/dx^2 for the Poisson lines) are normal numbers far from
+// the overflow and denormal ranges, where the estimate needs no scaling.
+template <typename T>
+__device__ __forceinline__ T prcp(T b);
+template <>
+__device__ __forceinline__ double prcp<double>(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return fma(r, fma(-b, r, 1.0), r);
+}
+template <>
+__device__ __forceinline__ float prcp<float>(float b) { return __builtin_amdgcn_rcpf(b); }
+
 // exp and sin of the viscosity laws: libm in FP64, the hardware exp2/sin
 // forms in FP32 (relative error ~1e-7 for the arguments here, |x| < 10)
 __device__ __forceinline__ double vexp(double x) { return exp(x); }

@@ -1539,10 +1539,10 @@ struct PChunk {
     __device__ __forceinline__ void fwd(int t, T a, T b, T c, T d)
     {
         if (t < 2) {
-            const T rb = recip<T>(b);
+            const T rb = prcp<T>(b);
             D[t] = d * rb; A[t] = a * rb; C[t] = c * rb;
         } else {
-            const T rb = recip<T>(b - a * C[t - 1]);
+            const T rb = prcp<T>(b - a * C[t - 1]);
             D[t] = rb * (d - a * D[t - 1]);
             A[t] = -rb * a * A[t - 1];
             C[t] = rb * c;
@@ -1559,7 +1559,7 @@ struct PChunk {
                 C[t] = -C[t] * C[t + 1];
             }
         }
-        const T rb = recip<T>(T(1.0) - C[0] * A[1]);
+        const T rb = prcp<T>(T(1.0) - C[0] * A[1]);
         D[0] = rb * (D[0] - C[0] * D[1]);
         A[0] = rb * A[0];
         C[0] = -rb * C[0] * C[1];
@@ -1587,7 +1587,7 @@ __device__ __forceinline__ void pchunk_reduced(T* __restrict__ red, int E, int L
     for (int e = 1; e < E; e++) {
         const int o = e * LPB + q;
         const T a = rA[o];
-        const T rb = recip<T>(T(1.0) - a * cp);
+        const T rb = prcp<T>(T(1.0) - a * cp);
         cp = rC[o] * rb;
         dp = (rD[o] - a * dp) * rb;
         rC[o] = cp;
@@ -1763,35 +1763,41 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
     if (nl <= 0) return;   // whole block
     // tiles tS[LPB][w], tM[LPB][w] (w >= n + 1: rows and the mu row above
     // the last), then the reduced rows
+    // constant-coefficient sets have no mu tile (the launch sizes the LDS
+    // for a mu tile when any of its sets has one)
+    const bool cst = L.mu == nullptr;
     T* tS = reinterpret_cast<T*>(pllc_smem);
     T* tM = tS + LPB * w;
-    T* red = tM + LPB * w;
+    T* red = cst ? tM : tM + LPB * w;
     const long base = IX(d, i, R.js + bt.x * LPB, R.ks);
-    const bool cst = L.mu == nullptr;
     const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
     const T del = L.del, h = L.h;
     const double rh = L.rh;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
 
-    // loads of 8 lines are issued before their LDS stores
-    for (int x = tid; x <= n; x += NT) {
+    // the mu row above the last (x = n) of each line, issued first; then
+    // rows x < n, the loads of 8 lines before their LDS stores
+    T mtop = T(0.0);
+    if (!cst && tid < nl) mtop = mu[base + (long)tid * d.sy + n];
+    for (int x = tid; x < n; x += NT) {
         for (int l0 = 0; l0 < nl; l0 += 8) {
             T vs[8], vm[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int lq = l0 + u;
                 const long gb = base + (long)lq * d.sy + x;
-                if (lq < nl && x < n) vs[u] = L.S[gb];
+                if (lq < nl) vs[u] = L.S[gb];
                 if (lq < nl && !cst) vm[u] = mu[gb];
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const int lq = l0 + u;
-                if (lq < nl && x < n) tS[lq * w + x] = vs[u];
+                if (lq < nl) tS[lq * w + x] = vs[u];
                 if (lq < nl && !cst) tM[lq * w + x] = vm[u];
             }
         }
     }
+    if (!cst && tid < nl) tM[tid * w + n] = mtop;
     __syncthreads();
 
     const bool live = q < nl;
@@ -2081,13 +2087,17 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
         }
         if (P > 0) {
             // z-lines: LPB lines of an i-plane per block, tiles of pitch w
+            // (S, and mu when a set has variable coefficients)
             const int w = (nmax + 1) | 1;
+            bool any_mu = false;
+            for (int q = 0; q < nset; q++) any_mu = any_mu || ls.s[q].mu != nullptr;
+            const size_t nt = any_mu ? 2 : 1;
             int LPB = 32;
-            while (LPB > 1 && ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 40960) LPB /= 2;
+            while (LPB > 1 && (nt * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 40960) LPB /= 2;
             if (c.lpb[2] > 0) LPB = c.lpb[2];   // tuned
-            while (LPB > 1 && ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 65536) LPB /= 2;
+            while (LPB > 1 && (nt * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 65536) LPB /= 2;
             while (LPB > 1 && LPB * P > 256) LPB /= 2;
-            const size_t sm = ((size_t)2 * LPB * w + (size_t)6 * P * LPB) * sizeof(T);
+            const size_t sm = (nt * LPB * w + (size_t)6 * P * LPB) * sizeof(T);
             const dim3 blk(LPB, P, 1), gz((nj + LPB - 1) / LPB, ni, nset);
             PLLC_L(c, nm[2], (k_sweep_part_z<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w)));
             continue;
