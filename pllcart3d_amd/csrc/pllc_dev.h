@@ -43,6 +43,15 @@ __device__ __forceinline__ float divh<float>(float a, float, double rh)
     return a * float(rh);
 }
 
+// a / h in the explicit stencils.  Reference arithmetic (FAST false): divh,
+// the IEEE quotient.  FAST: a * (1/h), within an ulp, one operation.
+template <typename T, bool FAST>
+__device__ __forceinline__ T sdh(T a, T h, double rh)
+{
+    if (FAST) return a * T(rh);
+    return divh<T>(a, h, rh);
+}
+
 // 1/b and a/b on data: the IEEE quotient in FP64; in FP32 the hardware
 // reciprocal (v_rcp_f32, 1 ulp) and one multiply.  The FP32 callers' divisors
 // are normal numbers (line pivots near 1, WENO denominators >= 1e-36).
@@ -176,7 +185,12 @@ template <> struct Tr<float> { static constexpr float tiny = 1.0e-18f; };
 // ---------------------------------------------------------------- WENO
 
 // calculate_weno (pllcart3d.c:3455-3507) on five divided differences.
-template <typename T>
+// Reference arithmetic (FAST false): its six divisions, as IEEE quotients.
+// FAST: the weights' reciprocals from prcp and one normalisation,
+// w1 c1 + w2 c2 + w3 c3 = (a1 c1 + a2 c2 + a3 c3) / (a1 + a2 + a3): the
+// same value to a few ulps in 4 reciprocals instead of 6 divisions.  The
+// denominators lie in [1e-198, 1e30] (epi >= 1e-99), normal numbers.
+template <typename T, bool FAST = false>
 __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
 {
     const T con1 = T(1.0) / T(6.0), con2 = T(13.0) / T(12.0), con3 = T(1.0) / T(4.0);
@@ -199,6 +213,12 @@ __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
     t2 = T(3.0) * v2 - T(4.0) * v3 + v4;
     T s3 = (con2 * t1 * t1) + (con3 * t2 * t2);
     T epi = T(1.0e-6) * maxd + Tr<T>::tiny;
+    if (FAST) {
+        const T a1 = T(0.1) * prcp<T>((s1 + epi) * (s1 + epi));
+        const T a2 = T(0.6) * prcp<T>((s2 + epi) * (s2 + epi));
+        const T a3 = T(0.3) * prcp<T>((s3 + epi) * (s3 + epi));
+        return (a1 * c1 + a2 * c2 + a3 * c3) * prcp<T>(a1 + a2 + a3);
+    }
     T a1 = fdiv<T>(T(0.1), (s1 + epi) * (s1 + epi));
     T a2 = fdiv<T>(T(0.6), (s2 + epi) * (s2 + epi));
     T a3 = fdiv<T>(T(0.3), (s3 + epi) * (s3 + epi));
@@ -247,19 +267,19 @@ __device__ __forceinline__ T upwind(const T* __restrict__ p, long st, int m, int
 // m-3+q, q = 0..5, from the seven values pc[-3 st .. 3 st] (pc points at m).
 // Every stencil below reads them at fixed offsets, so each address is a
 // uniform offset from one per-field pointer.
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ void line_d1(const T* __restrict__ pc, long st, T h, double rh, T (&D)[6])
 {
     T v[7];
 #pragma unroll
     for (int q = 0; q < 7; q++) v[q] = pc[(long)(q - 3) * st];
 #pragma unroll
-    for (int q = 0; q < 6; q++) D[q] = divh<T>(v[q + 1] - v[q], h, rh);
+    for (int q = 0; q < 6; q++) D[q] = sdh<T, FAST>(v[q + 1] - v[q], h, rh);
 }
 
 // upwind() on the differences of line_d1: the same ENO2 / HJ-WENO5 choice
 // and operands (1566-1574), with the faces picked by the upwind side
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, int sign, T h, double rh, T lo,
                                       T hi)
 {
@@ -269,12 +289,13 @@ __device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, in
         const T d1k = up ? D[2] : D[3];
         const T d1m = up ? D[1] : D[2];
         const T d1p = up ? D[3] : D[4];
-        const T a = (kk < ms) ? lo : (kk > me) ? hi : divh<T>(d1k - d1m, T(2.0) * h, 0.5 * rh);
-        const T b = (kk + 1 < ms) ? lo : (kk + 1 > me) ? hi : divh<T>(d1p - d1k, T(2.0) * h, 0.5 * rh);
+        const T a = (kk < ms) ? lo : (kk > me) ? hi : sdh<T, FAST>(d1k - d1m, T(2.0) * h, 0.5 * rh);
+        const T b = (kk + 1 < ms) ? lo : (kk + 1 > me) ? hi : sdh<T, FAST>(d1p - d1k, T(2.0) * h, 0.5 * rh);
         const T d2 = (fabs(a) <= fabs(b)) ? a : b;
         return d1k + d2 * h * T(sign);
     }
-    return weno5(up ? D[0] : D[5], up ? D[1] : D[4], up ? D[2] : D[3], up ? D[3] : D[2], up ? D[4] : D[1]);
+    return weno5<T, FAST>(up ? D[0] : D[5], up ? D[1] : D[4], up ? D[2] : D[3], up ? D[3] : D[2],
+                          up ? D[4] : D[1]);
 }
 
 // ------------------------------------------------ diffusion coefficients
@@ -286,7 +307,7 @@ __device__ __forceinline__ T upwind_d(const T (&D)[6], int m, int ms, int me, in
 template <typename T>
 struct Visc4 { const T* cell; const T* xy; const T* xz; const T* yz; };
 
-template <typename T, int F, int D>
+template <typename T, int F, int D, bool FAST = false>
 __device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4<T>& mu, long o, T& a, T& b, T& c)
 {
     const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
@@ -299,22 +320,22 @@ __device__ __forceinline__ void coefs(const Dims& d, const Phys& ph, const Visc4
         return;
     }
     if (F == FU) {
-        if (D == 0) { a = divh<T>(del * mu.cell[o - sx], h, rh); c = divh<T>(del * mu.cell[o], h, rh); b = -(a + c); }
-        else if (D == 1) { a = divh<T>(del * mu.xy[o], h, rh); c = divh<T>(del * mu.xy[o + sy], h, rh); b = -(a + c); }
-        else { a = divh<T>(del * mu.xz[o], h, rh); c = divh<T>(del * mu.xz[o + 1], h, rh); b = -a - c; }
+        if (D == 0) { a = sdh<T, FAST>(del * mu.cell[o - sx], h, rh); c = sdh<T, FAST>(del * mu.cell[o], h, rh); b = -(a + c); }
+        else if (D == 1) { a = sdh<T, FAST>(del * mu.xy[o], h, rh); c = sdh<T, FAST>(del * mu.xy[o + sy], h, rh); b = -(a + c); }
+        else { a = sdh<T, FAST>(del * mu.xz[o], h, rh); c = sdh<T, FAST>(del * mu.xz[o + 1], h, rh); b = -a - c; }
         return;
     }
     if (F == FV) {
-        if (D == 0) { a = divh<T>(del * mu.xy[o], h, rh); c = divh<T>(del * mu.xy[o + sx], h, rh); }
-        else if (D == 1) { a = divh<T>(del * mu.cell[o - sy], h, rh); c = divh<T>(del * mu.cell[o], h, rh); }
-        else { a = divh<T>(del * mu.yz[o], h, rh); c = divh<T>(del * mu.yz[o + 1], h, rh); }
+        if (D == 0) { a = sdh<T, FAST>(del * mu.xy[o], h, rh); c = sdh<T, FAST>(del * mu.xy[o + sx], h, rh); }
+        else if (D == 1) { a = sdh<T, FAST>(del * mu.cell[o - sy], h, rh); c = sdh<T, FAST>(del * mu.cell[o], h, rh); }
+        else { a = sdh<T, FAST>(del * mu.yz[o], h, rh); c = sdh<T, FAST>(del * mu.yz[o + 1], h, rh); }
         b = -a - c;
         return;
     }
     // FW
-    if (D == 0) { a = divh<T>(del * mu.xz[o], h, rh); c = divh<T>(del * mu.xz[o + sx], h, rh); }
-    else if (D == 1) { a = divh<T>(del * mu.yz[o], h, rh); c = divh<T>(del * mu.yz[o + sy], h, rh); }
-    else { a = divh<T>(del * mu.cell[o - 1], h, rh); c = divh<T>(del * mu.cell[o], h, rh); }
+    if (D == 0) { a = sdh<T, FAST>(del * mu.xz[o], h, rh); c = sdh<T, FAST>(del * mu.xz[o + sx], h, rh); }
+    else if (D == 1) { a = sdh<T, FAST>(del * mu.yz[o], h, rh); c = sdh<T, FAST>(del * mu.yz[o + sy], h, rh); }
+    else { a = sdh<T, FAST>(del * mu.cell[o - 1], h, rh); c = sdh<T, FAST>(del * mu.cell[o], h, rh); }
     b = -a - c;
 }
 

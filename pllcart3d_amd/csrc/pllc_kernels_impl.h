@@ -74,7 +74,7 @@ __device__ __forceinline__ int2 xcd_tile()
 
 // concentration's explicit part: Hc_op (1539-1644) and the source loop
 // (573-590).  One thread per interior cell.
-template <typename T>
+template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restrict__ cold, const T* __restrict__ un,
                                                const T* __restrict__ vn, const T* __restrict__ wn,
                                                const T* __restrict__ Nm1, T* __restrict__ N, T* __restrict__ S,
@@ -92,24 +92,24 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     const T* __restrict__ cc = cold + o;
     T D[6];
     const T uval = T(0.5) * (un[o + d.sx] + un[o]);
-    line_d1(cc, d.sx, T(d.dx), d.rdx, D);
-    const T cxd = upwind_d(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
+    line_d1<T, FAST>(cc, d.sx, T(d.dx), d.rdx, D);
+    const T cxd = upwind_d<T, FAST>(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
     const T convx = uval * cxd;
     const T vval = T(0.5) * (vn[o + d.sy] + vn[o]);
-    line_d1(cc, d.sy, T(d.dy), d.rdy, D);
-    const T cyd = upwind_d(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
+    line_d1<T, FAST>(cc, d.sy, T(d.dy), d.rdy, D);
+    const T cyd = upwind_d<T, FAST>(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
     const T convy = vval * cyd;
     const T wval = T(0.5) * (wn[o + 1] + wn[o]);
-    line_d1(cc, 1, T(d.dz), d.rdz, D);
-    const T czd = upwind_d(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
+    line_d1<T, FAST>(cc, 1, T(d.dz), d.rdz, D);
+    const T czd = upwind_d<T, FAST>(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
     const T nval = -(wval * czd + convx + convy);
     N[o] = nval;
 
     T ax, bx, cx, ay, by, cy, az, bz, cz;
     Visc4<T> none{nullptr, nullptr, nullptr, nullptr};
-    coefs<T, FC, 0>(d, ph, none, o, ax, bx, cx);
-    coefs<T, FC, 1>(d, ph, none, o, ay, by, cy);
-    coefs<T, FC, 2>(d, ph, none, o, az, bz, cz);
+    coefs<T, FC, 0, FAST>(d, ph, none, o, ax, bx, cx);
+    coefs<T, FC, 1, FAST>(d, ph, none, o, ay, by, cy);
+    coefs<T, FC, 2, FAST>(d, ph, none, o, az, bz, cz);
     const T conv = (g * nval + r * Nm1[o]);
     const T axx = ax * cold[o - d.sx] + bx * cold[o] + cx * cold[o + d.sx];
     const T axy = ay * cold[o - d.sy] + by * cold[o] + cy * cold[o + d.sy];
@@ -178,7 +178,7 @@ __device__ __forceinline__ T mom_explicit(const Dims& d, const Phys& ph, const V
 // xmom/ymom/zmom explicit part: H?_op with mu_old, then the source loop
 // with A(mu_old) and the body force (970-989, 1127-1146, 1280-1298).
 // u, v, w are the old velocities; q is the component's own old field.
-template <typename T, int F>
+template <typename T, int F, bool FAST>
 __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
                                                  const T* __restrict__ w, Visc4<T> mu, const T* __restrict__ cn,
                                                  const T* __restrict__ pg, const T* __restrict__ Nm1, T* __restrict__ N,
@@ -199,24 +199,24 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     const T* __restrict__ qc = q + o;
     T D[6];
     const T vx = mom_vel<T, F, 0>(d, u, v, w, o);
-    line_d1(qc, d.sx, T(d.dx), d.rdx, D);
-    const T dxd = upwind_d(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
+    line_d1<T, FAST>(qc, d.sx, T(d.dx), d.rdx, D);
+    const T dxd = upwind_d<T, FAST>(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
     const T convx = vx * dxd;
     const T vy = mom_vel<T, F, 1>(d, u, v, w, o);
-    line_d1(qc, d.sy, T(d.dy), d.rdy, D);
-    const T dyd = upwind_d(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
+    line_d1<T, FAST>(qc, d.sy, T(d.dy), d.rdy, D);
+    const T dyd = upwind_d<T, FAST>(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
     const T convy = vy * dyd;
     const T vz = mom_vel<T, F, 2>(d, u, v, w, o);
-    line_d1(qc, 1, T(d.dz), d.rdz, D);
-    const T dzd = upwind_d(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
+    line_d1<T, FAST>(qc, 1, T(d.dz), d.rdz, D);
+    const T dzd = upwind_d<T, FAST>(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
     const T ct = vz * dzd + convx + convy;
     const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, o, ct);
     N[o] = nval;
 
     T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
-    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
-    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
+    coefs<T, F, 0, FAST>(d, ph, mu, o, ax, bx, cx);
+    coefs<T, F, 1, FAST>(d, ph, mu, o, ay, by, cy);
+    coefs<T, F, 2, FAST>(d, ph, mu, o, az, bz, cz);
     const T conv = (g * nval + r * Nm1[o]);
     const T axx = ax * q[o - d.sx] + bx * q[o] + cx * q[o + d.sx];
     const T axy = ay * q[o - d.sy] + by * q[o] + cy * q[o + d.sy];
@@ -226,12 +226,12 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     T bodyforce;
     if (F == FU) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sx]);
-        bodyforce = alpha * (-T(ph.fre[0]) * cval - divh<T>(pg[o], Re, ph.rRe));
+        bodyforce = alpha * (-T(ph.fre[0]) * cval - sdh<T, FAST>(pg[o], Re, ph.rRe));
     } else if (F == FV) {
         const T cval = T(0.5) * (cn[o] + cn[o - d.sy]);
-        bodyforce = alpha * (-T(ph.fre[1]) * cval - divh<T>(pg[o], Re, ph.rRe));
+        bodyforce = alpha * (-T(ph.fre[1]) * cval - sdh<T, FAST>(pg[o], Re, ph.rRe));
     } else {
-        bodyforce = alpha * (-divh<T>(pg[o], Re, ph.rRe));
+        bodyforce = alpha * (-sdh<T, FAST>(pg[o], Re, ph.rRe));
     }
     S[o] = dt * (conv + visc + bodyforce);
 }
@@ -1388,16 +1388,23 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
 
 // ------------------------------------------------------ instantiations
 
+#define PLLC_INST_RHS(T, FAST)                                                                                 \
+    template __global__ void k_rhs_c<T, FAST>(Dims, Phys, const T*, const T*, const T*, const T*, const T*, T*,  \
+                                              T*, const T*, const double*, T, T);                               \
+    template __global__ void k_rhs_mom<T, FU, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
+                                                    const T*, const T*, const T*, T*, T*, const T*,             \
+                                                    const double*, T, T);                                       \
+    template __global__ void k_rhs_mom<T, FV, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
+                                                    const T*, const T*, const T*, T*, T*, const T*,             \
+                                                    const double*, T, T);                                       \
+    template __global__ void k_rhs_mom<T, FW, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
+                                                    const T*, const T*, const T*, T*, T*, const T*,             \
+                                                    const double*, T, T);
+
 #define PLLC_INST(T)                                                                                           \
     template __global__ void k_stale<T>(Dims, const T*, const T*, const T*, const T*, T*);                      \
-    template __global__ void k_rhs_c<T>(Dims, Phys, const T*, const T*, const T*, const T*, const T*, T*, T*,  \
-                                        const T*, const double*, T, T);                                         \
-    template __global__ void k_rhs_mom<T, FU>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
-                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
-    template __global__ void k_rhs_mom<T, FV>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
-                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
-    template __global__ void k_rhs_mom<T, FW>(Dims, Phys, const T*, const T*, const T*, Visc4<T>, const T*,    \
-                                              const T*, const T*, T*, T*, const T*, const double*, T, T);       \
+    PLLC_INST_RHS(T, true)                                                                                     \
+    PLLC_INST_RHS(T, false)                                                                                    \
     template __global__ void k_sweep<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 2>(Dims, LineSets<T>, const double*, int, T, T);                      \
@@ -1976,6 +1983,11 @@ const char* kernel_class_names =
         if ((c).hook) (c).hook((c).hook_user, name, 1);      \
     } while (0)
 
+// explicit stencils in reference arithmetic (IEEE quotients, the bits of the
+// reference's WENO) with the Thomas line solver, else the FAST forms
+template <typename T>
+static inline bool fast_stencils(const Ctx<T>& c) { return c.line_solver == 0; }
+
 static inline dim3 cell_grid(const Rng& R, dim3& blk)
 {
     blk = dim3(64, 4, 1);
@@ -2118,8 +2130,14 @@ void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un,
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
-    PLLC_L(c, "rhs_c", (k_rhs_c<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1], c.f[C_N],
-                                                           c.f[F_SRC], c.stale + 0, c.scal, T(g), T(r))));
+    if (fast_stencils(c))
+        PLLC_L(c, "rhs_c", (k_rhs_c<T, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
+                                                                     c.f[C_N], c.f[F_SRC], c.stale + 0, c.scal, T(g),
+                                                                     T(r))));
+    else
+        PLLC_L(c, "rhs_c", (k_rhs_c<T, false><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
+                                                                      c.f[C_N], c.f[F_SRC], c.stale + 0, c.scal, T(g),
+                                                                      T(r))));
     const int F[1] = {FC};
     T* S[1] = {c.f[F_SRC]};
     T* CC[1] = {c.f[F_CC]};
@@ -2144,9 +2162,14 @@ static void rhs_mom(Ctx<T>& c, T g, T r, T* S)
     const char* nm = F == FU ? "rhs_u" : F == FV ? "rhs_v" : "rhs_w";
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, F), blk);
-    PLLC_L(c, nm, (k_rhs_mom<T, F><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], muo(c),
-                                                           c.f[C_NEW], c.f[b + 2], c.f[b + 4], c.f[b + 3], S,
-                                                           c.stale + 4 * F, c.scal, g, r)));
+    if (fast_stencils(c))
+        PLLC_L(c, nm, (k_rhs_mom<T, F, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD],
+                                                                     muo(c), c.f[C_NEW], c.f[b + 2], c.f[b + 4],
+                                                                     c.f[b + 3], S, c.stale + 4 * F, c.scal, g, r)));
+    else
+        PLLC_L(c, nm, (k_rhs_mom<T, F, false><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD],
+                                                                      muo(c), c.f[C_NEW], c.f[b + 2], c.f[b + 4],
+                                                                      c.f[b + 3], S, c.stale + 4 * F, c.scal, g, r)));
 }
 
 // comps: bit 0 u, bit 1 v, bit 2 w.  With several components their sources
