@@ -185,14 +185,39 @@ template <> struct Tr<float> { static constexpr float tiny = 1.0e-18f; };
 // ---------------------------------------------------------------- WENO
 
 // calculate_weno (pllcart3d.c:3455-3507) on five divided differences.
-// Reference arithmetic (FAST false): its six divisions, as IEEE quotients.
-// FAST: the weights' reciprocals from prcp and one normalisation,
-// w1 c1 + w2 c2 + w3 c3 = (a1 c1 + a2 c2 + a3 c3) / (a1 + a2 + a3): the
-// same value to a few ulps in 4 reciprocals instead of 6 divisions.  The
-// denominators lie in [1e-198, 1e30] (epi >= 1e-99), normal numbers.
+// Reference arithmetic (FAST false): its operations in its order, the six
+// divisions as IEEE quotients.  FAST: the same expressions with fused
+// multiply-adds, and the weights from prcp reciprocals and one
+// normalisation, w1 c1 + w2 c2 + w3 c3 = (a1 c1 + a2 c2 + a3 c3) / (a1 + a2
+// + a3): the same value to a few ulps in 4 reciprocals instead of 6
+// divisions.  The denominators lie in [1e-198, 1e30] (epi >= 1e-99), normal
+// numbers.
 template <typename T, bool FAST = false>
 __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
 {
+    if (FAST) {
+        // the same expressions with fused multiply-adds and fmax
+        const T con1 = T(1.0) / T(6.0), con2 = T(13.0) / T(12.0), con3 = T(1.0) / T(4.0);
+        const T maxd = fmax(fmax(fmax(v0 * v0, v1 * v1), fmax(v2 * v2, v3 * v3)), v4 * v4);
+        const T c1 = con1 * fma(T(11.0), v2, fma(T(2.0), v0, T(-7.0) * v1));
+        const T c2 = con1 * fma(T(2.0), v3, fma(T(5.0), v2, -v1));
+        const T c3 = con1 * fma(T(5.0), v3, fma(T(2.0), v2, -v4));
+        T t1 = fma(T(-2.0), v1, v0) + v2;
+        T t2 = fma(T(3.0), v2, fma(T(-4.0), v1, v0));
+        const T s1 = fma(con2 * t1, t1, con3 * t2 * t2);
+        t1 = fma(T(-2.0), v2, v1) + v3;
+        t2 = v1 - v3;
+        const T s2 = fma(con2 * t1, t1, con3 * t2 * t2);
+        t1 = fma(T(-2.0), v3, v2) + v4;
+        t2 = fma(T(3.0), v2, fma(T(-4.0), v3, v4));
+        const T s3 = fma(con2 * t1, t1, con3 * t2 * t2);
+        const T epi = fma(T(1.0e-6), maxd, Tr<T>::tiny);
+        const T q1 = s1 + epi, q2 = s2 + epi, q3 = s3 + epi;
+        const T a1 = T(0.1) * prcp<T>(q1 * q1);
+        const T a2 = T(0.6) * prcp<T>(q2 * q2);
+        const T a3 = T(0.3) * prcp<T>(q3 * q3);
+        return fma(a1, c1, fma(a2, c2, a3 * c3)) * prcp<T>(a1 + a2 + a3);
+    }
     const T con1 = T(1.0) / T(6.0), con2 = T(13.0) / T(12.0), con3 = T(1.0) / T(4.0);
     T maxd = T(0.0), sq;
     sq = v0 * v0; maxd = (maxd > sq) ? maxd : sq;
@@ -213,12 +238,6 @@ __device__ __forceinline__ T weno5(T v0, T v1, T v2, T v3, T v4)
     t2 = T(3.0) * v2 - T(4.0) * v3 + v4;
     T s3 = (con2 * t1 * t1) + (con3 * t2 * t2);
     T epi = T(1.0e-6) * maxd + Tr<T>::tiny;
-    if (FAST) {
-        const T a1 = T(0.1) * prcp<T>((s1 + epi) * (s1 + epi));
-        const T a2 = T(0.6) * prcp<T>((s2 + epi) * (s2 + epi));
-        const T a3 = T(0.3) * prcp<T>((s3 + epi) * (s3 + epi));
-        return (a1 * c1 + a2 * c2 + a3 * c3) * prcp<T>(a1 + a2 + a3);
-    }
     T a1 = fdiv<T>(T(0.1), (s1 + epi) * (s1 + epi));
     T a2 = fdiv<T>(T(0.6), (s2 + epi) * (s2 + epi));
     T a3 = fdiv<T>(T(0.3), (s3 + epi) * (s3 + epi));
