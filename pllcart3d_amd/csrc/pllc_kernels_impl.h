@@ -9,6 +9,8 @@
 // instantiation and the FP64-only helpers of the Tds / fftw entry points.
 #pragma once
 #include <algorithm>
+#include <map>
+#include <utility>
 
 #include "pllc_dev.h"
 #include "pllc_kernels.h"
@@ -1411,7 +1413,9 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_sweep_z_lds<T>(Dims, LineSets<T>, const double*, T, T);                         \
     template __global__ void k_sweep_part<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                 \
     template __global__ void k_sweep_part<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                 \
-    template __global__ void k_sweep_part_z<T>(Dims, LineSets<T>, const double*, T, T, int);                   \
+    template __global__ void k_sweep_part_z<T>(Dims, LineSets<T>, const double*, T, T, int, int, int, int,     \
+                                               int);                                                            \
+    template __global__ void k_sweep_part_zt<T>(Dims, LineSets<T>, const double*, T, T, int);                  \
     template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
     template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
@@ -1750,12 +1754,12 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
     }
 }
 
-// z-lines (rows contiguous along k): the block's LPB lines (one i, LPB
-// consecutive j) are staged through LDS with consecutive threads on
+// z-lines, one tile per block (the FP32 path): the block's LPB lines (one
+// i, LPB consecutive j) are staged through LDS with consecutive threads on
 // consecutive k, solved by chunk from LDS, and written back the same way.
 // w is the LDS row pitch of a line (at least n + 1).
 template <typename T>
-__global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
+__global__ __launch_bounds__(256) void k_sweep_part_zt(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
                                                       T r, int w)
 {
     extern __shared__ unsigned char pllc_smem[];
@@ -1857,6 +1861,168 @@ __global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, 
             for (int u = 0; u < 8; u++)
                 if (l0 + u < nl) L.S[base + (long)(l0 + u) * d.sy + x] = vs[u];
         }
+    }
+}
+
+// z-lines (rows contiguous along k), persistent (the FP64 path).  A tile is
+// LPB lines of one i-plane
+// (LPB consecutive j) of one line set; tiles are numbered (j block, plane,
+// set) and a persistent block takes tiles t = blockIdx.x, + gridDim.x, ...
+// A tile is staged through LDS with consecutive threads on consecutive k,
+// solved by chunk from LDS and written back the same way.  The next tile's
+// rows are loaded into registers while the current one is solved, so the
+// load latency overlaps the elimination.  w is the LDS row pitch of a line
+// (at least n + 1); mu_tiles says whether the launch's sets have a mu tile.
+constexpr int ZMAX = 16;   // staged values per thread and array: n <= 16 P
+
+template <typename T>
+struct ZTile {
+    int set, nl;   // line set; lines in the tile (0: nothing to do)
+    long base;     // first row of the first line
+};
+
+template <typename T>
+__device__ __forceinline__ ZTile<T> z_tile(const Dims& d, const LineSets<T>& sets, int t, int ntiles, int nbx,
+                                           int ni, int LPB)
+{
+    ZTile<T> z{0, 0, 0};
+    if (t >= ntiles) return z;
+    const int per_set = nbx * ni;
+    z.set = t / per_set;
+    const int rem = t - z.set * per_set;
+    const int ii = rem / nbx, bx = rem - ii * nbx;
+    const Rng R = sets.s[z.set].R;
+    const int i = R.is + ii, nj = R.je - R.js + 1;
+    z.nl = (i <= R.ie) ? max(0, min(LPB, nj - bx * LPB)) : 0;
+    z.base = IX(d, i, R.js + bx * LPB, R.ks);
+    return z;
+}
+
+// the tile's nl x n rows of S (and of mu, with the mu row above the last
+// of each line in mtop) into registers: element e = tid + NT m is row
+// e % n of line e / n
+template <typename T>
+__device__ __forceinline__ void z_load(const Dims& d, const LineSets<T>& sets, const ZTile<T>& z, int tid, int NT,
+                                       T (&vs)[ZMAX], T (&vm)[ZMAX], T& mtop)
+{
+    if (z.nl == 0) return;
+    const LineSet<T>& L = sets.s[z.set];
+    const int n = L.R.ke - L.R.ks + 1;
+    const T* __restrict__ S = L.S + z.base;
+    const T* __restrict__ mu = L.mu ? L.mu + L.mu_off + z.base : nullptr;
+    const int tot = z.nl * n, dl = NT / n, dx = NT - dl * n;
+    int lq = tid / n, x = tid - lq * n;
+#pragma unroll
+    for (int m = 0; m < ZMAX; m++) {
+        if (tid + m * NT < tot) {
+            const long g = (long)lq * d.sy + x;
+            vs[m] = S[g];
+            if (mu) vm[m] = mu[g];
+        }
+        x += dx; lq += dl;
+        if (x >= n) { x -= n; lq++; }
+    }
+    if (mu && tid < z.nl) mtop = mu[(long)tid * d.sy + n];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sweep_part_z(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
+                                                      T r, int w, int mu_tiles, int nbx, int ni, int ntiles)
+{
+    extern __shared__ unsigned char pllc_smem[];
+    const int LPB = blockDim.x, P = blockDim.y, E = 2 * P, NT = LPB * P;
+    const int q = threadIdx.x, s = threadIdx.y, tid = s * LPB + q;
+    // tiles tS[LPB][w] (and tM[LPB][w]: rows and the mu row above the
+    // last), then the reduced rows
+    T* tS = reinterpret_cast<T*>(pllc_smem);
+    T* tM = tS + LPB * w;
+    T* red = tS + (mu_tiles ? 2 : 1) * LPB * w;
+    const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
+
+    T vs[ZMAX], vm[ZMAX], mtop = T(0.0);
+    int t = blockIdx.x;
+    ZTile<T> cur = z_tile<T>(d, sets, t, ntiles, nbx, ni, LPB);
+    z_load<T>(d, sets, cur, tid, NT, vs, vm, mtop);
+    while (t < ntiles) {
+        const LineSet<T>& L = sets.s[cur.set];
+        const bool cst = L.mu == nullptr;
+        const int nl = cur.nl;
+        const int n = L.R.ke - L.R.ks + 1;   // line length of this set
+        const int x0 = (s * n) / P;
+        const int m = ((s + 1) * n) / P - x0;
+        if (nl > 0) {
+            const int tot = nl * n, dl = NT / n, dx = NT - dl * n;
+            int lq = tid / n, x = tid - lq * n;
+#pragma unroll
+            for (int mm = 0; mm < ZMAX; mm++) {
+                if (tid + mm * NT < tot) {
+                    tS[lq * w + x] = vs[mm];
+                    if (!cst) tM[lq * w + x] = vm[mm];
+                }
+                x += dx; lq += dl;
+                if (x >= n) { x -= n; lq++; }
+            }
+            if (!cst && tid < nl) tM[tid * w + n] = mtop;
+        }
+        __syncthreads();
+        // the next tile's rows, in flight during this tile's solve
+        const int tn = t + gridDim.x;
+        const ZTile<T> nxt = z_tile<T>(d, sets, tn, ntiles, nbx, ni, LPB);
+        z_load<T>(d, sets, nxt, tid, NT, vs, vm, mtop);
+        if (nl > 0) {
+            const T del = L.del, h = L.h;
+            const double rh = L.rh;
+            const bool live = q < nl;
+            PChunk<T> ch;
+            if (live) {
+                const T* rs = tS + q * w + x0;
+                const T* rm = tM + q * w + x0;
+                T araw = cst ? L.ca : divh<T>(del * rm[0], h, rh);
+#pragma unroll
+                for (int tt = 0; tt < PM; tt++) {
+                    if (tt < m) {
+                        const T craw = cst ? L.cc : divh<T>(del * rm[tt + 1], h, rh);
+                        T a, b, c, rhs = rs[tt];
+                        prow<T, 2>(L, x0 + tt, n, fac, araw, craw, T(0.0), a, b, c, rhs);
+                        ch.fwd(tt, a, b, c, rhs);
+                        araw = craw;
+                    }
+                }
+                ch.fold(m);
+                T la, lc, lD;
+                ch.last(m, la, lc, lD);
+                const int e0 = 2 * s;
+                red[e0 * LPB + q] = ch.A[0];
+                red[(E + e0) * LPB + q] = ch.C[0];
+                red[(2 * E + e0) * LPB + q] = ch.D[0];
+                red[(e0 + 1) * LPB + q] = la;
+                red[(E + e0 + 1) * LPB + q] = lc;
+                red[(2 * E + e0 + 1) * LPB + q] = lD;
+            }
+            __syncthreads();
+            if (live && s == 0) pchunk_reduced<T>(red, E, LPB, q);
+            __syncthreads();
+            if (live) {
+                const T X0 = red[(2 * E + 2 * s) * LPB + q], X1 = red[(2 * E + 2 * s + 1) * LPB + q];
+                T* ws = tS + q * w + x0;
+#pragma unroll
+                for (int tt = 0; tt < PM; tt++)
+                    if (tt < m) ws[tt] = tt == 0 ? X0 : tt == m - 1 ? X1 : ch.D[tt] - ch.A[tt] * X0 - ch.C[tt] * X1;
+            }
+            __syncthreads();
+            T* __restrict__ S = L.S + cur.base;
+            const int tot = nl * n, dl = NT / n, dx = NT - dl * n;
+            int lq = tid / n, x = tid - lq * n;
+#pragma unroll
+            for (int mm = 0; mm < ZMAX; mm++) {
+                if (tid + mm * NT < tot) S[(long)lq * d.sy + x] = tS[lq * w + x];
+                x += dx; lq += dl;
+                if (x >= n) { x -= n; lq++; }
+            }
+        }
+        __syncthreads();   // tS is refilled by the next tile
+        cur = nxt;
+        t = tn;
     }
 }
 
@@ -1982,6 +2148,28 @@ const char* kernel_class_names =
         expr;                                                \
         if ((c).hook) (c).hook((c).hook_user, name, 1);      \
     } while (0)
+
+// blocks of kernel f (nthreads threads, sm bytes of LDS) resident on the
+// device at once: the occupancy per CU times the CUs (cached per shape)
+template <typename K>
+static int resident_blocks(K f, int nthreads, size_t sm)
+{
+    static std::map<std::pair<int, size_t>, int> cache;
+    static int cus = 0;
+    const auto key = std::make_pair(nthreads, sm);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nthreads, sm) != hipSuccess || per_cu <= 0)
+        return 4 * cus;   // not cached: asked again next time
+    cache[key] = per_cu * cus;
+    return per_cu * cus;
+}
 
 // explicit stencils in reference arithmetic (IEEE quotients, the bits of the
 // reference's WENO) with the Thomas line solver, else the FAST forms
@@ -2110,8 +2298,22 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
             while (LPB > 1 && (nt * LPB * w + (size_t)6 * P * LPB) * sizeof(T) > 65536) LPB /= 2;
             while (LPB > 1 && LPB * P > 256) LPB /= 2;
             const size_t sm = (nt * LPB * w + (size_t)6 * P * LPB) * sizeof(T);
-            const dim3 blk(LPB, P, 1), gz((nj + LPB - 1) / LPB, ni, nset);
-            PLLC_L(c, nm[2], (k_sweep_part_z<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w)));
+            // FP64: persistent blocks, as many as fit on the device at once,
+            // each loading its next tile while it solves the current one.
+            // FP32 (half the bytes per tile, fewer registers without the
+            // prefetch, more blocks per CU): one block per tile measured
+            // faster.
+            const dim3 blk(LPB, P, 1);
+            const int nbx = (nj + LPB - 1) / LPB, ntiles = nbx * ni * nset;
+            const int mt = any_mu ? 1 : 0;
+            if (sizeof(T) == 8) {
+                const int G = std::max(1, std::min(ntiles, resident_blocks(k_sweep_part_z<T>, LPB * P, sm)));
+                PLLC_L(c, nm[2], (k_sweep_part_z<T><<<G, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w, mt, nbx, ni,
+                                                                          ntiles)));
+            } else {
+                const dim3 gz(nbx, ni, nset);
+                PLLC_L(c, nm[2], (k_sweep_part_zt<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w)));
+            }
             continue;
         }
         const dim3 grd((unsigned)((lines + 255) / 256), nset, 1);
