@@ -1709,17 +1709,18 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
         red[(e0 + 1) * LPB + q] = la;
         red[(E + e0 + 1) * LPB + q] = lc;
         red[(2 * E + e0 + 1) * LPB + q] = ld;
+        if (DIR == 0) {
+            // q_old for the update, in flight during the reduced solve (and
+            // all of it loaded before the first store of q_new)
+#pragma unroll
+            for (int t = 0; t < PM; t++)
+                if (t < m) qo[t] = L.qold[o0 + (long)(x0 + t) * st];
+        }
     }
     __syncthreads();
     if (live && s == 0) pchunk_reduced<T>(red, E, LPB, q);
     __syncthreads();
     if (!live) return;
-    if (DIR == 0) {
-        // all loads of q_old before the first store of q_new
-#pragma unroll
-        for (int t = 0; t < PM; t++)
-            if (t < m) qo[t] = L.qold[o0 + (long)(x0 + t) * st];
-    }
     const T X0 = red[(2 * E + 2 * s) * LPB + q], X1 = red[(2 * E + 2 * s + 1) * LPB + q];
 #pragma unroll
     for (int t = 0; t < PM; t++) {
