@@ -120,66 +120,155 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     S[o] = dt * (conv + diff);
 }
 
-// Advecting velocity of Hu/Hv/Hw in direction D (1669/1696/1722,
-// 1780/1807/1834, 1892/1919/1946): the component itself along its own
-// direction, 4-point averages of the others.
-template <typename T, int F, int D>
-__device__ __forceinline__ T mom_vel(const Dims& d, const T* __restrict__ u, const T* __restrict__ v,
-                                     const T* __restrict__ w, long o)
+// xmom/ymom/zmom explicit part at one point of component F's range: H?_op
+// with mu_old, then the source loop with A(mu_old) and the body force
+// (970-989, 1127-1146, 1280-1298).  u, v, w are the old velocities; q is the
+// component's own old field.  Every operand of the point is loaded first,
+// so the point's 37 loads are in flight together; the arithmetic below
+// then follows the reference's expressions in its order.
+//
+// Operands, by component (the "first" and "second" other components):
+//   advecting velocity in direction D (1669/1696/1722, 1780/1807/1834,
+//   1892/1919/1946): q itself along its own direction, 4-point averages of
+//   the others, summed in the order of the offsets below (ao, bo);
+//   explicit cross-viscous terms (Hu 1734-1744, Hv 1846-1856, Hw 1958-1970
+//   with the live index of 1962): differences of the same 4 values of each
+//   other component with the edge viscosities mA (first) and mB (second);
+//   A(mu_old) (A_umom/A_vmom/A_wmom, 1443-1536): mu.cell at o - s_F and o
+//   along F's own direction, mA and mB along the others.
+template <typename T, int F, bool FAST>
+__device__ __forceinline__ void coef_vals(const Dims& d, const Phys& ph, int D, T m0, T m1, T& a, T& b, T& c)
 {
-    const long sx = d.sx, sy = d.sy;
-    if (F == FU) {
-        if (D == 0) return u[o];
-        if (D == 1) return T(0.25) * (v[o - sx] + v[o] + v[o + sy] + v[o - sx + sy]);
-        return T(0.25) * (w[o - sx] + w[o] + w[o + 1] + w[o - sx + 1]);
-    }
-    if (F == FV) {
-        if (D == 0) return T(0.25) * (u[o - sy] + u[o + sx - sy] + u[o] + u[o + sx]);
-        if (D == 1) return v[o];
-        return T(0.25) * (w[o - sy] + w[o - sy + 1] + w[o] + w[o + 1]);
-    }
-    if (D == 0) return T(0.25) * (u[o - 1] + u[o + sx - 1] + u[o] + u[o + sx]);
-    if (D == 1) return T(0.25) * (v[o - 1] + v[o] + v[o + sy - 1] + v[o + sy]);
-    return w[o];
+    const T h = D == 0 ? T(d.dx) : D == 1 ? T(d.dy) : T(d.dz);
+    const double rh = D == 0 ? d.rdx : D == 1 ? d.rdy : d.rdz;
+    const T del = T(ph.cdel[F][D]);
+    a = sdh<T, FAST>(del * m0, h, rh);
+    c = sdh<T, FAST>(del * m1, h, rh);
+    // coefs(): -(a + c) for u along x and y, -a - c otherwise
+    b = (F == FU && D != 2) ? -(a + c) : -a - c;
 }
 
-// explicit_n of a velocity component: -convec_term + the explicit
-// cross-viscous terms with mu_old, left to right (Hu 1734-1744,
-// Hv 1846-1856, Hw 1958-1970 with the live index of 1962).
-template <typename T, int F>
-__device__ __forceinline__ T mom_explicit(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ u,
-                                          const T* __restrict__ v, const T* __restrict__ w, long o, T ct)
+template <typename T, int F, bool FAST>
+__device__ __forceinline__ void rhs_mom_point(const Dims& d, const Phys& ph, const Rng& R, int i, int j, int k,
+                                              const T* __restrict__ u, const T* __restrict__ v,
+                                              const T* __restrict__ w, const Visc4<T>& mu, const T* __restrict__ cn,
+                                              const T* __restrict__ pg, const T* __restrict__ Nm1,
+                                              T* __restrict__ N, T* __restrict__ S, const T* __restrict__ stale,
+                                              T dt, T g, T r)
 {
     const long sx = d.sx, sy = d.sy;
-    T del, p1, m1, p2, m2;
+    const long o = IX(d, i, j, k);
+    const long sF = F == FU ? sx : F == FV ? sy : 1;
+    const T* __restrict__ q = F == FU ? u : F == FV ? v : w;
+    const T* __restrict__ pa = F == FU ? v : u;   // first other component
+    const T* __restrict__ pb = F == FW ? v : w;   // second other component
+    const T* __restrict__ mA = F == FU ? mu.xy : F == FV ? mu.xy : mu.xz;
+    const T* __restrict__ mB = F == FU ? mu.xz : mu.yz;
+    const long sA = F == FU ? sy : sx;            // stride of mA's second value
+    const long sB = F == FW ? sy : 1;
+    // offsets of the 4-point averages, in the reference's summation order
+    long ao[4], bo[4];
     if (F == FU) {
-        del = T(ph.edel[0][0]);
-        p1 = del * mu.xy[o + sy] * (v[o + sy] - v[o - sx + sy]);
-        m1 = del * mu.xy[o] * (v[o] - v[o - sx]);
-        del = T(ph.edel[0][1]);
-        p2 = del * mu.xz[o + 1] * (w[o + 1] - w[o - sx + 1]);
-        m2 = del * mu.xz[o] * (w[o] - w[o - sx]);
+        ao[0] = -sx; ao[1] = 0; ao[2] = sy; ao[3] = -sx + sy;
+        bo[0] = -sx; bo[1] = 0; bo[2] = 1; bo[3] = -sx + 1;
     } else if (F == FV) {
-        del = T(ph.edel[1][0]);
-        p1 = del * mu.xy[o + sx] * (u[o + sx] - u[o + sx - sy]);
-        m1 = del * mu.xy[o] * (u[o] - u[o - sy]);
-        del = T(ph.edel[1][1]);
-        p2 = del * mu.yz[o + 1] * (w[o + 1] - w[o - sy + 1]);
-        m2 = del * mu.yz[o] * (w[o] - w[o - sy]);
+        ao[0] = -sy; ao[1] = sx - sy; ao[2] = 0; ao[3] = sx;
+        bo[0] = -sy; bo[1] = -sy + 1; bo[2] = 0; bo[3] = 1;
     } else {
-        del = T(ph.edel[2][0]);
-        p1 = del * mu.xz[o + sx] * (u[o + sx] - u[o + sx - 1]);
-        m1 = del * mu.xz[o] * (u[o] - u[o - 1]);
-        del = T(ph.edel[2][1]);
-        p2 = del * mu.yz[o + sy] * (v[o + sy] - v[o + sy - 1]);
-        m2 = del * mu.yz[o] * (v[o] - v[o - 1]);
+        ao[0] = -1; ao[1] = sx - 1; ao[2] = 0; ao[3] = sx;
+        bo[0] = -1; bo[1] = 0; bo[2] = sy - 1; bo[3] = sy;
     }
-    return -ct + p1 - m1 + p2 - m2;
+
+    T qx[7], qy[7], qz[7], av[4], bv[4];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        qx[m] = q[o + (long)(m - 3) * sx];
+        qy[m] = m == 3 ? T(0) : q[o + (long)(m - 3) * sy];
+        qz[m] = m == 3 ? T(0) : q[o + m - 3];
+    }
+    qy[3] = qx[3];
+    qz[3] = qx[3];
+#pragma unroll
+    for (int m = 0; m < 4; m++) { av[m] = pa[o + ao[m]]; bv[m] = pb[o + bo[m]]; }
+    const T mc0 = mu.cell[o - sF], mc1 = mu.cell[o];
+    const T ma0 = mA[o], ma1 = mA[o + sA], mb0 = mB[o], mb1 = mB[o + sB];
+    const T cn0 = F == FW ? T(0) : cn[o], cn1 = F == FW ? T(0) : cn[o - sF];
+    const T pgv = pg[o], nm1 = Nm1[o];
+
+    // convec_term: upwind derivatives along x, y, z
+    const T alpha = g + r;
+    T D[6];
+    const T vx = F == FU ? qx[3] : T(0.25) * (av[0] + av[1] + av[2] + av[3]);
+    const T vy = F == FV ? qy[3] : F == FU ? T(0.25) * (av[0] + av[1] + av[2] + av[3])
+                                           : T(0.25) * (bv[0] + bv[1] + bv[2] + bv[3]);
+    const T vz = F == FW ? qz[3] : T(0.25) * (bv[0] + bv[1] + bv[2] + bv[3]);
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(qx[m + 1] - qx[m], T(d.dx), d.rdx);
+    const T dxd = upwind_d<T, FAST>(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
+    const T convx = vx * dxd;
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(qy[m + 1] - qy[m], T(d.dy), d.rdy);
+    const T dyd = upwind_d<T, FAST>(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
+    const T convy = vy * dyd;
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(qz[m + 1] - qz[m], T(d.dz), d.rdz);
+    const T dzd = upwind_d<T, FAST>(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
+    const T ct = vz * dzd + convx + convy;
+
+    // explicit_n: -convec_term + the cross-viscous terms, left to right
+    const int fi = F - FU;
+    T del = T(ph.edel[fi][0]);
+    T p1, m1, p2, m2;
+    if (F == FU) {
+        p1 = del * ma1 * (av[2] - av[3]);
+        m1 = del * ma0 * (av[1] - av[0]);
+        del = T(ph.edel[fi][1]);
+        p2 = del * mb1 * (bv[2] - bv[3]);
+        m2 = del * mb0 * (bv[1] - bv[0]);
+    } else if (F == FV) {
+        p1 = del * ma1 * (av[3] - av[1]);
+        m1 = del * ma0 * (av[2] - av[0]);
+        del = T(ph.edel[fi][1]);
+        p2 = del * mb1 * (bv[3] - bv[1]);
+        m2 = del * mb0 * (bv[2] - bv[0]);
+    } else {
+        p1 = del * ma1 * (av[3] - av[1]);
+        m1 = del * ma0 * (av[2] - av[0]);
+        del = T(ph.edel[fi][1]);
+        p2 = del * mb1 * (bv[3] - bv[2]);
+        m2 = del * mb0 * (bv[1] - bv[0]);
+    }
+    const T nval = -ct + p1 - m1 + p2 - m2;
+    N[o] = nval;
+
+    // source: dt [g N + r N_nm1 + alpha A(mu_old) q + alpha (body force)]
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    if (F == FU) {
+        coef_vals<T, F, FAST>(d, ph, 0, mc0, mc1, ax, bx, cx);
+        coef_vals<T, F, FAST>(d, ph, 1, ma0, ma1, ay, by, cy);
+        coef_vals<T, F, FAST>(d, ph, 2, mb0, mb1, az, bz, cz);
+    } else if (F == FV) {
+        coef_vals<T, F, FAST>(d, ph, 0, ma0, ma1, ax, bx, cx);
+        coef_vals<T, F, FAST>(d, ph, 1, mc0, mc1, ay, by, cy);
+        coef_vals<T, F, FAST>(d, ph, 2, mb0, mb1, az, bz, cz);
+    } else {
+        coef_vals<T, F, FAST>(d, ph, 0, ma0, ma1, ax, bx, cx);
+        coef_vals<T, F, FAST>(d, ph, 1, mb0, mb1, ay, by, cy);
+        coef_vals<T, F, FAST>(d, ph, 2, mc0, mc1, az, bz, cz);
+    }
+    const T conv = (g * nval + r * nm1);
+    const T axx = ax * qx[2] + bx * qx[3] + cx * qx[4];
+    const T axy = ay * qy[2] + by * qy[3] + cy * qy[4];
+    const T axz = az * qz[2] + bz * qz[3] + cz * qz[4];
+    const T visc = alpha * (axx + axy + axz);
+    const T Re = T(ph.Re);
+    T bodyforce;
+    if (F == FW) bodyforce = alpha * (-sdh<T, FAST>(pgv, Re, ph.rRe));
+    else bodyforce = alpha * (-T(ph.fre[fi]) * (T(0.5) * (cn0 + cn1)) - sdh<T, FAST>(pgv, Re, ph.rRe));
+    S[o] = dt * (conv + visc + bodyforce);
 }
 
-// xmom/ymom/zmom explicit part: H?_op with mu_old, then the source loop
-// with A(mu_old) and the body force (970-989, 1127-1146, 1280-1298).
-// u, v, w are the old velocities; q is the component's own old field.
+// one component over its range (one thread per point)
 template <typename T, int F, bool FAST>
 __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
                                                  const T* __restrict__ w, Visc4<T> mu, const T* __restrict__ cn,
@@ -193,49 +282,7 @@ __global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __res
     const int j = bt.y * blockDim.y + threadIdx.y + R.js;
     const int i = blockIdx.z + R.is;
     if (k > R.ke || j > R.je) return;
-    const T* __restrict__ q = F == FU ? u : F == FV ? v : w;
-    const long o = IX(d, i, j, k);
-    const T dt = T(scal[S_DT]);
-    const T alpha = g + r;
-
-    const T* __restrict__ qc = q + o;
-    T D[6];
-    const T vx = mom_vel<T, F, 0>(d, u, v, w, o);
-    line_d1<T, FAST>(qc, d.sx, T(d.dx), d.rdx, D);
-    const T dxd = upwind_d<T, FAST>(D, i, R.is, R.ie, vx >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
-    const T convx = vx * dxd;
-    const T vy = mom_vel<T, F, 1>(d, u, v, w, o);
-    line_d1<T, FAST>(qc, d.sy, T(d.dy), d.rdy, D);
-    const T dyd = upwind_d<T, FAST>(D, j, R.js, R.je, vy >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
-    const T convy = vy * dyd;
-    const T vz = mom_vel<T, F, 2>(d, u, v, w, o);
-    line_d1<T, FAST>(qc, 1, T(d.dz), d.rdz, D);
-    const T dzd = upwind_d<T, FAST>(D, k, R.ks, R.ke, vz >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
-    const T ct = vz * dzd + convx + convy;
-    const T nval = mom_explicit<T, F>(d, ph, mu, u, v, w, o, ct);
-    N[o] = nval;
-
-    T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0, FAST>(d, ph, mu, o, ax, bx, cx);
-    coefs<T, F, 1, FAST>(d, ph, mu, o, ay, by, cy);
-    coefs<T, F, 2, FAST>(d, ph, mu, o, az, bz, cz);
-    const T conv = (g * nval + r * Nm1[o]);
-    const T axx = ax * q[o - d.sx] + bx * q[o] + cx * q[o + d.sx];
-    const T axy = ay * q[o - d.sy] + by * q[o] + cy * q[o + d.sy];
-    const T axz = az * q[o - 1] + bz * q[o] + cz * q[o + 1];
-    const T visc = alpha * (axx + axy + axz);
-    const T Re = T(ph.Re);
-    T bodyforce;
-    if (F == FU) {
-        const T cval = T(0.5) * (cn[o] + cn[o - d.sx]);
-        bodyforce = alpha * (-T(ph.fre[0]) * cval - sdh<T, FAST>(pg[o], Re, ph.rRe));
-    } else if (F == FV) {
-        const T cval = T(0.5) * (cn[o] + cn[o - d.sy]);
-        bodyforce = alpha * (-T(ph.fre[1]) * cval - sdh<T, FAST>(pg[o], Re, ph.rRe));
-    } else {
-        bodyforce = alpha * (-sdh<T, FAST>(pg[o], Re, ph.rRe));
-    }
-    S[o] = dt * (conv + visc + bodyforce);
+    rhs_mom_point<T, F, FAST>(d, ph, R, i, j, k, u, v, w, mu, cn, pg, Nm1, N, S, stale, T(scal[S_DT]), g, r);
 }
 
 // --------------------------------------------------------- line solvers
