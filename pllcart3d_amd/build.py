@@ -19,7 +19,7 @@ OBJDIR = os.path.join(HERE, "_build")
 # multiply-adds; the FP32 kernels of the mixed path may
 HIP_SOURCES = [
     ("pllc_kernels_f64.hip", ["-ffp-contract=off"]),
-    ("pllc_kernels_f32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("pllc_kernels_f32.hip", ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("pllc_runtime.hip", ["-ffp-contract=off"]),
     ("pllc_abi.hip", ["-ffp-contract=off"]),
 ]

@@ -88,21 +88,36 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const Rng R = field_rng(d, FC);
-    const long o = IX(d, i, j, k);
+    const long o = IX(d, i, j, k), sx = d.sx, sy = d.sy;
     const T dt = T(scal[S_DT]);
 
-    const T* __restrict__ cc = cold + o;
+    // every operand first (one memory round trip), then the arithmetic
+    T cx7[7], cy7[7], cz7[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        cx7[m] = cold[o + (long)(m - 3) * sx];
+        cy7[m] = m == 3 ? T(0) : cold[o + (long)(m - 3) * sy];
+        cz7[m] = m == 3 ? T(0) : cold[o + m - 3];
+    }
+    cy7[3] = cx7[3];
+    cz7[3] = cx7[3];
+    const T u0 = un[o], u1 = un[o + sx], v0 = vn[o], v1 = vn[o + sy], w0 = wn[o], w1 = wn[o + 1];
+    const T nm1 = Nm1[o];
+
     T D[6];
-    const T uval = T(0.5) * (un[o + d.sx] + un[o]);
-    line_d1<T, FAST>(cc, d.sx, T(d.dx), d.rdx, D);
+    const T uval = T(0.5) * (u1 + u0);
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(cx7[m + 1] - cx7[m], T(d.dx), d.rdx);
     const T cxd = upwind_d<T, FAST>(D, i, R.is, R.ie, uval >= T(0.0) ? 1 : -1, T(d.dx), d.rdx, T(0.0), T(0.0));
     const T convx = uval * cxd;
-    const T vval = T(0.5) * (vn[o + d.sy] + vn[o]);
-    line_d1<T, FAST>(cc, d.sy, T(d.dy), d.rdy, D);
+    const T vval = T(0.5) * (v1 + v0);
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(cy7[m + 1] - cy7[m], T(d.dy), d.rdy);
     const T cyd = upwind_d<T, FAST>(D, j, R.js, R.je, vval >= T(0.0) ? 1 : -1, T(d.dy), d.rdy, stale[0], stale[1]);
     const T convy = vval * cyd;
-    const T wval = T(0.5) * (wn[o + 1] + wn[o]);
-    line_d1<T, FAST>(cc, 1, T(d.dz), d.rdz, D);
+    const T wval = T(0.5) * (w1 + w0);
+#pragma unroll
+    for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(cz7[m + 1] - cz7[m], T(d.dz), d.rdz);
     const T czd = upwind_d<T, FAST>(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
     const T nval = -(wval * czd + convx + convy);
     N[o] = nval;
@@ -112,10 +127,10 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     coefs<T, FC, 0, FAST>(d, ph, none, o, ax, bx, cx);
     coefs<T, FC, 1, FAST>(d, ph, none, o, ay, by, cy);
     coefs<T, FC, 2, FAST>(d, ph, none, o, az, bz, cz);
-    const T conv = (g * nval + r * Nm1[o]);
-    const T axx = ax * cold[o - d.sx] + bx * cold[o] + cx * cold[o + d.sx];
-    const T axy = ay * cold[o - d.sy] + by * cold[o] + cy * cold[o + d.sy];
-    const T axz = az * cold[o - 1] + bz * cold[o] + cz * cold[o + 1];
+    const T conv = (g * nval + r * nm1);
+    const T axx = ax * cx7[2] + bx * cx7[3] + cx * cx7[4];
+    const T axy = ay * cy7[2] + by * cy7[3] + cy * cy7[4];
+    const T axz = az * cz7[2] + bz * cz7[3] + cz * cz7[4];
     const T diff = (g + r) * (axx + axy + axz);
     S[o] = dt * (conv + diff);
 }
@@ -834,6 +849,80 @@ __global__ __launch_bounds__(256) void k_phi_scatter(Dims d, const T* __restrict
     phi[IX(d, i, j, k)] = fdiv<T>(H[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)], norm);
 }
 
+// The operands of a point of component F: phi at the 7 gradient points p
+// (o, o -/+ sx, o -/+ sy, o -/+ 1) and at p - s_F, the six viscosities of
+// A(mu_new) in coefs' order, the component and its pg.
+template <typename T>
+struct CorrOps { T p0[7], p1[7], m[6], q, pg; };
+
+template <typename T, int F>
+__device__ __forceinline__ void corr_load(const Dims& d, const Visc4<T>& mu, const T* __restrict__ phi,
+                                          const T* __restrict__ q, const T* __restrict__ pg, long o, CorrOps<T>& c)
+{
+    const long sx = d.sx, sy = d.sy;
+    const long sF = F == FU ? sx : F == FV ? sy : 1;
+    const long pt[7] = {0, -sx, sx, -sy, sy, -1, 1};
+#pragma unroll
+    for (int g = 0; g < 7; g++) {
+        c.p0[g] = phi[o + pt[g]];
+        c.p1[g] = phi[o + pt[g] - sF];
+    }
+    if (F == FU) {
+        c.m[0] = mu.cell[o - sx]; c.m[1] = mu.cell[o]; c.m[2] = mu.xy[o]; c.m[3] = mu.xy[o + sy];
+        c.m[4] = mu.xz[o]; c.m[5] = mu.xz[o + 1];
+    } else if (F == FV) {
+        c.m[0] = mu.xy[o]; c.m[1] = mu.xy[o + sx]; c.m[2] = mu.cell[o - sy]; c.m[3] = mu.cell[o];
+        c.m[4] = mu.yz[o]; c.m[5] = mu.yz[o + 1];
+    } else {
+        c.m[0] = mu.xz[o]; c.m[1] = mu.xz[o + sx]; c.m[2] = mu.yz[o]; c.m[3] = mu.yz[o + sy];
+        c.m[4] = mu.cell[o - 1]; c.m[5] = mu.cell[o];
+    }
+    c.q = q[o];
+    c.pg = pg[o];
+}
+
+// the phi gradient g of component F at gradient point g (pgrad)
+template <typename T, int F>
+__device__ __forceinline__ T corr_grad(const Dims& d, const CorrOps<T>& c, int g)
+{
+    const T h = F == FU ? T(d.dx) : F == FV ? T(d.dy) : T(d.dz);
+    const double rh = F == FU ? d.rdx : F == FV ? d.rdy : d.rdz;
+    return divh<T>(c.p0[g] - c.p1[g], h, rh);
+}
+
+// velo_correction's new value of the component (2144-2169)
+template <typename T, int F>
+__device__ __forceinline__ T corr_q(const Dims& d, const CorrOps<T>& c, T facv)
+{
+    return c.q - facv * corr_grad<T, F>(d, c, 0);
+}
+
+// pressure_gradient's new pg (2964-3044): pg + grad phi - fac A(mu_new) grad phi
+template <typename T, int F>
+__device__ __forceinline__ T corr_pg_new(const Dims& d, const Phys& ph, const CorrOps<T>& c, T fac)
+{
+    T gr[7];
+#pragma unroll
+    for (int g = 0; g < 7; g++) gr[g] = corr_grad<T, F>(d, c, g);
+    T ax, bx, cx, ay, by, cy, az, bz, cz;
+    coef_vals<T, F, false>(d, ph, 0, c.m[0], c.m[1], ax, bx, cx);
+    coef_vals<T, F, false>(d, ph, 1, c.m[2], c.m[3], ay, by, cy);
+    coef_vals<T, F, false>(d, ph, 2, c.m[4], c.m[5], az, bz, cz);
+    const T g0 = gr[0];
+    const T pxx = (ax) * gr[1] + (bx) * g0 + (cx) * gr[2];
+    const T pyy = (ay) * gr[3] + (by) * g0 + (cy) * gr[4];
+    const T pzz = (az) * gr[5] + (bz) * g0 + (cz) * gr[6];
+    return c.pg + g0 - fac * (pyy + pxx + pzz);
+}
+
+template <typename T, int F>
+__device__ __forceinline__ void corr_apply(const Dims& d, const Phys& ph, const CorrOps<T>& c, T* __restrict__ q,
+                                           T* __restrict__ pg, long o, T fac, T facv)
+{
+    q[o] = corr_q<T, F>(d, c, facv);
+    pg[o] = corr_pg_new<T, F>(d, ph, c, fac);
+}
+
 // velo_correction (2144-2169): u -= dt*alpha/Re * grad(phi) on each
 // component's unknowns (so neither the inlet nor the outlet face).
 template <typename T>
@@ -848,17 +937,19 @@ __global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __r
     if (k > d.nz || j > d.ny) return;
     const long o = IX(d, i, j, k);
     const T fac = T(scal[S_DT]) * (g + r) / T(ph.Re);
+    // the fused kernel's expressions (corr_q) on the operands it needs
+    CorrOps<T> c;
     if (j >= 2) {
-        const T dp = divh<T>(phi[o] - phi[o - d.sy], T(d.dy), d.rdy);
-        v[o] = v[o] - fac * dp;
+        c.p0[0] = phi[o]; c.p1[0] = phi[o - d.sy]; c.q = v[o];
+        v[o] = corr_q<T, FV>(d, c, fac);
     }
     if (k >= 2) {
-        const T dp = divh<T>(phi[o] - phi[o - 1], T(d.dz), d.rdz);
-        w[o] = w[o] - fac * dp;
+        c.p0[0] = phi[o]; c.p1[0] = phi[o - 1]; c.q = w[o];
+        w[o] = corr_q<T, FW>(d, c, fac);
     }
     if (i >= 4) {
-        const T dp = divh<T>(phi[o] - phi[o - d.sx], T(d.dx), d.rdx);
-        u[o] = u[o] - fac * dp;
+        c.p0[0] = phi[o]; c.p1[0] = phi[o - d.sx]; c.q = u[o];
+        u[o] = corr_q<T, FU>(d, c, fac);
     }
 }
 
@@ -919,54 +1010,10 @@ __global__ void k_add_deficit(Dims d, T* __restrict__ u, const double* __restric
     u[o] = u[o] + T(scal[S_RK + 4 * rk + 3]);
 }
 
-// pressure_gradient (2964-3044): pg += grad(phi) - fac A(mu_new) grad(phi)
-// for u, v and w at their unknowns; the gradients at the 7 stencil points
-// are recomputed from phi instead of stored.
-template <typename T, int F>
-__device__ __forceinline__ T pgrad(const Dims& d, const T* __restrict__ phi, long o)
-{
-    if (F == FU) return divh<T>(phi[o] - phi[o - d.sx], T(d.dx), d.rdx);
-    if (F == FV) return divh<T>(phi[o] - phi[o - d.sy], T(d.dy), d.rdy);
-    return divh<T>(phi[o] - phi[o - 1], T(d.dz), d.rdz);
-}
-
-template <typename T, int F>
-__device__ __forceinline__ void pg_update(const Dims& d, const Phys& ph, const Visc4<T>& mu, const T* __restrict__ phi,
-                                          T* __restrict__ pg, int i, int j, int k, T fac)
-{
-    const long o = IX(d, i, j, k);
-    T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
-    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
-    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
-    const T g0 = pgrad<T, F>(d, phi, o);
-    const T pxx = (ax) * pgrad<T, F>(d, phi, o - d.sx) + (bx) * g0 + (cx) * pgrad<T, F>(d, phi, o + d.sx);
-    const T pyy = (ay) * pgrad<T, F>(d, phi, o - d.sy) + (by) * g0 + (cy) * pgrad<T, F>(d, phi, o + d.sy);
-    const T pzz = (az) * pgrad<T, F>(d, phi, o - 1) + (bz) * g0 + (cz) * pgrad<T, F>(d, phi, o + 1);
-    pg[o] = pg[o] + g0 - fac * (pyy + pxx + pzz);
-}
-
 // velo_correction (2144-2169) and pressure_gradient (2964-3044) of one
 // substage in one pass: both use the same phi gradient g0 at a velocity
 // point, and pg reads neither u, v nor w, so updating it here, before the
 // outflow deficit and the ghost copies, gives the reference's values.
-template <typename T, int F>
-__device__ __forceinline__ void corr_pg_update(const Dims& d, const Phys& ph, const Visc4<T>& mu,
-                                               const T* __restrict__ phi, T* __restrict__ q, T* __restrict__ pg,
-                                               long o, T fac, T facv)
-{
-    T ax, bx, cx, ay, by, cy, az, bz, cz;
-    coefs<T, F, 0>(d, ph, mu, o, ax, bx, cx);
-    coefs<T, F, 1>(d, ph, mu, o, ay, by, cy);
-    coefs<T, F, 2>(d, ph, mu, o, az, bz, cz);
-    const T g0 = pgrad<T, F>(d, phi, o);
-    q[o] = q[o] - facv * g0;
-    const T pxx = (ax) * pgrad<T, F>(d, phi, o - d.sx) + (bx) * g0 + (cx) * pgrad<T, F>(d, phi, o + d.sx);
-    const T pyy = (ay) * pgrad<T, F>(d, phi, o - d.sy) + (by) * g0 + (cy) * pgrad<T, F>(d, phi, o + d.sy);
-    const T pzz = (az) * pgrad<T, F>(d, phi, o - 1) + (bz) * g0 + (cz) * pgrad<T, F>(d, phi, o + 1);
-    pg[o] = pg[o] + g0 - fac * (pyy + pxx + pzz);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
                                                  T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
@@ -981,11 +1028,20 @@ __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, c
     const long o = IX(d, i, j, k);
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
     const T facv = T(scal[S_DT]) * (g + r) / T(ph.Re);
-    if (j >= 2) corr_pg_update<T, FV>(d, ph, mu, phi, v, pgv, o, fac, facv);
-    if (k >= 2) corr_pg_update<T, FW>(d, ph, mu, phi, w, pgw, o, fac, facv);
-    if (i >= 4) corr_pg_update<T, FU>(d, ph, mu, phi, u, pgu, o, fac, facv);
+    // the three components' operands are loaded before any update
+    const bool dv = j >= 2, dw = k >= 2, du = i >= 4;
+    CorrOps<T> cv, cw, cu;
+    if (dv) corr_load<T, FV>(d, mu, phi, v, pgv, o, cv);
+    if (dw) corr_load<T, FW>(d, mu, phi, w, pgw, o, cw);
+    if (du) corr_load<T, FU>(d, mu, phi, u, pgu, o, cu);
+    if (dv) corr_apply<T, FV>(d, ph, cv, v, pgv, o, fac, facv);
+    if (dw) corr_apply<T, FW>(d, ph, cw, w, pgw, o, fac, facv);
+    if (du) corr_apply<T, FU>(d, ph, cu, u, pgu, o, fac, facv);
 }
 
+// pressure_gradient alone (2964-3044): pg += grad(phi) - fac A(mu_new)
+// grad(phi) for the components in mask (bit 0 u, 1 v, 2 w) at their
+// unknowns; the gradients at the 7 stencil points are recomputed from phi.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
                                             T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
@@ -997,9 +1053,12 @@ __global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const 
     const int i = blockIdx.z + 3;
     if (k > d.nz || j > d.ny) return;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
-    if ((mask & 2) && j >= 2) pg_update<T, FV>(d, ph, mu, phi, pgv, i, j, k, fac);
-    if ((mask & 1) && i >= 4) pg_update<T, FU>(d, ph, mu, phi, pgu, i, j, k, fac);
-    if ((mask & 4) && k >= 2) pg_update<T, FW>(d, ph, mu, phi, pgw, i, j, k, fac);
+    const long o = IX(d, i, j, k);
+    // the fused kernel's expressions (corr_pg_new); q is not used
+    CorrOps<T> c;
+    if ((mask & 2) && j >= 2) { corr_load<T, FV>(d, mu, phi, pgv, pgv, o, c); pgv[o] = corr_pg_new<T, FV>(d, ph, c, fac); }
+    if ((mask & 1) && i >= 4) { corr_load<T, FU>(d, mu, phi, pgu, pgu, o, c); pgu[o] = corr_pg_new<T, FU>(d, ph, c, fac); }
+    if ((mask & 4) && k >= 2) { corr_load<T, FW>(d, mu, phi, pgw, pgw, o, c); pgw[o] = corr_pg_new<T, FW>(d, ph, c, fac); }
 }
 
 // ------------------------------------------------------------ reductions
