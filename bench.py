@@ -96,10 +96,10 @@ PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
 # and profiles/step_mixed_256x128x128.md (MB, per kernel class; the DCT is
 # the mean of its two contraction kernels)
 MEASURED_TRAFFIC_MB = {
-    8.0: {"sweep_z": 188.2, "sweep_x": 352.6, "sweep_y": 253.9, "dct": 84.8, "corr_pg": 758.1,
-          "rhs_c": 356.9, "rhs_u": 536.1, "rhs_v": 552.9, "rhs_w": 503.7, "div": 170.0, "visc": 250.2},
-    4.0: {"sweep_z": 94.1, "sweep_x": 226.8, "sweep_y": 129.8, "dct": 42.5, "corr_pg": 433.8,
-          "rhs_c": 208.6, "rhs_u": 315.3, "rhs_v": 326.3, "rhs_w": 295.9, "div": 98.6, "visc": 145.9},
+    8.0: {"sweep_z": 188.1, "sweep_x": 259.5, "sweep_y": 188.4, "dct": 84.8, "corr_pg": 593.3,
+          "rhs_c": 255.5, "rhs_u": 393.7, "rhs_v": 396.0, "rhs_w": 360.3, "div": 138.8, "visc": 251.9},
+    4.0: {"sweep_z": 93.8, "sweep_x": 130.2, "sweep_y": 94.4, "dct": 42.5, "corr_pg": 298.2,
+          "rhs_c": 128.4, "rhs_u": 198.1, "rhs_v": 199.2, "rhs_w": 181.2, "div": 69.8, "visc": 146.8},
 }
 
 

@@ -631,22 +631,17 @@ __global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restri
     const int j = blockIdx.y * blockDim.y + threadIdx.y;       // 0 .. ny+1
     const int i = blockIdx.z + 2;                              // 2 .. nx+3
     if (k > d.nz + 1 || j > d.ny + 1) return;
-    const long o = IX(d, i, j, k);
-    cell[o] = visc_law<T>(ph, cn[o]);
+    const long o = IX(d, i, j, k), sx = d.sx, sy = d.sy;
+    // the seven values first (all inside the array: i >= 2 leaves a ghost
+    // plane below o - sx - sy - 1), then the laws
+    const T c0 = cn[o], cxm = cn[o - sx], cym = cn[o - sy], czm = cn[o - 1];
+    const T cxy = cn[o - sx - sy], cxz = cn[o - sx - 1], cyz = cn[o - sy - 1];
+    cell[o] = visc_law<T>(ph, c0);
     const bool ix = i >= 3, iy = j >= 1, iz = k >= 1;
     const bool jx = j <= d.ny, kx = k <= d.nz, ii = i <= d.nx + 2;
-    if (ix && iy && kx && iz) {
-        const T cavg = T(0.25) * (cn[o - d.sx - d.sy] + cn[o - d.sy] + cn[o - d.sx] + cn[o]);
-        xy[o] = visc_law<T>(ph, cavg);
-    }
-    if (ix && iy && jx && iz) {
-        const T cavg = T(0.25) * (cn[o - d.sx - 1] + cn[o - 1] + cn[o - d.sx] + cn[o]);
-        xz[o] = visc_law<T>(ph, cavg);
-    }
-    if (ix && ii && iy && iz) {
-        const T cavg = T(0.25) * (cn[o - d.sy - 1] + cn[o - 1] + cn[o - d.sy] + cn[o]);
-        yz[o] = visc_law<T>(ph, cavg);
-    }
+    if (ix && iy && kx && iz) xy[o] = visc_law<T>(ph, T(0.25) * (cxy + cym + cxm + c0));
+    if (ix && iy && jx && iz) xz[o] = visc_law<T>(ph, T(0.25) * (cxz + czm + cxm + c0));
+    if (ix && ii && iy && iz) yz[o] = visc_law<T>(ph, T(0.25) * (cyz + czm + cym + c0));
 }
 
 // ----------------------------------------------------------- ghost rules
