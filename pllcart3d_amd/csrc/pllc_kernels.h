@@ -73,7 +73,8 @@ struct Ops {
     static void momentum3(Ctx<T>& c, double g, double r, int slot);
     static void velo_ghost(Ctx<T>& c);
     static void divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack);
-    static void poisson(Ctx<T>& c, double g, double r, bool from_div);
+    // ghosts: phi's face ghosts too (update_phi_ghosts) in the same pass
+    static void poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts = false);
     static void phi_ghost(Ctx<T>& c);
     static void velo_correction(Ctx<T>& c, double g, double r, int slot);
     // mask bit 0 u, bit 1 v, bit 2 w; coefficients A(mu) from c.f[MUN_*]
@@ -91,7 +92,8 @@ struct Ops {
     static void set_scalar(Ctx<T>& c, int idx, double v);
     // dense 2D DCT of a packed block: REDFT10 (forward) or REDFT01
     // (inverse), z then y, `in` -> `out` through the temporary c.Q
-    static void dct2(Ctx<T>& c, const T* in, T* out, bool inverse);
+    // phi (inverse only): the result goes to phi's interior instead of out
+    static void dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi = nullptr, bool ghosts = false);
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)

@@ -830,20 +830,6 @@ __global__ __launch_bounds__(256) void k_pois_x(Dims d, const T* __restrict__ la
     else pois_line<T, T>(d, lam_y[jy], lam_z[kz], jy, kz, l, P, CC);
 }
 
-// idct normalisation and copy back into phi (fftw_3d.h:101,
-// pllcart3d.c:2106-2112)
-template <typename T>
-__global__ __launch_bounds__(256) void k_phi_scatter(Dims d, const T* __restrict__ H, T* __restrict__ phi)
-{
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + 1;
-    const int j = bt.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 3;
-    if (k > d.nz || j > d.ny) return;
-    const T norm = T(4 * d.ny * d.nz);
-    phi[IX(d, i, j, k)] = fdiv<T>(H[((long)(i - 3) * d.ny + (j - 1)) * d.nz + (k - 1)], norm);
-}
-
 // The operands of a point of component F: phi at the 7 gradient points p
 // (o, o -/+ sx, o -/+ sy, o -/+ 1) and at p - s_F, the six viscosities of
 // A(mu_new) in coefs' order, the component and its pg.
@@ -1204,9 +1190,18 @@ template <> struct Mma<float> {
     }
 };
 
+// Epilogue of the last inverse contraction when phi is set: batch b, row
+// gm, column gn of the packed [x][y][z] block is phi's interior point
+// (3 + b, 1 + gm, 1 + gn), divided by the inverse transform's norm 4 ny nz
+// (fftw_3d.h:101); with ghosts, a point next to a face also writes its
+// face ghost copy (update_phi_ghosts, 2758-2807).
+template <typename T>
+struct PhiOut { T* phi; long sx, sy; int nx, ny, nz, ghosts; };
+
 template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
-                                              const T* __restrict__ B, long sB, T* __restrict__ C, long sC)
+                                              const T* __restrict__ B, long sB, T* __restrict__ C, long sC,
+                                              PhiOut<T> po)
 {
     constexpr int BM = 64, BN = 128, BK = 16;
     constexpr int NA = BM * BK / 256, NB = BN * BK / 256;   // per-thread loads per chunk
@@ -1276,6 +1271,33 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
                 for (int t = 0; t < 4; t++) acc[s][t] = Mma<T>::run(av[s], bv[t], acc[s][t]);
         }
         buf ^= 1;
+    }
+    if (po.phi) {
+        const T norm = T(4 * po.ny * po.nz);
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int gn = n0 + wn + 16 * t + li;
+                if (gn >= N) continue;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int gm = m0 + wm + 16 * s + Mma<T>::row(lk, r);
+                    if (gm >= M) continue;
+                    const long o = (long)(3 + b) * po.sx + (long)(1 + gm) * po.sy + (1 + gn);
+                    const T val = fdiv<T>(acc[s][t][r], norm);
+                    po.phi[o] = val;
+                    if (po.ghosts) {
+                        if (gn == 0) po.phi[o - 1] = val;
+                        if (gn == po.nz - 1) po.phi[o + 1] = val;
+                        if (gm == 0) po.phi[o - po.sy] = val;
+                        if (gm == po.ny - 1) po.phi[o + po.sy] = val;
+                        if (b == 0) po.phi[o - po.sx] = val;
+                        if (b == po.nx - 1) po.phi[o + po.sx] = val;
+                    }
+                }
+            }
+        return;
     }
 #pragma unroll
     for (int s = 0; s < 2; s++)
@@ -1523,15 +1545,16 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_div<T>(Dims, Phys, const T*, const T*, const T*, T*, T*, const double*, T, T);  \
     template __global__ void k_pack_div<T>(Dims, Phys, const T*, T*, const double*, T, T);                     \
     template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*, const double*);                     \
-    template __global__ void k_phi_scatter<T>(Dims, const T*, T*);                                             \
     template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
     template __global__ void k_flux_rows<T>(Dims, const T*, double*);                                          \
     template __global__ void k_flux_deficit<T>(Dims, Phys, T*, double*, const double*, int);                   \
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
     template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
-    template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long);         \
-    template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long);        \
+    template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long,          \
+                                             PhiOut<T>);                                                        \
+    template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long,         \
+                                              PhiOut<T>);                                                       \
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
 
@@ -2239,7 +2262,7 @@ namespace pllc {
 
 #ifdef PLLC_F64_TU
 const char* kernel_class_names =
-    "stale;rhs_c;rhs_u;rhs_v;rhs_w;sweep_y;sweep_z;sweep_x;visc;velo_ghost;div;dct;pois_x;phi_scatter;"
+    "stale;rhs_c;rhs_u;rhs_v;rhs_w;sweep_y;sweep_z;sweep_x;visc;velo_ghost;div;dct;pois_x;"
     "phi_ghost;velo_corr;flux;pg;corr_pg;cfl;bulk;copy";
 #endif
 
@@ -2532,24 +2555,29 @@ void Ops<T>::divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack
 }
 
 template <typename T>
-void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse)
+void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool ghosts)
 {
     const int nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
     const long R = (long)nx * ny, plane = (long)ny * nz;
     const dim3 gz((nz + 127) / 128, (unsigned)((R + 63) / 64), 1), gy((nz + 127) / 128, (ny + 63) / 64, nx);
     if (!inverse) {
         // z: Q[r][kz] = sum_m in[r][m] Cfz[kz][m]; then y: out[x] = Cfy Q[x]
-        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Cfz, 0, c.Q, 0)));
-        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Cfy, 0, c.Q, plane, out, plane)));
+        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Cfz, 0, c.Q, 0,
+                                                                  PhiOut<T>{})));
+        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Cfy, 0, c.Q, plane, out, plane,
+                                                                   PhiOut<T>{})));
     } else {
-        // z then y, as the forward transform
-        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Ciz, 0, c.Q, 0)));
-        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Ciy, 0, c.Q, plane, out, plane)));
+        // z then y, as the forward transform; with phi set the y pass writes
+        // phi (and with ghosts its face ghosts) instead of out
+        PLLC_L(c, "dct", (k_gemm<T, true><<<gz, 256, 0, c.st>>>((int)R, nz, nz, in, 0, c.Ciz, 0, c.Q, 0,
+                                                                  PhiOut<T>{})));
+        const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+        PLLC_L(c, "dct", (k_gemm<T, false><<<gy, 256, 0, c.st>>>(ny, nz, ny, c.Ciy, 0, c.Q, plane, out, plane, po)));
     }
 }
 
 template <typename T>
-void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div)
+void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts)
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
@@ -2566,8 +2594,8 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div)
     } else {
         PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC, c.lam00)));
     }
-    dct2(c, c.P, c.P, true);
-    PLLC_L(c, "phi_scatter", (k_phi_scatter<T><<<grd, blk, 0, c.st>>>(c.d, c.P, c.f[PHI])));
+    // the inverse transform's last pass writes phi, normalised
+    dct2(c, c.P, c.P, true, c.f[PHI], ghosts);
 }
 
 template <typename T>
@@ -2685,7 +2713,8 @@ void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int
 void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,
              hipStream_t st)
 {
-    k_gemm<double, false><<<dim3((N + 127) / 128, (M + 63) / 64, batch), 256, 0, st>>>(M, N, K, A, sA, B, sB, C, sC);
+    k_gemm<double, false><<<dim3((N + 127) / 128, (M + 63) / 64, batch), 256, 0, st>>>(M, N, K, A, sA, B, sB, C, sC,
+                                                                                         PhiOut<double>{});
 }
 
 __global__ void k_scale_div(double* __restrict__ a, long n, double div)

@@ -684,8 +684,7 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::momentum3(c, g, r, rk);
         Ops<T>::velo_ghost(c);
         Ops<T>::divergence(c, g, r, false, true);
-        Ops<T>::poisson(c, g, r, false);
-        Ops<T>::phi_ghost(c);
+        Ops<T>::poisson(c, g, r, false, true);   // with update_phi_ghosts
         Ops<T>::correction_and_gradient(c, g, r, rk);   // velo_correction + pressure_gradient
         Ops<T>::velo_ghost(c);
         if (s->copy_next) {
