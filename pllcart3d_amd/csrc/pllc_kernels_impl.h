@@ -1198,12 +1198,16 @@ template <> struct Mma<float> {
 template <typename T>
 struct PhiOut { T* phi; long sx, sy; int nx, ny, nz, ghosts; };
 
+#ifndef GEMM_BK
+#define GEMM_BK 16
+#endif
+
 template <typename T, bool TRB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __restrict__ A, long sA,
                                               const T* __restrict__ B, long sB, T* __restrict__ C, long sC,
                                               PhiOut<T> po)
 {
-    constexpr int BM = 64, BN = 128, BK = 16;
+    constexpr int BM = 64, BN = 128, BK = GEMM_BK;
     constexpr int NA = BM * BK / 256, NB = BN * BK / 256;   // per-thread loads per chunk
     // two LDS buffers: the next chunk's global loads are in flight (in
     // registers) while the matrix cores work on the current one
@@ -1226,15 +1230,15 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 #pragma unroll
         for (int p = 0; p < NA; p++) {
             const int e = threadIdx.x + 256 * p;
-            const int mm = e >> 4, kk = e & 15;
+            const int mm = e / BK, kk = e % BK;
             const int gm = m0 + mm, gk = k0 + kk;
             ra[p] = (gm < M && gk < K) ? A[(long)gm * K + gk] : T(0);
         }
 #pragma unroll
         for (int p = 0; p < NB; p++) {
             const int e = threadIdx.x + 256 * p;
-            const int nn = TRB ? (e >> 4) : (e & (BN - 1));
-            const int kb = TRB ? (e & 15) : (e / BN);
+            const int nn = TRB ? (e / BK) : (e % BN);
+            const int kb = TRB ? (e % BK) : (e / BN);
             const int gn = n0 + nn, gkb = k0 + kb;
             T bv = T(0);
             if (gn < N && gkb < K) bv = TRB ? B[(long)gn * K + gkb] : B[(long)gkb * N + gn];
@@ -1247,13 +1251,13 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 #pragma unroll
         for (int p = 0; p < NA; p++) {
             const int e = threadIdx.x + 256 * p;
-            As[buf][e & 15][e >> 4] = ra[p];
+            As[buf][e % BK][e / BK] = ra[p];
         }
 #pragma unroll
         for (int p = 0; p < NB; p++) {
             const int e = threadIdx.x + 256 * p;
-            const int nn = TRB ? (e >> 4) : (e & (BN - 1));
-            const int kb = TRB ? (e & 15) : (e / BN);
+            const int nn = TRB ? (e / BK) : (e % BN);
+            const int kb = TRB ? (e % BK) : (e / BN);
             Bs[buf][kb][nn] = rb[p];
         }
         __syncthreads();
