@@ -70,7 +70,8 @@ struct Ops {
     static void viscosity(Ctx<T>& c, const T* cn, T* const mu[4]);
     static void momentum(Ctx<T>& c, int comp, double g, double r, int slot);
     // xmom, ymom and zmom together (needs F_SRC2/3, F_CC2/3)
-    static void momentum3(Ctx<T>& c, double g, double r, int slot);
+    // ghosts: the x-sweeps also write update_velo_boundary's ghost copies
+    static void momentum3(Ctx<T>& c, double g, double r, int slot, bool ghosts = false);
     static void velo_ghost(Ctx<T>& c);
     static void divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack);
     // ghosts: phi's face ghosts too (update_phi_ghosts) in the same pass
@@ -84,7 +85,8 @@ struct Ops {
     static void copy(Ctx<T>& c, const T* a, T* b, long n);
     // velocity correction, outflow deficit and the pressure-gradient update
     // of all three components (mu_new) in one pass over the cells
-    static void correction_and_gradient(Ctx<T>& c, double g, double r, int slot);
+    // ghosts: also update_veloghost_points' ghost copies of the corrected values
+    static void correction_and_gradient(Ctx<T>& c, double g, double r, int slot, bool ghosts = false);
     // shift_conc on field q: every (j,k) line re-sampled at Xc - shift[k]
     // (x: the NX cell centres, shift: NZ values, both on the device)
     static void shift_conc(Ctx<T>& c, T* q, const double* x, const double* shift);

@@ -336,12 +336,35 @@ struct LineSet {
     T* N;
     const T* Nm1;
     int ghost_c;          // also fill c_new's y/z ghosts (700-712)
+    int ghost_v;          // 0, or FU/FV/FW: also fill that component's ghosts (velo_ghost_point)
 };
 
 template <typename T>
 struct LineSets {
     LineSet<T> s[3];
 };
+
+// The ghost copies of update_velo_boundary / update_veloghost_points
+// (2701-2755, 2810-2885; k_velo_ghost) whose source is the new value q of
+// component F at interior point (i, j, k): z ghosts of u and v (copies),
+// y ghosts of u and w (negated), inlet ghosts of v and w (negated).  The
+// fused step writes them where q is written; ghost sources a step never
+// rewrites (u at the inlet face, v at the bottom wall, w at the side wall)
+// keep the ghosts they were given at set-up.
+template <typename T>
+__device__ __forceinline__ void velo_ghost_point(const Dims& d, int F, T* __restrict__ p, int i, int j, int k, T val)
+{
+    // p points at (i, j, k); i is in [3, nx + 2] for every caller
+    if (F != FW) {
+        if (k == 1) p[-1] = val;
+        if (k == d.nz) p[1] = val;
+    }
+    if (F != FV) {
+        if (j == 1) p[-d.sy] = -val;
+        if (j == d.ny) p[d.sy] = -val;
+    }
+    if (F != FU && i == 3) p[-d.sx] = -val;
+}
 
 template <typename T, int DIR>
 __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const double* __restrict__ scal, int slot,
@@ -471,6 +494,13 @@ __global__ __launch_bounds__(256) void k_sweep(Dims d, LineSets<T> sets, const d
         for (int t = 0; t < LB; t++) { bs[t] = qs[t]; bc[t] = qc[t]; bq[t] = qq[t]; }
     }
 
+    if (DIR == 0 && L.ghost_v) {
+        T* qn = L.qnew;
+        for (int ii = R.is; ii <= R.ie; ii++) {
+            T* p = qn + IX(d, ii, j, k);
+            velo_ghost_point<T>(d, L.ghost_v, p, ii, j, k, *p);
+        }
+    }
     if (DIR == 0 && L.ghost_c) {
         // ghost copies of this line's values: y ghosts (701-704), then z
         // ghosts incl. the y-ghost rows (707-712)
@@ -897,11 +927,13 @@ __device__ __forceinline__ T corr_pg_new(const Dims& d, const Phys& ph, const Co
 }
 
 template <typename T, int F>
-__device__ __forceinline__ void corr_apply(const Dims& d, const Phys& ph, const CorrOps<T>& c, T* __restrict__ q,
-                                           T* __restrict__ pg, long o, T fac, T facv)
+__device__ __forceinline__ T corr_apply(const Dims& d, const Phys& ph, const CorrOps<T>& c, T* __restrict__ q,
+                                        T* __restrict__ pg, long o, T fac, T facv)
 {
-    q[o] = corr_q<T, F>(d, c, facv);
+    const T qn = corr_q<T, F>(d, c, facv);
+    q[o] = qn;
     pg[o] = corr_pg_new<T, F>(d, ph, c, fac);
+    return qn;
 }
 
 // velo_correction (2144-2169): u -= dt*alpha/Re * grad(phi) on each
@@ -999,7 +1031,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
                                                  T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
                                                  T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
-                                                 const double* __restrict__ scal, T g, T r)
+                                                 const double* __restrict__ scal, T g, T r, int vghost)
 {
     const int2 bt = xcd_tile();
     const int k = bt.x * blockDim.x + threadIdx.x + 1;
@@ -1015,9 +1047,19 @@ __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, c
     if (dv) corr_load<T, FV>(d, mu, phi, v, pgv, o, cv);
     if (dw) corr_load<T, FW>(d, mu, phi, w, pgw, o, cw);
     if (du) corr_load<T, FU>(d, mu, phi, u, pgu, o, cu);
-    if (dv) corr_apply<T, FV>(d, ph, cv, v, pgv, o, fac, facv);
-    if (dw) corr_apply<T, FW>(d, ph, cw, w, pgw, o, fac, facv);
-    if (du) corr_apply<T, FU>(d, ph, cu, u, pgu, o, fac, facv);
+    // with vghost, update_veloghost_points for the values just written
+    if (dv) {
+        const T val = corr_apply<T, FV>(d, ph, cv, v, pgv, o, fac, facv);
+        if (vghost) velo_ghost_point<T>(d, FV, v + o, i, j, k, val);
+    }
+    if (dw) {
+        const T val = corr_apply<T, FW>(d, ph, cw, w, pgw, o, fac, facv);
+        if (vghost) velo_ghost_point<T>(d, FW, w + o, i, j, k, val);
+    }
+    if (du) {
+        const T val = corr_apply<T, FU>(d, ph, cu, u, pgu, o, fac, facv);
+        if (vghost) velo_ghost_point<T>(d, FU, u + o, i, j, k, val);
+    }
 }
 
 // pressure_gradient alone (2964-3044): pg += grad(phi) - fac A(mu_new)
@@ -1859,6 +1901,7 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
                 const T val = qo[t] + sol;
                 T* qn = L.qnew;
                 qn[o0 + o] = val;
+                if (L.ghost_v) velo_ghost_point<T>(d, L.ghost_v, qn + o0 + o, R.is + x0 + t, j, k, val);
                 if (L.ghost_c) {
                     // c_new's ghost copies of this value: y ghosts (701-704),
                     // then z ghosts incl. the y-ghost rows (707-712)
@@ -2353,6 +2396,7 @@ static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T
     else { L.bc0 = bc_kind(F, 4); L.bc1 = 0; }
     L.qold = qold; L.qnew = qnew; L.N = N; L.Nm1 = Nm1;
     L.ghost_c = (F == FC && DIR == 0) ? 1 : 0;
+    L.ghost_v = 0;
     return L;
 }
 
@@ -2369,7 +2413,8 @@ static inline int part_chunks(int nmin, int nmax)
 // y, z then x sweeps of up to three fields at once (one launch per direction)
 template <typename T>
 static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* const* S, T* const* CC,
-                     const Visc4<T>& mu, T* const* qold, T* const* qnew, T* const* N, T* const* Nm1)
+                     const Visc4<T>& mu, T* const* qold, T* const* qnew, T* const* N, T* const* Nm1,
+                     bool vghost = false)
 {
     static const char* nm[3] = {"sweep_x", "sweep_y", "sweep_z"};
     const int order[3] = {1, 2, 0};
@@ -2379,6 +2424,7 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
         long lines = 0;
         for (int q = 0; q < nset; q++) {
             ls.s[q] = line_set<T>(c, F[q], DIR, S[q], CC[q], mu, qold[q], qnew[q], N[q], Nm1[q]);
+            if (vghost && DIR == 0 && F[q] != FC) ls.s[q].ghost_v = F[q];
             const Rng& R = ls.s[q].R;
             const long nl = DIR == 0 ? (long)(R.je - R.js + 1) * (R.ke - R.ks + 1)
                           : DIR == 1 ? (long)(R.ie - R.is + 1) * (R.ke - R.ks + 1)
@@ -2508,7 +2554,7 @@ static void rhs_mom(Ctx<T>& c, T g, T r, T* S)
 // the components are independent (each reads only old velocities, c_new and
 // the viscosities), so this equals xmom; ymom; zmom (179-181).
 template <typename T>
-static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot)
+static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot, bool vghost = false)
 {
     const T g = T(gd), r = T(rd);
     int F[3], n = 0;
@@ -2526,7 +2572,7 @@ static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot)
         qo[n] = c.f[b]; qn[n] = c.f[b + 1]; N[n] = c.f[b + 3]; Nm[n] = c.f[b + 4];
         n++;
     }
-    adi_sets<T>(c, g, r, slot, F, n, S, CC, mun(c), qo, qn, N, Nm);
+    adi_sets<T>(c, g, r, slot, F, n, S, CC, mun(c), qo, qn, N, Nm, vghost);
 }
 
 template <typename T>
@@ -2536,9 +2582,9 @@ void Ops<T>::momentum(Ctx<T>& c, int comp, double g, double r, int slot)
 }
 
 template <typename T>
-void Ops<T>::momentum3(Ctx<T>& c, double g, double r, int slot)
+void Ops<T>::momentum3(Ctx<T>& c, double g, double r, int slot, bool ghosts)
 {
-    momentum_set<T>(c, 7, g, r, slot);
+    momentum_set<T>(c, 7, g, r, slot, ghosts);
 }
 
 template <typename T>
@@ -2623,13 +2669,13 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
 }
 
 template <typename T>
-void Ops<T>::correction_and_gradient(Ctx<T>& c, double g, double r, int slot)
+void Ops<T>::correction_and_gradient(Ctx<T>& c, double g, double r, int slot, bool ghosts)
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "corr_pg", (k_corr_pg<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_NEW], c.f[V_NEW],
                                                                c.f[W_NEW], c.f[U_PG], c.f[V_PG], c.f[W_PG], c.scal,
-                                                               T(g), T(r))));
+                                                               T(g), T(r), ghosts ? 1 : 0)));
     PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
     PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
     PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,

@@ -681,12 +681,11 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
         Ops<T>::viscosity(c, c.f[C_NEW], mun);
-        Ops<T>::momentum3(c, g, r, rk);
-        Ops<T>::velo_ghost(c);
+        Ops<T>::momentum3(c, g, r, rk, true);   // with update_velo_boundary
         Ops<T>::divergence(c, g, r, false, true);
         Ops<T>::poisson(c, g, r, false, true);   // with update_phi_ghosts
-        Ops<T>::correction_and_gradient(c, g, r, rk);   // velo_correction + pressure_gradient
-        Ops<T>::velo_ghost(c);
+        // velo_correction + pressure_gradient + update_veloghost_points
+        Ops<T>::correction_and_gradient(c, g, r, rk, true);
         if (s->copy_next) {
             update_old_copy<T>(s, c);
             s->copy_next = false;
