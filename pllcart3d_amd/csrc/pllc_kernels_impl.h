@@ -1359,6 +1359,224 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         }
 }
 
+// ------------------------------------------------- fast DCT of a plane
+//
+// The (y, z) transforms of phisolver_dct for ny = nz = 128 (fftw_3d.h
+// REDFT10 forward, REDFT01 inverse, unnormalised) by Makhoul's
+// reordering: a DCT-II of length N is the real part of a twiddled DFT of the
+// even samples followed by the reversed odd ones, and that real-input DFT of
+// length N is one complex DFT of length N/2 = 64 plus a split, done here as
+// 8 x 8 (two radix-8 passes with a twiddle and a transpose through LDS).
+// REDFT01 runs the same steps backwards.  One block per x-plane holds the
+// 128 x 128 plane in LDS (row pitch 129), transforms its 128 rows (z) and
+// then its 128 columns (y), 8 threads per line; in the inverse the result
+// goes to phi as in the dense path's epilogue.  O(N log N) per line instead
+// of the dense contractions' O(N^2), so the plane is a load and a store of
+// HBM plus LDS work.  Twiddles (host double precision, rounded to T):
+// tw[0..63] = e^{-2 pi i k/64}, tw[64..128] = e^{-2 pi i k/128} (k <= 64),
+// tw[129..256] = e^{-i pi k/256}, as (cos, sin) pairs.
+constexpr int FD_N = 128, FD_P = 129;
+
+template <typename T>
+struct Cx { T r, i; };
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> cconj(Cx<T> a) { return {a.r, -a.i}; }
+
+// in-place 8-point DFT, natural order; INV uses e^{+2 pi i/8}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(Cx<T> (&a)[8])
+{
+    const T h = T(0.70710678118654752440);
+    Cx<T> s[4], d[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        s[j] = {a[j].r + a[j + 4].r, a[j].i + a[j + 4].i};
+        d[j] = {a[j].r - a[j + 4].r, a[j].i - a[j + 4].i};
+    }
+    // d[j] *= w8^j
+    const T sg = INV ? T(1) : T(-1);
+    d[1] = {h * (d[1].r - sg * d[1].i), h * (d[1].i + sg * d[1].r)};
+    d[2] = {-sg * d[2].i, sg * d[2].r};
+    d[3] = {h * (-d[3].r - sg * d[3].i), h * (-d[3].i + sg * d[3].r)};
+    Cx<T>* p[2] = {s, d};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        Cx<T>* v = p[q];
+        const Cx<T> s0 = {v[0].r + v[2].r, v[0].i + v[2].i}, d0 = {v[0].r - v[2].r, v[0].i - v[2].i};
+        const Cx<T> s1 = {v[1].r + v[3].r, v[1].i + v[3].i};
+        const Cx<T> t1 = {v[1].r - v[3].r, v[1].i - v[3].i};
+        const Cx<T> d1 = {-sg * t1.i, sg * t1.r};   // t1 * w4
+        a[q] = {s0.r + s1.r, s0.i + s1.i};
+        a[q + 4] = {s0.r - s1.r, s0.i - s1.i};
+        a[q + 2] = {d0.r + d1.r, d0.i + d1.i};
+        a[q + 6] = {d0.r - d1.r, d0.i - d1.i};
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> twl(const T* __restrict__ tw, int k) { return {tw[2 * k], tw[2 * k + 1]}; }
+
+// complex value c of a line stored as reals (2c, 2c+1) at stride st
+template <typename T>
+__device__ __forceinline__ Cx<T> lcx(const T* L, int st, int c) { return {L[2 * c * st], L[(2 * c + 1) * st]}; }
+template <typename T>
+__device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v) { L[2 * c * st] = v.r; L[(2 * c + 1) * st] = v.i; }
+
+// 64-point DFT of the line's complex values (in place, through LDS), thread
+// t of the line's 8; every thread of the block calls it (block barriers)
+template <typename T, bool INV>
+__device__ __forceinline__ void fft64_line(T* L, int st, int t, const T* __restrict__ tw)
+{
+    Cx<T> a[8];
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) a[m1] = lcx(L, st, 8 * m1 + t);
+    dft8<T, INV>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < 8; k1++) {
+        const Cx<T> w = twl(tw, t * k1);
+        a[k1] = cmul(a[k1], INV ? cconj(w) : w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; k1++) scx(L, st, 8 * k1 + t, a[k1]);
+    __syncthreads();
+#pragma unroll
+    for (int m2 = 0; m2 < 8; m2++) a[m2] = lcx(L, st, 8 * t + m2);
+    dft8<T, INV>(a);
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) scx(L, st, t + 8 * k2, a[k2]);
+    __syncthreads();
+}
+
+// REDFT10 of one line of 128 reals at stride st, in place
+template <typename T>
+__device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __restrict__ tw)
+{
+    // z[m] = v[2m] + i v[2m+1], v = even samples then reversed odd ones
+    Cx<T> a[8];
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) {
+        const int m = 8 * m1 + t;
+        a[m1] = m1 < 4 ? Cx<T>{L[(4 * m) * st], L[(4 * m + 2) * st]}
+                       : Cx<T>{L[(255 - 4 * m) * st], L[(253 - 4 * m) * st]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m1 = 0; m1 < 8; m1++) scx(L, st, 8 * m1 + t, a[m1]);
+    __syncthreads();
+    fft64_line<T, false>(L, st, t, tw);
+    // V[k] = (Z[k] + conj Z[64-k])/2 - i/2 W128^k (Z[k] - conj Z[64-k]);
+    // X[k] = 2 Re(W512^k V[k]), X[128-k] = -2 Im(W512^k V[k])
+    T xr[9], xi[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const int k = t + 8 * j;
+        if (j < 8 || t == 0) {
+            const Cx<T> zk = lcx(L, st, k & 63), zc = cconj(lcx(L, st, (64 - k) & 63));
+            const Cx<T> ve = {T(0.5) * (zk.r + zc.r), T(0.5) * (zk.i + zc.i)};
+            const Cx<T> df = {zk.r - zc.r, zk.i - zc.i};
+            const Cx<T> vo = {T(0.5) * df.i, T(-0.5) * df.r};   // -i/2 (Zk - Zc)
+            const Cx<T> wv = cmul(twl(tw, 64 + k), vo);
+            const Cx<T> v = {ve.r + wv.r, ve.i + wv.i};
+            const Cx<T> q = cmul(twl(tw, 129 + (k & 127)), v);
+            xr[j] = T(2) * q.r;
+            xi[j] = T(-2) * q.i;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const int k = t + 8 * j;
+        if (j < 8 || t == 0) {
+            L[k * st] = xr[j];
+            if (k >= 1 && k <= 63) L[(128 - k) * st] = xi[j];
+        }
+    }
+    __syncthreads();
+}
+
+// REDFT01 of one line of 128 reals at stride st, in place
+template <typename T>
+__device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __restrict__ tw)
+{
+    // V[k] = conj(W512^k) (X[k] - i X[128-k]) (V[0] = X[0]); c[k] = E + i O with
+    // E = V[k] + conj V[64-k], O = (V[k] - conj V[64-k]) conj(W128^k)
+    auto V = [&](int k) -> Cx<T> {
+        if (k == 0) return Cx<T>{L[0], T(0)};
+        const Cx<T> x = {L[k * st], -L[(128 - k) * st]};
+        return cmul(cconj(twl(tw, 129 + k)), x);
+    };
+    Cx<T> c[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k = t + 8 * j;
+        const Cx<T> vk = V(k), vh = cconj(V(64 - k));
+        const Cx<T> e = {vk.r + vh.r, vk.i + vh.i};
+        const Cx<T> o = cmul(Cx<T>{vk.r - vh.r, vk.i - vh.i}, cconj(twl(tw, 64 + k)));
+        c[j] = {e.r - o.i, e.i + o.r};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; j++) scx(L, st, t + 8 * j, c[j]);
+    __syncthreads();
+    fft64_line<T, true>(L, st, t, tw);
+    // y[2n] = v[n], y[2n+1] = v[127-n], v[2m] = Re z[m], v[2m+1] = Im z[m]
+    Cx<T> z[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) z[k2] = lcx(L, st, t + 8 * k2);
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) {
+        const int m = t + 8 * k2;
+        if (k2 < 4) { L[(4 * m) * st] = z[k2].r; L[(4 * m + 2) * st] = z[k2].i; }
+        else { L[(255 - 4 * m) * st] = z[k2].r; L[(253 - 4 * m) * st] = z[k2].i; }
+    }
+    __syncthreads();
+}
+
+// the 2D transform of plane b: rows (z) then columns (y); out (or phi)
+template <typename T, bool INV>
+__global__ __launch_bounds__(1024) void k_dct_plane(const T* __restrict__ in, T* __restrict__ out,
+                                                    const T* __restrict__ tw, PhiOut<T> po)
+{
+    extern __shared__ __align__(16) unsigned char pllc_smem[];
+    T* pl = reinterpret_cast<T*>(pllc_smem);   // [128][FD_P]
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const long base = (long)b * FD_N * FD_N;
+    for (int e = tid; e < FD_N * FD_N; e += 1024) pl[(e >> 7) * FD_P + (e & 127)] = in[base + e];
+    __syncthreads();
+    const int line = tid >> 3, t = tid & 7;
+    if (INV) {
+        redft01_line<T>(pl + line * FD_P, 1, t, tw);
+        redft01_line<T>(pl + line, FD_P, t, tw);
+    } else {
+        redft10_line<T>(pl + line * FD_P, 1, t, tw);
+        redft10_line<T>(pl + line, FD_P, t, tw);
+    }
+    if (INV && po.phi) {
+        const T norm = T(4 * po.ny * po.nz);
+        for (int e = tid; e < FD_N * FD_N; e += 1024) {
+            const int gm = e >> 7, gn = e & 127;
+            const long o = (long)(3 + b) * po.sx + (long)(1 + gm) * po.sy + (1 + gn);
+            const T val = fdiv<T>(pl[gm * FD_P + gn], norm);
+            po.phi[o] = val;
+            if (po.ghosts) {
+                if (gn == 0) po.phi[o - 1] = val;
+                if (gn == po.nz - 1) po.phi[o + 1] = val;
+                if (gm == 0) po.phi[o - po.sy] = val;
+                if (gm == po.ny - 1) po.phi[o + po.sy] = val;
+                if (b == 0) po.phi[o - po.sx] = val;
+                if (b == po.nx - 1) po.phi[o + po.sx] = val;
+            }
+        }
+        return;
+    }
+    for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[(e >> 7) * FD_P + (e & 127)];
+}
+
 // ------------------------------------------------------------ utilities
 
 // stream-ordered store of one device scalar
@@ -1601,6 +1819,8 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
                                              PhiOut<T>);                                                        \
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long,         \
                                               PhiOut<T>);                                                       \
+    template __global__ void k_dct_plane<T, false>(const T*, T*, const T*, PhiOut<T>);                        \
+    template __global__ void k_dct_plane<T, true>(const T*, T*, const T*, PhiOut<T>);                         \
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
 
@@ -2605,9 +2825,38 @@ void Ops<T>::divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack
 }
 
 template <typename T>
+bool Ops<T>::dct_plane_ready()
+{
+    const int sm = FD_N * FD_P * (int)sizeof(T);
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || lim <= 0)
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
+    if (sm > lim) return false;
+    if (hipFuncSetAttribute((const void*)k_dct_plane<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sm) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void*)k_dct_plane<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sm) !=
+            hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
 void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool ghosts)
 {
     const int nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
+    if (c.tw && ny == FD_N && nz == FD_N) {
+        // 128 x 128 planes: the fast transform, one block per x-plane
+        const size_t sm = (size_t)FD_N * FD_P * sizeof(T);
+        const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+        if (inverse)
+            PLLC_L(c, "dct", (k_dct_plane<T, true><<<nx, 1024, sm, c.st>>>(in, out, c.tw, po)));
+        else
+            PLLC_L(c, "dct", (k_dct_plane<T, false><<<nx, 1024, sm, c.st>>>(in, out, c.tw, PhiOut<T>{})));
+        return;
+    }
     const long R = (long)nx * ny, plane = (long)ny * nz;
     const dim3 gz((nz + 127) / 128, (unsigned)((R + 63) / 64), 1), gy((nz + 127) / 128, (ny + 63) / 64, nx);
     if (!inverse) {

@@ -300,6 +300,25 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
     if (!put(c.Cfz, (size_t)d.nz * d.nz)) return false;
     pllc_dct_matrix(d.nz, true, h.data());
     if (!put(c.Ciz, (size_t)d.nz * d.nz)) return false;
+    // the fast plane transform for 128 x 128 planes (PLLC_DCT=gemm keeps the
+    // dense contractions), when the device gives a block the plane's LDS
+    const char* dm = getenv("PLLC_DCT");
+    if (d.ny == 128 && d.nz == 128 && !(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_plane_ready()) {
+        std::vector<double> w(514);
+        const double pi = 3.14159265358979323846;
+        for (int k = 0; k < 64; k++) { w[2 * k] = cos(2 * pi * k / 64); w[2 * k + 1] = -sin(2 * pi * k / 64); }
+        for (int k = 0; k <= 64; k++) {
+            w[2 * (64 + k)] = cos(2 * pi * k / 128);
+            w[2 * (64 + k) + 1] = -sin(2 * pi * k / 128);
+        }
+        for (int k = 0; k < 128; k++) {
+            w[2 * (129 + k)] = cos(pi * k / 256);
+            w[2 * (129 + k) + 1] = -sin(pi * k / 256);
+        }
+        if (!get((void**)&c.tw, 514 * ts)) return false;
+        for (size_t e = 0; e < 514; e++) ht[e] = T(w[e]);
+        if (!pllc_hip_ok(where, hipMemcpy(c.tw, ht.data(), 514 * ts, hipMemcpyHostToDevice))) return false;
+    }
     return true;
 }
 template bool pllc_ctx_tables<double>(Ctx<double>&, std::vector<void*>&, const char*);
