@@ -38,6 +38,9 @@ SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
 def touches_per_launch(k, launches_per_step):
     if k in SWEEP_TOUCHES_PER_SUBSTAGE:
         return SWEEP_TOUCHES_PER_SUBSTAGE[k] * 3.0 / launches_per_step
+    if k == "dct":
+        # fast plane transform: the packed block read and written once
+        return 2.0 if launches_per_step < 6.5 else None
     return TOUCHES.get(k)
 
 
@@ -116,7 +119,9 @@ def roofline(kernels, cells, bytes_per_value, prm):
     tmb = MEASURED_TRAFFIC_MB.get(bytes_per_value, {}).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
     traffic = {dom: tmb * 1e6} if tmb else None
     avg_s = kernels[dom]["avg_ms"] * 1e-3
-    if dom == "dct":
+    if dom == "dct" and kernels[dom]["launches_per_step"] > 6.5:
+        # the dense contractions (four launches per substage) are matrix-core
+        # work; the fast plane transform (two per substage) counts as HBM
         fl = dct_flops_per_launch(prm)
         ach = fl / avg_s / 1e12
         peak = PEAK_TFLOPS[bytes_per_value]
