@@ -60,23 +60,3 @@ def test_mixed_128_planes_track_fp64_oracle(oracle_mod):
     print("[mixed 8x128x128] after 10 steps: %s" % " ".join("%s=%.2e" % kv for kv in errs.items()), flush=True)
     for f, e in errs.items():
         assert e <= TOL[f], (f, e)
-
-
-def test_mixed_fused_equals_mixed_ops():
-    """The fused step (pointer swaps, batched u/v/w sweeps) and the per-op
-    sequence are the same arithmetic in FP32 too."""
-    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
-    c0 = P.ic_fill(prm, P.IC_NOISE, 3, 0.05)
-    a = P.Sim(prm, "mixed"); a.init(c0)
-    b = P.Sim(prm, "mixed"); b.init(c0)
-    a.run(2)
-    for _ in range(2):
-        dt = min(b.op("FIND_CFL"), b.timestep)
-        for rk in range(3):
-            vb = b.op("BULK_VELO", rk=rk)
-            for op in ["CONCENTRATION", "VISCOSITY", "XMOM", "YMOM", "ZMOM", "VELO_BOUNDARY", "DIVERGENCE",
-                       "PHISOLVER", "PHI_GHOSTS", "VELO_CORRECTION", "VELOGHOST", "PRESSURE_GRADIENT",
-                       "UPDATE_OLD"]:
-                b.op(op, dt, rk, vb)
-    for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG"):
-        assert np.array_equal(a.download(f), b.download(f)), f
