@@ -892,7 +892,7 @@ __device__ __forceinline__ void corr_load(const Dims& d, const Visc4<T>& mu, con
     c.pg = pg[o];
 }
 
-// the phi gradient g of component F at gradient point g (pgrad)
+// the phi gradient of component F at gradient point g (o, o -/+ sx, ...)
 template <typename T, int F>
 __device__ __forceinline__ T corr_grad(const Dims& d, const CorrOps<T>& c, int g)
 {
