@@ -607,9 +607,10 @@ static void conc_coef(const orc_sim *s, int d, double *a, double *b, double *c)
 
 /* ---------------------------------------------------------- ADI machinery */
 
-/* boundary rows of the ADI systems (App. A of the survey; pllcart3d.c
- * 610-611, 640-641, 674-676, 1010-1011, 1039-1040, 1075, 1167-1168,
- * 1196-1197, 1231, 1317-1318, 1344-1345, 1378) */
+/* boundary rows of the ADI systems per field and side (Neumann, Dirichlet,
+ * or the inlet/outlet right-hand-side rule; pllcart3d.c 610-611, 640-641,
+ * 674-676, 1010-1011, 1039-1040, 1075, 1167-1168, 1196-1197, 1231,
+ * 1317-1318, 1344-1345, 1378) */
 enum { BC_NEU = 0, BC_DIR, BC_RHS0 };
 static const int bc_tab[4][5] = {
     /* y-lo    y-hi    z-lo     z-hi     x-lo */
