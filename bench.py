@@ -35,12 +35,18 @@ TOUCHES = {
 SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
 
 
-def touches_per_launch(k, launches_per_step):
+def dct_is_fast(prm):
+    """ny = nz = 128 or 256 run the fast transform (unless PLLC_DCT=gemm):
+    every launch reads and writes the packed block once; other plane sizes
+    run the dense contractions on the matrix cores."""
+    return prm.ny == prm.nz and prm.ny in (128, 256) and os.environ.get("PLLC_DCT") != "gemm"
+
+
+def touches_per_launch(k, launches_per_step, prm):
     if k in SWEEP_TOUCHES_PER_SUBSTAGE:
         return SWEEP_TOUCHES_PER_SUBSTAGE[k] * 3.0 / launches_per_step
     if k == "dct":
-        # fast plane transform: the packed block read and written once
-        return 2.0 if launches_per_step < 6.5 else None
+        return 2.0 if dct_is_fast(prm) else None
     return TOUCHES.get(k)
 
 
@@ -119,16 +125,16 @@ def roofline(kernels, cells, bytes_per_value, prm):
     tmb = MEASURED_TRAFFIC_MB.get(bytes_per_value, {}).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
     traffic = {dom: tmb * 1e6} if tmb else None
     avg_s = kernels[dom]["avg_ms"] * 1e-3
-    if dom == "dct" and kernels[dom]["launches_per_step"] > 6.5:
-        # the dense contractions (four launches per substage) are matrix-core
-        # work; the fast plane transform (two per substage) counts as HBM
+    if dom == "dct" and not dct_is_fast(prm):
+        # the dense contractions are matrix-core work; the fast transform
+        # counts as HBM
         fl = dct_flops_per_launch(prm)
         ach = fl / avg_s / 1e12
         peak = PEAK_TFLOPS[bytes_per_value]
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": (traffic or {}).get(dom), "kernel": dom,
                 "algorithmic_flops_per_launch": fl, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
-    t = touches_per_launch(dom, kernels[dom]["launches_per_step"])
+    t = touches_per_launch(dom, kernels[dom]["launches_per_step"], prm)
     if t is None:
         return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": dom}
@@ -139,11 +145,11 @@ def roofline(kernels, cells, bytes_per_value, prm):
             "algorithmic_bytes_per_launch": ab, "avg_launch_ms": round(kernels[dom]["avg_ms"], 5)}
 
 
-def hbm_table(kernels, cells, bytes_per_value):
+def hbm_table(kernels, cells, bytes_per_value, prm):
     """Achieved algorithmic GB/s of every kernel class with an AB model."""
     out = {}
     for k, v in kernels.items():
-        t = touches_per_launch(k, v["launches_per_step"])
+        t = touches_per_launch(k, v["launches_per_step"], prm)
         if t is None or not v["avg_ms"]:
             continue
         out[k] = round(t * cells * bytes_per_value / (v["avg_ms"] * 1e-3) / 1e9, 1)
@@ -268,8 +274,8 @@ def main():
                   "speedup_vs_plain": ms / ms_m,
                   "rel_l2_vs_fp64_after_%d_steps" % (a.warmup + a.steps): err,
                   "roofline": roofline(kernels_m, cells, 4.0, prm) if kernels_m else None,
-                  "kernel_gbs": hbm_table(kernels_m, cells, 4.0) if kernels_m else None},
-        "kernel_gbs": hbm_table(kernels, cells, 8.0) if kernels else None,
+                  "kernel_gbs": hbm_table(kernels_m, cells, 4.0, prm) if kernels_m else None},
+        "kernel_gbs": hbm_table(kernels, cells, 8.0, prm) if kernels else None,
         "ensemble": ens,
         "large": large,
         "sweep_lines_per_block": {k: {"x": v[0], "y": v[1], "z": v[2]} for k, v in tuned.items()},

@@ -44,8 +44,9 @@ struct Ctx {
     T* Cfz = nullptr;
     T* Ciy = nullptr;    // REDFT01 matrices
     T* Ciz = nullptr;
-    // twiddles of the fast plane transform (ny = nz = 128): 257 (cos, sin)
-    // pairs, see k_dct_plane; null when the dense contractions are used
+    // twiddles of the fast transform (ny = nz = N, N = 128 or 256): 2N + 1
+    // (cos, sin) pairs, see k_dct_plane / k_dct_lines; null when the dense
+    // contractions are used
     T* tw = nullptr;
     double* scal = nullptr;   // S_SIZE device scalars
     double* part = nullptr;   // reduction partials
@@ -99,9 +100,10 @@ struct Ops {
     // (inverse), z then y, `in` -> `out` through the temporary c.Q
     // phi (inverse only): the result goes to phi's interior instead of out
     static void dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi = nullptr, bool ghosts = false);
-    // whether the device gives one block the LDS of a 128 x 128 plane of T
-    // (and the fast plane transform is set up to use it)
-    static bool dct_plane_ready();
+    // whether there is a fast transform for ny = nz = n and the device gives
+    // its blocks the LDS it needs (n = 128: a whole plane of T; n = 256: 32
+    // lines), with the kernels set up to use it
+    static bool dct_fast_ready(int n);
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)

@@ -1116,17 +1116,28 @@ __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict
 
 // pass 2: max over the partials; dt = min(CFLmax/max, Timestep)
 // (pllcart3d.c:159-160, 3115)
+// (one block of 1024 threads, eight loads per thread in flight per round;
+// a max in any order is the same value)
 template <typename T>
-__global__ __launch_bounds__(256) void k_cfl_final(const double* __restrict__ part, long np, double CFLmax,
-                                                   double Timestep, double* __restrict__ scal)
+__global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ part, long np, double CFLmax,
+                                                    double Timestep, double* __restrict__ scal)
 {
-    __shared__ double red[256];
+    constexpr int NT = 1024, U = 8;
+    __shared__ double red[NT];
     const int t = threadIdx.x;
     double m = 0.0;
-    for (long l = t; l < np; l += blockDim.x) m = m > part[l] ? m : part[l];
+    long l = t;
+    for (; l + (U - 1) * NT < np; l += U * NT) {
+        double a[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) a[q] = part[l + q * NT];
+#pragma unroll
+        for (int q = 0; q < U; q++) m = m > a[q] ? m : a[q];
+    }
+    for (; l < np; l += NT) m = m > part[l] ? m : part[l];
     red[t] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if (t < s) red[t] = red[t] > red[t + s] ? red[t] : red[t + s];
         __syncthreads();
     }
@@ -1240,6 +1251,23 @@ template <> struct Mma<float> {
 template <typename T>
 struct PhiOut { T* phi; long sx, sy; int nx, ny, nz, ghosts; };
 
+// phi's interior point (3 + b, 1 + gm, 1 + gn) from the inverse transform's
+// value val (already normalised), and with ghosts its face ghost copies
+template <typename T>
+__device__ __forceinline__ void phi_store(const PhiOut<T>& po, int b, int gm, int gn, T val)
+{
+    const long o = (long)(3 + b) * po.sx + (long)(1 + gm) * po.sy + (1 + gn);
+    po.phi[o] = val;
+    if (po.ghosts) {
+        if (gn == 0) po.phi[o - 1] = val;
+        if (gn == po.nz - 1) po.phi[o + 1] = val;
+        if (gm == 0) po.phi[o - po.sy] = val;
+        if (gm == po.ny - 1) po.phi[o + po.sy] = val;
+        if (b == 0) po.phi[o - po.sx] = val;
+        if (b == po.nx - 1) po.phi[o + po.sx] = val;
+    }
+}
+
 #ifndef GEMM_BK
 #define GEMM_BK 16
 #endif
@@ -1330,17 +1358,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
                 for (int r = 0; r < 4; r++) {
                     const int gm = m0 + wm + 16 * s + Mma<T>::row(lk, r);
                     if (gm >= M) continue;
-                    const long o = (long)(3 + b) * po.sx + (long)(1 + gm) * po.sy + (1 + gn);
-                    const T val = fdiv<T>(acc[s][t][r], norm);
-                    po.phi[o] = val;
-                    if (po.ghosts) {
-                        if (gn == 0) po.phi[o - 1] = val;
-                        if (gn == po.nz - 1) po.phi[o + 1] = val;
-                        if (gm == 0) po.phi[o - po.sy] = val;
-                        if (gm == po.ny - 1) po.phi[o + po.sy] = val;
-                        if (b == 0) po.phi[o - po.sx] = val;
-                        if (b == po.nx - 1) po.phi[o + po.sx] = val;
-                    }
+                    phi_store(po, b, gm, gn, fdiv<T>(acc[s][t][r], norm));
                 }
             }
         return;
@@ -1359,23 +1377,23 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         }
 }
 
-// ------------------------------------------------- fast DCT of a plane
+// ------------------------------------------------- fast DCT of planes
 //
-// The (y, z) transforms of phisolver_dct for ny = nz = 128 (fftw_3d.h
-// REDFT10 forward, REDFT01 inverse, unnormalised) by Makhoul's
+// The (y, z) transforms of phisolver_dct for ny = nz = N, N = 128 or 256
+// (fftw_3d.h REDFT10 forward, REDFT01 inverse, unnormalised) by Makhoul's
 // reordering: a DCT-II of length N is the real part of a twiddled DFT of the
 // even samples followed by the reversed odd ones, and that real-input DFT of
-// length N is one complex DFT of length N/2 = 64 plus a split, done here as
-// 8 x 8 (two radix-8 passes with a twiddle and a transpose through LDS).
-// REDFT01 runs the same steps backwards.  One block per x-plane holds the
-// 128 x 128 plane in LDS (row pitch 129), transforms its 128 rows (z) and
-// then its 128 columns (y), 8 threads per line; in the inverse the result
-// goes to phi as in the dense path's epilogue.  O(N log N) per line instead
-// of the dense contractions' O(N^2), so the plane is a load and a store of
-// HBM plus LDS work.  Twiddles (host double precision, rounded to T):
-// tw[0..63] = e^{-2 pi i k/64}, tw[64..128] = e^{-2 pi i k/128} (k <= 64),
-// tw[129..256] = e^{-i pi k/256}, as (cos, sin) pairs.
+// length N is one complex DFT of length M = N/2 plus a split.  The complex
+// DFT is four-step: M = 64 as 8 x 8 and M = 128 as 16 x 8 (radix-8 passes in
+// registers, a twiddle, a transpose through LDS).  REDFT01 runs the same
+// steps backwards.  Eight threads transform one line held in LDS.
+// O(N log N) per line instead of the dense contractions' O(N^2), so a pass is
+// a load and a store of HBM plus LDS work.  Twiddles for N (host double
+// precision, rounded to T), as (cos, sin) pairs: tw[0 .. M-1] =
+// e^{-2 pi i k/M}, tw[M .. M+N/2] = e^{-2 pi i k/N}, tw[N+1 .. 2N] =
+// e^{-i pi k/(2N)}, 2N + 1 values.
 constexpr int FD_N = 128, FD_P = 129;
+constexpr int fd_tw_len(int n) { return 2 * n + 1; }
 
 template <typename T>
 struct Cx { T r, i; };
@@ -1418,129 +1436,168 @@ __device__ __forceinline__ void dft8(Cx<T> (&a)[8])
 template <typename T>
 __device__ __forceinline__ Cx<T> twl(const T* __restrict__ tw, int k) { return {tw[2 * k], tw[2 * k + 1]}; }
 
+// in-place 16-point DFT as two 8-point DFTs (even, odd samples) and one
+// radix-2 step; w16^k = tw[ws k] (the table's e^{-2 pi i k/M}, ws = M/16)
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(Cx<T> (&a)[16], const T* __restrict__ tw, int ws)
+{
+    Cx<T> e[8], o[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) { e[j] = a[2 * j]; o[j] = a[2 * j + 1]; }
+    dft8<T, INV>(e);
+    dft8<T, INV>(o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        Cx<T> t = o[k];
+        if (k > 0) {
+            const Cx<T> w = twl(tw, ws * k);
+            t = cmul(t, INV ? cconj(w) : w);
+        }
+        a[k] = {e[k].r + t.r, e[k].i + t.i};
+        a[k + 8] = {e[k].r - t.r, e[k].i - t.i};
+    }
+}
+
 // complex value c of a line stored as reals (2c, 2c+1) at stride st
 template <typename T>
 __device__ __forceinline__ Cx<T> lcx(const T* L, int st, int c) { return {L[2 * c * st], L[(2 * c + 1) * st]}; }
 template <typename T>
 __device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v) { L[2 * c * st] = v.r; L[(2 * c + 1) * st] = v.i; }
 
-// 64-point DFT of the line's complex values (in place, through LDS), thread
-// t of the line's 8; every thread of the block calls it (block barriers)
-template <typename T, bool INV>
-__device__ __forceinline__ void fft64_line(T* L, int st, int t, const T* __restrict__ tw)
+// M-point DFT (M = 64 or 128) of the line's complex values, in place
+// through LDS, natural order in and out; thread t of the line's 8.  Four
+// steps with M = P x 8: thread t takes the P values n = 8 n1 + t, DFTs them
+// over n1, twiddles by e^{-2 pi i t k1/M} and stores at 8 k1 + t; then the
+// 8-point DFTs over the contiguous 8 k1 .. 8 k1 + 7 (P/8 of them per thread)
+// give X[k1 + P k2].  Every thread of the block calls it (block barriers).
+template <typename T, bool INV, int M>
+__device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restrict__ tw)
 {
-    Cx<T> a[8];
+    static_assert(M == 64 || M == 128, "fft_line: M is 64 or 128");
+    constexpr int P = M / 8;
+    Cx<T> a[P];
 #pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) a[m1] = lcx(L, st, 8 * m1 + t);
-    dft8<T, INV>(a);
+    for (int n1 = 0; n1 < P; n1++) a[n1] = lcx(L, st, 8 * n1 + t);
+    if constexpr (P == 8) dft8<T, INV>(a);
+    else dft16<T, INV>(a, tw, M / 16);
 #pragma unroll
-    for (int k1 = 1; k1 < 8; k1++) {
+    for (int k1 = 1; k1 < P; k1++) {
         const Cx<T> w = twl(tw, t * k1);
         a[k1] = cmul(a[k1], INV ? cconj(w) : w);
     }
     __syncthreads();
 #pragma unroll
-    for (int k1 = 0; k1 < 8; k1++) scx(L, st, 8 * k1 + t, a[k1]);
+    for (int k1 = 0; k1 < P; k1++) scx(L, st, 8 * k1 + t, a[k1]);
     __syncthreads();
+    Cx<T> b[P / 8][8];
 #pragma unroll
-    for (int m2 = 0; m2 < 8; m2++) a[m2] = lcx(L, st, 8 * t + m2);
-    dft8<T, INV>(a);
-    __syncthreads();
+    for (int g = 0; g < P / 8; g++) {
 #pragma unroll
-    for (int k2 = 0; k2 < 8; k2++) scx(L, st, t + 8 * k2, a[k2]);
-    __syncthreads();
-}
-
-// REDFT10 of one line of 128 reals at stride st, in place
-template <typename T>
-__device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __restrict__ tw)
-{
-    // z[m] = v[2m] + i v[2m+1], v = even samples then reversed odd ones
-    Cx<T> a[8];
-#pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) {
-        const int m = 8 * m1 + t;
-        a[m1] = m1 < 4 ? Cx<T>{L[(4 * m) * st], L[(4 * m + 2) * st]}
-                       : Cx<T>{L[(255 - 4 * m) * st], L[(253 - 4 * m) * st]};
+        for (int m2 = 0; m2 < 8; m2++) b[g][m2] = lcx(L, st, 8 * (t + 8 * g) + m2);
+        dft8<T, INV>(b[g]);
     }
     __syncthreads();
 #pragma unroll
-    for (int m1 = 0; m1 < 8; m1++) scx(L, st, 8 * m1 + t, a[m1]);
-    __syncthreads();
-    fft64_line<T, false>(L, st, t, tw);
-    // V[k] = (Z[k] + conj Z[64-k])/2 - i/2 W128^k (Z[k] - conj Z[64-k]);
-    // X[k] = 2 Re(W512^k V[k]), X[128-k] = -2 Im(W512^k V[k])
-    T xr[9], xi[9];
+    for (int g = 0; g < P / 8; g++)
 #pragma unroll
-    for (int j = 0; j < 9; j++) {
+        for (int k2 = 0; k2 < 8; k2++) scx(L, st, t + 8 * g + P * k2, b[g][k2]);
+    __syncthreads();
+}
+
+// REDFT10 of one line of N reals at stride st, in place
+template <typename T, int N>
+__device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __restrict__ tw)
+{
+    constexpr int M = N / 2, V = M / 8;
+    // z[m] = v[2m] + i v[2m+1], v = even samples then reversed odd ones
+    Cx<T> a[V];
+#pragma unroll
+    for (int m1 = 0; m1 < V; m1++) {
+        const int m = 8 * m1 + t;
+        a[m1] = m < N / 4 ? Cx<T>{L[(4 * m) * st], L[(4 * m + 2) * st]}
+                          : Cx<T>{L[(2 * N - 1 - 4 * m) * st], L[(2 * N - 3 - 4 * m) * st]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m1 = 0; m1 < V; m1++) scx(L, st, 8 * m1 + t, a[m1]);
+    __syncthreads();
+    fft_line<T, false, M>(L, st, t, tw);
+    // V[k] = (Z[k] + conj Z[M-k])/2 - i/2 WN^k (Z[k] - conj Z[M-k]);
+    // X[k] = 2 Re(W4N^k V[k]), X[N-k] = -2 Im(W4N^k V[k])
+    T xr[V + 1], xi[V + 1];
+#pragma unroll
+    for (int j = 0; j <= V; j++) {
         const int k = t + 8 * j;
-        if (j < 8 || t == 0) {
-            const Cx<T> zk = lcx(L, st, k & 63), zc = cconj(lcx(L, st, (64 - k) & 63));
+        if (j < V || t == 0) {
+            const Cx<T> zk = lcx(L, st, k & (M - 1)), zc = cconj(lcx(L, st, (M - k) & (M - 1)));
             const Cx<T> ve = {T(0.5) * (zk.r + zc.r), T(0.5) * (zk.i + zc.i)};
             const Cx<T> df = {zk.r - zc.r, zk.i - zc.i};
             const Cx<T> vo = {T(0.5) * df.i, T(-0.5) * df.r};   // -i/2 (Zk - Zc)
-            const Cx<T> wv = cmul(twl(tw, 64 + k), vo);
+            const Cx<T> wv = cmul(twl(tw, M + k), vo);
             const Cx<T> v = {ve.r + wv.r, ve.i + wv.i};
-            const Cx<T> q = cmul(twl(tw, 129 + (k & 127)), v);
+            const Cx<T> q = cmul(twl(tw, N + 1 + k), v);
             xr[j] = T(2) * q.r;
             xi[j] = T(-2) * q.i;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 9; j++) {
+    for (int j = 0; j <= V; j++) {
         const int k = t + 8 * j;
-        if (j < 8 || t == 0) {
+        if (j < V || t == 0) {
             L[k * st] = xr[j];
-            if (k >= 1 && k <= 63) L[(128 - k) * st] = xi[j];
+            if (k >= 1 && k <= M - 1) L[(N - k) * st] = xi[j];
         }
     }
     __syncthreads();
 }
 
-// REDFT01 of one line of 128 reals at stride st, in place
-template <typename T>
+// REDFT01 of one line of N reals at stride st, in place
+template <typename T, int N>
 __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __restrict__ tw)
 {
-    // V[k] = conj(W512^k) (X[k] - i X[128-k]) (V[0] = X[0]); c[k] = E + i O with
-    // E = V[k] + conj V[64-k], O = (V[k] - conj V[64-k]) conj(W128^k)
-    auto V = [&](int k) -> Cx<T> {
+    constexpr int M = N / 2, V = M / 8;
+    // Vk = conj(W4N^k) (X[k] - i X[N-k]) (V0 = X[0]); c[k] = E + i O with
+    // E = Vk + conj V(M-k), O = (Vk - conj V(M-k)) conj(WN^k)
+    auto vk_of = [&](int k) -> Cx<T> {
         if (k == 0) return Cx<T>{L[0], T(0)};
-        const Cx<T> x = {L[k * st], -L[(128 - k) * st]};
-        return cmul(cconj(twl(tw, 129 + k)), x);
+        const Cx<T> x = {L[k * st], -L[(N - k) * st]};
+        return cmul(cconj(twl(tw, N + 1 + k)), x);
     };
-    Cx<T> c[8];
+    Cx<T> c[V];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < V; j++) {
         const int k = t + 8 * j;
-        const Cx<T> vk = V(k), vh = cconj(V(64 - k));
+        const Cx<T> vk = vk_of(k), vh = cconj(vk_of(M - k));
         const Cx<T> e = {vk.r + vh.r, vk.i + vh.i};
-        const Cx<T> o = cmul(Cx<T>{vk.r - vh.r, vk.i - vh.i}, cconj(twl(tw, 64 + k)));
+        const Cx<T> o = cmul(Cx<T>{vk.r - vh.r, vk.i - vh.i}, cconj(twl(tw, M + k)));
         c[j] = {e.r - o.i, e.i + o.r};
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 8; j++) scx(L, st, t + 8 * j, c[j]);
+    for (int j = 0; j < V; j++) scx(L, st, t + 8 * j, c[j]);
     __syncthreads();
-    fft64_line<T, true>(L, st, t, tw);
-    // y[2n] = v[n], y[2n+1] = v[127-n], v[2m] = Re z[m], v[2m+1] = Im z[m]
-    Cx<T> z[8];
+    fft_line<T, true, M>(L, st, t, tw);
+    // y[2n] = v[n], y[2n+1] = v[N-1-n], v[2m] = Re z[m], v[2m+1] = Im z[m]
+    Cx<T> z[V];
 #pragma unroll
-    for (int k2 = 0; k2 < 8; k2++) z[k2] = lcx(L, st, t + 8 * k2);
+    for (int k2 = 0; k2 < V; k2++) z[k2] = lcx(L, st, t + 8 * k2);
     __syncthreads();
 #pragma unroll
-    for (int k2 = 0; k2 < 8; k2++) {
+    for (int k2 = 0; k2 < V; k2++) {
         const int m = t + 8 * k2;
-        if (k2 < 4) { L[(4 * m) * st] = z[k2].r; L[(4 * m + 2) * st] = z[k2].i; }
-        else { L[(255 - 4 * m) * st] = z[k2].r; L[(253 - 4 * m) * st] = z[k2].i; }
+        if (m < N / 4) { L[(4 * m) * st] = z[k2].r; L[(4 * m + 2) * st] = z[k2].i; }
+        else { L[(2 * N - 1 - 4 * m) * st] = z[k2].r; L[(2 * N - 3 - 4 * m) * st] = z[k2].i; }
     }
     __syncthreads();
 }
 
-// the 2D transform of plane b: rows (z) then columns (y); out (or phi)
+// N = 128: the 2D transform of plane b in one block (the plane in LDS, row
+// pitch 129): rows (z) then columns (y), 8 threads per line; the inverse
+// writes phi (or out).  in may equal out: a block reads and writes only its
+// own plane.
 template <typename T, bool INV>
-__global__ __launch_bounds__(1024) void k_dct_plane(const T* __restrict__ in, T* __restrict__ out,
-                                                    const T* __restrict__ tw, PhiOut<T> po)
+__global__ __launch_bounds__(1024) void k_dct_plane(const T* in, T* out, const T* __restrict__ tw, PhiOut<T> po)
 {
     extern __shared__ __align__(16) unsigned char pllc_smem[];
     T* pl = reinterpret_cast<T*>(pllc_smem);   // [128][FD_P]
@@ -1550,31 +1607,69 @@ __global__ __launch_bounds__(1024) void k_dct_plane(const T* __restrict__ in, T*
     __syncthreads();
     const int line = tid >> 3, t = tid & 7;
     if (INV) {
-        redft01_line<T>(pl + line * FD_P, 1, t, tw);
-        redft01_line<T>(pl + line, FD_P, t, tw);
+        redft01_line<T, FD_N>(pl + line * FD_P, 1, t, tw);
+        redft01_line<T, FD_N>(pl + line, FD_P, t, tw);
     } else {
-        redft10_line<T>(pl + line * FD_P, 1, t, tw);
-        redft10_line<T>(pl + line, FD_P, t, tw);
+        redft10_line<T, FD_N>(pl + line * FD_P, 1, t, tw);
+        redft10_line<T, FD_N>(pl + line, FD_P, t, tw);
     }
     if (INV && po.phi) {
         const T norm = T(4 * po.ny * po.nz);
-        for (int e = tid; e < FD_N * FD_N; e += 1024) {
-            const int gm = e >> 7, gn = e & 127;
-            const long o = (long)(3 + b) * po.sx + (long)(1 + gm) * po.sy + (1 + gn);
-            const T val = fdiv<T>(pl[gm * FD_P + gn], norm);
-            po.phi[o] = val;
-            if (po.ghosts) {
-                if (gn == 0) po.phi[o - 1] = val;
-                if (gn == po.nz - 1) po.phi[o + 1] = val;
-                if (gm == 0) po.phi[o - po.sy] = val;
-                if (gm == po.ny - 1) po.phi[o + po.sy] = val;
-                if (b == 0) po.phi[o - po.sx] = val;
-                if (b == po.nx - 1) po.phi[o + po.sx] = val;
-            }
-        }
+        for (int e = tid; e < FD_N * FD_N; e += 1024)
+            phi_store(po, b, e >> 7, e & 127, fdiv<T>(pl[(e >> 7) * FD_P + (e & 127)], norm));
         return;
     }
     for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[(e >> 7) * FD_P + (e & 127)];
+}
+
+// N = 256: a plane does not fit a block's LDS, so the 2D transform is two
+// launches of FL lines per block (8 FL threads, 8 per line; LDS [FL][N + 1]):
+// AX 2 transforms rows (z, contiguous): block B the packed rows FL B ..
+// FL B + FL - 1; AX 1 transforms columns (y, stride N) of an FL-wide slab of
+// z: block B plane B / (N/FL), z from FL (B % (N/FL)), staged as [z][y] so
+// each column is a unit-stride LDS line.  The forward runs AX 2 then AX 1,
+// the inverse AX 1 then AX 2, whose epilogue writes phi.  in may equal out:
+// a block reads and writes only its own lines.  FL: 16 lines in FP64, 32 in
+// FP32 (33 KB of LDS either way; measured fastest of 16 and 32 in each).
+template <typename T>
+constexpr int fl_lines() { return sizeof(T) == 8 ? 16 : 32; }
+template <typename T, bool INV, int N, int AX, int FL>
+__global__ __launch_bounds__(8 * FL) void k_dct_lines(const T* in, T* out, const T* __restrict__ tw, PhiOut<T> po)
+{
+    extern __shared__ __align__(16) unsigned char pllc_smem[];
+    T* ls = reinterpret_cast<T*>(pllc_smem);   // [FL][N + 1]
+    constexpr int P = N + 1, E = FL * N, NT = 8 * FL;
+    const int tid = threadIdx.x;
+    const long blk = blockIdx.x;
+    const long base = AX == 2 ? blk * E : (blk / (N / FL)) * N * N + (blk % (N / FL)) * FL;
+    // element e of the block: its LDS slot and its offset from base
+    auto slot = [&](int e) { return AX == 2 ? (e / N) * P + e % N : (e % FL) * P + e / FL; };
+    auto goff = [&](int e) { return AX == 2 ? (long)e : (long)(e / FL) * N + e % FL; };
+    constexpr int R = E / NT;   // values per thread
+    {
+        // every load of the thread in flight before the first LDS write
+        T v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = in[base + goff(tid + NT * r)];
+#pragma unroll
+        for (int r = 0; r < R; r++) ls[slot(tid + NT * r)] = v[r];
+    }
+    __syncthreads();
+    const int line = tid >> 3, t = tid & 7;
+    if (INV) redft01_line<T, N>(ls + line * P, 1, t, tw);
+    else redft10_line<T, N>(ls + line * P, 1, t, tw);
+    if (AX == 2 && INV && po.phi) {
+        const T norm = T(4 * po.ny * po.nz);
+#pragma unroll 2
+        for (int q = 0; q < R; q++) {
+            const int e = tid + NT * q;
+            const long r = blk * FL + e / N;   // packed row b ny + gm
+            phi_store(po, (int)(r / N), (int)(r % N), e % N, fdiv<T>(ls[slot(e)], norm));
+        }
+        return;
+    }
+#pragma unroll 4
+    for (int q = 0; q < R; q++) out[base + goff(tid + NT * q)] = ls[slot(tid + NT * q)];
 }
 
 // ------------------------------------------------------------ utilities
@@ -1821,6 +1916,10 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
                                               PhiOut<T>);                                                       \
     template __global__ void k_dct_plane<T, false>(const T*, T*, const T*, PhiOut<T>);                        \
     template __global__ void k_dct_plane<T, true>(const T*, T*, const T*, PhiOut<T>);                         \
+    template __global__ void k_dct_lines<T, false, 256, 1, fl_lines<T>()>(const T*, T*, const T*, PhiOut<T>); \
+    template __global__ void k_dct_lines<T, false, 256, 2, fl_lines<T>()>(const T*, T*, const T*, PhiOut<T>); \
+    template __global__ void k_dct_lines<T, true, 256, 1, fl_lines<T>()>(const T*, T*, const T*, PhiOut<T>);  \
+    template __global__ void k_dct_lines<T, true, 256, 2, fl_lines<T>()>(const T*, T*, const T*, PhiOut<T>);  \
     template __global__ void k_copy<T>(const T*, T*, long);                                                    \
     template __global__ void k_fill<T>(T*, long, T);
 
@@ -2824,23 +2923,38 @@ void Ops<T>::divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack
                                                        T(g), T(r))));
 }
 
-template <typename T>
-bool Ops<T>::dct_plane_ready()
+// opts kernel f in to sm bytes of dynamic LDS per block; false when the
+// device cannot give them
+static bool lds_optin(const void* f, int sm)
 {
-    const int sm = FD_N * FD_P * (int)sizeof(T);
     int dev = 0, lim = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || lim <= 0)
         if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
     if (sm > lim) return false;
-    if (hipFuncSetAttribute((const void*)k_dct_plane<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, sm) !=
-            hipSuccess ||
-        hipFuncSetAttribute((const void*)k_dct_plane<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, sm) !=
-            hipSuccess) {
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, sm) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
     return true;
+}
+
+template <typename T>
+bool Ops<T>::dct_fast_ready(int n)
+{
+    if (n == FD_N) {
+        const int sm = FD_N * FD_P * (int)sizeof(T);
+        return lds_optin((const void*)k_dct_plane<T, false>, sm) && lds_optin((const void*)k_dct_plane<T, true>, sm);
+    }
+    if (n == 256) {
+        constexpr int FL = fl_lines<T>();
+        const int sm = FL * (256 + 1) * (int)sizeof(T);
+        return lds_optin((const void*)k_dct_lines<T, false, 256, 1, FL>, sm) &&
+               lds_optin((const void*)k_dct_lines<T, false, 256, 2, FL>, sm) &&
+               lds_optin((const void*)k_dct_lines<T, true, 256, 1, FL>, sm) &&
+               lds_optin((const void*)k_dct_lines<T, true, 256, 2, FL>, sm);
+    }
+    return false;
 }
 
 template <typename T>
@@ -2855,6 +2969,22 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
             PLLC_L(c, "dct", (k_dct_plane<T, true><<<nx, 1024, sm, c.st>>>(in, out, c.tw, po)));
         else
             PLLC_L(c, "dct", (k_dct_plane<T, false><<<nx, 1024, sm, c.st>>>(in, out, c.tw, PhiOut<T>{})));
+        return;
+    }
+    if (c.tw && ny == 256 && nz == 256) {
+        // 256 x 256 planes: the fast transform in two launches of FL lines
+        // per block, rows (z) then columns (y); the inverse's rows write phi
+        constexpr int N = 256, FL = fl_lines<T>();
+        const size_t sm = (size_t)FL * (N + 1) * sizeof(T);
+        const unsigned nb = (unsigned)((long)nx * N / FL);
+        if (!inverse) {
+            PLLC_L(c, "dct", (k_dct_lines<T, false, N, 2, FL><<<nb, 8 * FL, sm, c.st>>>(in, out, c.tw, PhiOut<T>{})));
+            PLLC_L(c, "dct", (k_dct_lines<T, false, N, 1, FL><<<nb, 8 * FL, sm, c.st>>>(out, out, c.tw, PhiOut<T>{})));
+        } else {
+            const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+            PLLC_L(c, "dct", (k_dct_lines<T, true, N, 1, FL><<<nb, 8 * FL, sm, c.st>>>(in, out, c.tw, PhiOut<T>{})));
+            PLLC_L(c, "dct", (k_dct_lines<T, true, N, 2, FL><<<nb, 8 * FL, sm, c.st>>>(out, out, c.tw, po)));
+        }
         return;
     }
     const long R = (long)nx * ny, plane = (long)ny * nz;
@@ -2946,8 +3076,8 @@ void Ops<T>::cfl(Ctx<T>& c)
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     PLLC_L(c, "cfl", (k_cfl_partial<T><<<grd, blk, 0, c.st>>>(c.d, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.part)));
-    PLLC_L(c, "cfl", (k_cfl_final<T><<<1, 256, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
-                                                         c.ph.Timestep, c.scal)));
+    PLLC_L(c, "cfl", (k_cfl_final<T><<<1, 1024, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
+                                                          c.ph.Timestep, c.scal)));
 }
 
 template <typename T>

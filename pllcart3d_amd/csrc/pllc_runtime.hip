@@ -300,24 +300,24 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
     if (!put(c.Cfz, (size_t)d.nz * d.nz)) return false;
     pllc_dct_matrix(d.nz, true, h.data());
     if (!put(c.Ciz, (size_t)d.nz * d.nz)) return false;
-    // the fast plane transform for 128 x 128 planes (PLLC_DCT=gemm keeps the
-    // dense contractions), when the device gives a block the plane's LDS
+    // the fast transform for 128 x 128 and 256 x 256 planes (PLLC_DCT=gemm
+    // keeps the dense contractions), when the device gives its blocks the LDS
     const char* dm = getenv("PLLC_DCT");
-    if (d.ny == 128 && d.nz == 128 && !(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_plane_ready()) {
-        std::vector<double> w(514);
+    const int n = d.ny;
+    if (d.nz == n && (n == 128 || n == 256) && !(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_fast_ready(n)) {
+        // (cos, sin) pairs: e^{-2 pi i k/M} (k < M = n/2), e^{-2 pi i k/n}
+        // (k <= n/2), e^{-i pi k/(2n)} (k < n)
+        const int m = n / 2, len = 2 * n + 1;
+        std::vector<double> w(2 * (size_t)len);
         const double pi = 3.14159265358979323846;
-        for (int k = 0; k < 64; k++) { w[2 * k] = cos(2 * pi * k / 64); w[2 * k + 1] = -sin(2 * pi * k / 64); }
-        for (int k = 0; k <= 64; k++) {
-            w[2 * (64 + k)] = cos(2 * pi * k / 128);
-            w[2 * (64 + k) + 1] = -sin(2 * pi * k / 128);
-        }
-        for (int k = 0; k < 128; k++) {
-            w[2 * (129 + k)] = cos(pi * k / 256);
-            w[2 * (129 + k) + 1] = -sin(pi * k / 256);
-        }
-        if (!get((void**)&c.tw, 514 * ts)) return false;
-        for (size_t e = 0; e < 514; e++) ht[e] = T(w[e]);
-        if (!pllc_hip_ok(where, hipMemcpy(c.tw, ht.data(), 514 * ts, hipMemcpyHostToDevice))) return false;
+        auto set = [&](int at, double ang) { w[2 * at] = cos(ang); w[2 * at + 1] = -sin(ang); };
+        for (int k = 0; k < m; k++) set(k, 2 * pi * k / m);
+        for (int k = 0; k <= n / 2; k++) set(m + k, 2 * pi * k / n);
+        for (int k = 0; k < n; k++) set(n + 1 + k, pi * k / (2 * n));
+        if (!get((void**)&c.tw, 2 * (size_t)len * ts)) return false;
+        std::vector<T> wt(w.size());
+        for (size_t e = 0; e < w.size(); e++) wt[e] = T(w[e]);
+        if (!pllc_hip_ok(where, hipMemcpy(c.tw, wt.data(), wt.size() * ts, hipMemcpyHostToDevice))) return false;
     }
     return true;
 }

@@ -43,20 +43,22 @@ def test_mixed_tracks_fp64_oracle(oracle_mod, shape):
         assert errs[f] <= tol, (f, errs[f])
 
 
-def test_mixed_128_planes_track_fp64_oracle(oracle_mod):
-    """ny = nz = 128 (the fast plane transform in FP32): c, the velocity
-    vector and phi after 10 steps against the FP64 oracle, to the bars above
-    (the state's w is zero, so its own relative error is not held)."""
-    prm = P.make_params(8, 128, 128, Vratio=2.0, Pe=500.0, Inf_thick=0.05)
+@pytest.mark.parametrize("n", [128, 256])
+def test_mixed_square_planes_track_fp64_oracle(oracle_mod, n):
+    """ny = nz = n (the fast transform in FP32: one plane per block at 128,
+    lines at 256): c, the velocity vector and phi after 10 steps against the
+    FP64 oracle, to the bars above (the state's w is zero, so its own
+    relative error is not held)."""
+    prm = P.make_params(8, n, n, Vratio=2.0, Pe=500.0, Inf_thick=0.05)
     c0 = P.ic_fill(prm, P.IC_NOISE, 1, 0.01)
     orc = oracle_mod.Oracle(prm)
     orc.init(c0)
     sim = P.Sim(prm, "mixed")
     sim.init(c0)
-    for n in range(10):
+    for k in range(10):
         orc.run(1)
         sim.run(1)
     errs = {f: _err(sim, orc, f) for f in ("C_OLD", "U_OLD+V_OLD+W_OLD", "PHI")}
-    print("[mixed 8x128x128] after 10 steps: %s" % " ".join("%s=%.2e" % kv for kv in errs.items()), flush=True)
+    print("[mixed 8x%dx%d] after 10 steps: %s" % (n, n, " ".join("%s=%.2e" % kv for kv in errs.items())), flush=True)
     for f, e in errs.items():
         assert e <= TOL[f], (f, e)

@@ -121,24 +121,27 @@ def test_fused_steps_track_oracle(oracle_mod, solver, monkeypatch):
         _compare_step(orc, sim, "%s step %d" % (solver, n + 1))
 
 
-@pytest.mark.parametrize("dct", ["plane", "gemm"])
-def test_128_planes_track_oracle(oracle_mod, dct, monkeypatch):
-    """ny = nz = 128: the Poisson transforms run as the fast plane transform
-    (REDFT10/01 through 64-point complex FFTs, one block per x-plane) or, with
+@pytest.mark.parametrize("n", [128, 256])
+@pytest.mark.parametrize("dct", ["fast", "gemm"])
+def test_square_planes_track_oracle(oracle_mod, dct, n, monkeypatch):
+    """ny = nz = n, n = 128 or 256: the Poisson transforms run as the fast
+    transform (REDFT10/01 through n/2-point complex FFTs; n = 128 one block
+    per x-plane, n = 256 two launches of 32 lines per block) or, with
     PLLC_DCT=gemm, as the dense contractions; whole steps vs the oracle.
     c, the velocity and phi are held to 1e-11.  The pressure gradient adds
     fac A(mu) grad(phi), second differences of grad(phi) scaled by
-    1/(Re dy^2) = 1/(Re dz^2) ~ 5e4 here, which multiplies phi's rounding
-    (a transform that rounds differently from the oracle's dense sums, as
-    FFTW's own does): measured 3e-11 (dense) and 9e-11 (plane) at the first
-    step, so pg is held to 1e-9 at this resolution."""
+    1/(Re dy^2) = 1/(Re dz^2) (~5e4 at n = 128, ~2e5 at n = 256), which
+    multiplies phi's rounding (a transform that rounds differently from the
+    oracle's dense sums, as FFTW's own does): measured 3e-11 (dense) and
+    9e-11 (fast) at the first step at n = 128, so pg is held to 1e-9 at
+    these resolutions."""
     if dct == "gemm":
         monkeypatch.setenv("PLLC_DCT", "gemm")
-    prm, orc, sim = _pair(oracle_mod, nx=8, ny=128, nz=128)
-    for n in range(3):
+    prm, orc, sim = _pair(oracle_mod, nx=8, ny=n, nz=n)
+    for k in range(3):
         orc.run(1)
         sim.run(1)
-        _compare_step(orc, sim, "%s 8x128x128 step %d" % (dct, n + 1), pg_tol=1e-9, components=False)
+        _compare_step(orc, sim, "%s 8x%dx%d step %d" % (dct, n, n, k + 1), pg_tol=1e-9, components=False)
 
 
 def test_nmvp_steps_track_oracle(oracle_mod):
@@ -272,14 +275,15 @@ OPS_SEQ = ["CONCENTRATION", "VISCOSITY", "XMOM", "YMOM", "ZMOM", "VELO_BOUNDARY"
 
 
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
-@pytest.mark.parametrize("shape", [(24, 12, 10), (8, 128, 128)])
+@pytest.mark.parametrize("shape", [(24, 12, 10), (8, 128, 128), (8, 256, 256)])
 def test_fused_step_equals_operator_sequence(precision, shape):
     """The fused step (pointer swaps, batched u/v/w sweeps, the ghost copies
     written by the x-sweeps, the correction kernel and the last inverse
     transform, velo_correction and pressure_gradient in one kernel) against
     the reference's call sequence one operator at a time: equal in every
     bit of every field incl. ghosts, in both precisions, with the dense
-    (24x12x10) and the plane (8x128x128) transforms."""
+    (24x12x10) and the fast (8x128x128 one plane per block, 8x256x256 lines)
+    transforms."""
     prm = P.make_params(*shape, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
     c0 = P.ic_fill(prm, P.IC_NOISE, 3, 0.05)
     a = P.Sim(prm, precision)
