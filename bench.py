@@ -6,7 +6,7 @@ resident in HBM before the timed region; the timed region is K whole steps
 bracketed by stream synchronisation, timed with HIP events on the solver's
 stream.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config front_256x128x128] [--no-cpu]
 """
 import argparse
 import json
@@ -35,6 +35,10 @@ TOUCHES = {
 SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
 
 
+# initial-condition recipes, as the workload names them
+IC_NAMES = {0: "erf front", 1: "noise front", 2: "16x16-mode front"}
+
+
 def dct_is_fast(prm):
     """ny = nz = 128 or 256 run the fast transform (unless PLLC_DCT=gemm):
     every launch reads and writes the packed block once; other plane sizes
@@ -55,7 +59,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--config", default="front_256x128x128")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel timing")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -181,8 +185,8 @@ def large_line(a):
     import numpy as np
     import pllcart3d_amd as P
     from pllcart3d_amd.configs import CONFIGS, config_params
-    prm = config_params("cfg3")
-    cfg = CONFIGS["cfg3"]
+    prm = config_params("front_512x256x256")
+    cfg = CONFIGS["front_512x256x256"]
     c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
     cells = prm.nx * prm.ny * prm.nz
     steps = max(2, min(a.steps, 6))
@@ -199,7 +203,7 @@ def large_line(a):
         err[f] = float(np.linalg.norm(sims["mixed"].download(f) - ref) / np.linalg.norm(ref))
     for s in sims.values():
         s.close()
-    out["workload"] = "cfg3: 512x256x256 box 4x1x1, R=3 Pe=4000, 16x16-mode front"
+    out["workload"] = "512x256x256 box 4x1x1, R=3 Pe=4000, 16x16-mode front"
     out["steps"] = steps
     out["speedup_mixed_vs_plain"] = out["plain"]["ms_per_step"] / out["mixed"]["ms_per_step"]
     out["mixed_rel_l2_vs_fp64_after_%d_steps" % (steps + 2)] = err
@@ -265,8 +269,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "%s: %dx%dx%d box 4x1x1, R=%g Pe=%g, RK3-CN step (plain FP64; mixed in 'mixed')" %
-                   (a.config, prm.nx, prm.ny, prm.nz, prm.Vratio, prm.Pe), "cells": cells},
+        "config": {"workload": "%dx%dx%d box 4x1x1, R=%g Pe=%g, %s, RK3-CN step (plain FP64; mixed in 'mixed')" %
+                   (prm.nx, prm.ny, prm.nz, prm.Vratio, prm.Pe, IC_NAMES[cfg["ic"]]), "cells": cells},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_hbm_fraction": round(step_ab / (ms / a.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -10,13 +10,13 @@ from ._lib import IC_ERF, IC_MULTIMODE, IC_NOISE, Params
 # name: (nx, ny, nz, Vratio R, Pe, front width, IC recipe, seed, amplitude, steps)
 CONFIGS = {
     # plain FP64 reference case, 64x32x32, R=2, Pe=500, w=0.05, +-0.01 noise (seed 1), 200 steps
-    "cfg1": dict(nx=64, ny=32, nz=32, Vratio=2.0, Pe=500.0, Inf_thick=0.05,
+    "noise_64x32x32": dict(nx=64, ny=32, nz=32, Vratio=2.0, Pe=500.0, Inf_thick=0.05,
                  ic=IC_NOISE, seed=1, amp=0.01, steps=200),
     # 256x128x128 (4.2M cells), R=3, Pe=2000, w=0.02, 16x16-mode front (seed 2), 1000 steps
-    "cfg2": dict(nx=256, ny=128, nz=128, Vratio=3.0, Pe=2000.0, Inf_thick=0.02,
+    "front_256x128x128": dict(nx=256, ny=128, nz=128, Vratio=3.0, Pe=2000.0, Inf_thick=0.02,
                  ic=IC_MULTIMODE, seed=2, amp=0.01, steps=1000),
     # 512x256x256 (33.5M cells), Pe=4000, same front recipe, 500 steps
-    "cfg3": dict(nx=512, ny=256, nz=256, Vratio=3.0, Pe=4000.0, Inf_thick=0.02,
+    "front_512x256x256": dict(nx=512, ny=256, nz=256, Vratio=3.0, Pe=4000.0, Inf_thick=0.02,
                  ic=IC_MULTIMODE, seed=2, amp=0.01, steps=500),
 }
 

@@ -1,6 +1,6 @@
 """Profiling workload and rocprofv3 summaries for the pllcart3d time step.
 
-    python pllcart3d_amd/profiling.py run [--config cfg2] [--steps 2] [--precision plain|mixed]
+    python pllcart3d_amd/profiling.py run [--config front_256x128x128] [--steps 2] [--precision plain|mixed]
         set-up, one UPDATE_OLD (12 whole-field copies of known size: the
         calibration of the HBM counters) and `steps` fused steps; run it
         under rocprofv3 --kernel-trace --stats or --pmc FETCH_SIZE / WRITE_SIZE.
@@ -89,7 +89,7 @@ def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
-    r.add_argument("--config", default="cfg2")
+    r.add_argument("--config", default="front_256x128x128")
     r.add_argument("--steps", type=int, default=2)
     r.add_argument("--precision", default="plain", choices=["plain", "mixed"])
     s = sub.add_parser("summary")
