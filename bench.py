@@ -69,8 +69,25 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(prm, c0, cells, threads):
-    """The FP64 oracle on the host cores: a bounded sample of the same workload."""
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def _oracle_rate(prm, c0, cells, threads, max_steps, max_s):
     from oracle import oracle
     oracle.set_threads(threads)
     orc = oracle.Oracle(prm)
@@ -81,11 +98,23 @@ def cpu_baseline(prm, c0, cells, threads):
         orc.run(1)
         n += 1
         el = time.perf_counter() - t0
-        if el > 10.0 or n >= 5:
+        if el > max_s or n >= max_steps:
             break
-    return {"value": cells * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+    return cells * n / el, n, el
+
+
+def cpu_baseline(prm, c0, cells, threads):
+    """The FP64 oracle on the host cores: a bounded sample of the same
+    workload on `threads` OpenMP threads (the reported value), and one step
+    on a single thread."""
+    v, n, el = _oracle_rate(prm, c0, cells, threads, 5, 10.0)
+    v1, n1, el1 = _oracle_rate(prm, c0, cells, 1, 1, 0.0)
+    return {"value": v, "unit": "cell-steps/s", "cores": threads, "kind": "port",
             "sample": "%d step(s) of the %dx%dx%d workload from the same initial state, %.1f s" %
-                      (n, prm.nx, prm.ny, prm.nz, el)}
+                      (n, prm.nx, prm.ny, prm.nz, el),
+            "one_thread": {"value": v1, "unit": "cell-steps/s", "cores": 1,
+                           "sample": "%d step(s), %.1f s" % (n1, el1)},
+            "host": host_cpu()}
 
 
 def kernel_table(sim, ksteps):
