@@ -653,14 +653,24 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
 // calculate_viscosity (843-886) / calculate_viscosity_nmvp (889-948): cell
 // centres on a one-cell ring, and edges from the 4-point mean of c (mean
 // first, then the law).
+//
+// The (j, k) plane of the index space (j 0 .. ny+1, k 0 .. nz+1) is
+// contiguous, so a plane is cut into chunks of 256 consecutive points, one
+// per block; gridDim.x (chunks per plane) is a multiple of 8 so that the
+// XCD a block runs on (its linear id mod 8) depends only on blockIdx.x, and
+// XCD x takes chunks x nb/8 .. (x+1) nb/8 - 1 of every plane: the previous
+// plane's c (o - sx) was read into the same XCD's L2.
 template <typename T>
 __global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restrict__ cn, T* __restrict__ cell,
                                               T* __restrict__ xy, T* __restrict__ xz, T* __restrict__ yz)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;       // 0 .. nz+1
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;       // 0 .. ny+1
-    const int i = blockIdx.z + 2;                              // 2 .. nx+3
-    if (k > d.nz + 1 || j > d.ny + 1) return;
+    const int nb8 = gridDim.x >> 3;
+    const int chunk = (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+    const int l = chunk * 256 + threadIdx.x;
+    const int pk = d.nz + 2;
+    if (l >= (d.ny + 2) * pk) return;
+    const int j = l / pk, k = l - j * pk;                      // 0 .. ny+1, 0 .. nz+1
+    const int i = blockIdx.y + 2;                              // 2 .. nx+3
     const long o = IX(d, i, j, k), sx = d.sx, sy = d.sy;
     // the seven values first (all inside the array: i >= 2 leaves a ghost
     // plane below o - sx - sy - 1), then the laws
@@ -2847,8 +2857,10 @@ void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un,
 template <typename T>
 void Ops<T>::viscosity(Ctx<T>& c, const T* cn, T* const mu[4])
 {
-    dim3 blk(64, 4, 1), grd((c.d.nz + 2 + 63) / 64, (c.d.ny + 2 + 3) / 4, c.d.nx + 2);
-    PLLC_L(c, "visc", (k_visc<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2], mu[3])));
+    // chunks of 256 points per (j, k) plane, rounded up to a multiple of 8
+    const int nb = (int)(((long)(c.d.ny + 2) * (c.d.nz + 2) + 256 * 8 - 1) / (256 * 8)) * 8;
+    PLLC_L(c, "visc", (k_visc<T><<<dim3(nb, c.d.nx + 2, 1), 256, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2],
+                                                                           mu[3])));
 }
 
 template <typename T, int F>
