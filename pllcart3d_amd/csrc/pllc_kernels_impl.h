@@ -1103,8 +1103,9 @@ __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict
                                                      const T* __restrict__ w, double* __restrict__ part)
 {
     __shared__ double red[256];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 bt = xcd_tile();
+    const int k = bt.x * blockDim.x + threadIdx.x + 1;
+    const int j = bt.y * blockDim.y + threadIdx.y + 1;
     const int i = blockIdx.z + 3;
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     double m = 0.0;
@@ -2677,10 +2678,14 @@ static int resident_blocks(K f, int nthreads, size_t sm)
 template <typename T>
 static inline bool fast_stencils(const Ctx<T>& c) { return c.line_solver == 0; }
 
+// 256-thread tiles of one i-plane: 128 x 2 (k, j) when a k row has 96 or
+// more points (measured faster than 64 x 4, 32 x 8 and 256 x 1 for the
+// cell kernels at 128-point rows, mostly in FP32), 64 x 4 for shorter rows
 static inline dim3 cell_grid(const Rng& R, dim3& blk)
 {
-    blk = dim3(64, 4, 1);
-    return dim3((R.ke - R.ks + 1 + 63) / 64, (R.je - R.js + 1 + 3) / 4, R.ie - R.is + 1);
+    const int nk = R.ke - R.ks + 1, bx = nk >= 96 ? 128 : 64, by = 256 / bx;
+    blk = dim3(bx, by, 1);
+    return dim3((nk + bx - 1) / bx, (R.je - R.js + 1 + by - 1) / by, R.ie - R.is + 1);
 }
 static inline dim3 line_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 static inline int bulk_grid(long n) { long b = (n + 255) / 256; return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b)); }

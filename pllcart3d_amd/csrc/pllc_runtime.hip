@@ -270,7 +270,9 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
         const char* ls = getenv("PLLC_LINE_SOLVER");
         c.line_solver = (ls && strcmp(ls, "thomas") == 0) ? 1 : 0;
     }
-    c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 3) / 4) * d.nx + 64;
+    // one partial per block of the CFL pass: its cell tiles are 64 x 4 or
+    // 128 x 2 (k, j), so ceil(nz/64) ceil(ny/2) nx bounds both
+    c.npart = (long)((d.nz + 63) / 64) * ((d.ny + 1) / 2) * d.nx + 64;
     c.npart = std::max(c.npart, (long)(2 * 1056 + 2 * (((long)d.ny * d.nz + 255) / 256) + 64));
     if (!get((void**)&c.part, (size_t)c.npart * 8)) return false;
     if (!get((void**)&c.lam_y, d.ny * ts) || !get((void**)&c.lam_z, d.nz * ts) || !get((void**)&c.lam00, 2 * 8))
