@@ -86,6 +86,9 @@ struct Ops {
     static void pressure_gradient(Ctx<T>& c, double g, double r, int mask);
     static void cfl(Ctx<T>& c);
     static void bulk(Ctx<T>& c, int slot, double alpha, bool advance_t);
+    // bulk() and stale() in one launch where the plane allows it
+    static void bulk_stale(Ctx<T>& c, int slot, double alpha, bool advance_t, const T* qc, const T* qu, const T* qv,
+                           const T* qw);
     static void copy(Ctx<T>& c, const T* a, T* b, long n);
     // velocity correction, outflow deficit and the pressure-gradient update
     // of all three components (mu_new) in one pass over the cells

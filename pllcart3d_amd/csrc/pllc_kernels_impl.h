@@ -27,11 +27,9 @@ namespace pllc {
 // je+1, [+2] z-pass at ks-1, [+3] z-pass at ke+1.  The x pass runs first on
 // a zeroed scratch, so its outside reads are 0.
 template <typename T>
-__global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ q1, const T* __restrict__ q2,
-                        const T* __restrict__ q3, T* __restrict__ stale)
+__device__ __forceinline__ T stale_value(const Dims& d, const T* __restrict__ q0, const T* __restrict__ q1,
+                                         const T* __restrict__ q2, const T* __restrict__ q3, int t)
 {
-    const int t = threadIdx.x;
-    if (t >= 16) return;
     const int f = t >> 2, w = t & 3;
     const T* q = f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : q3;
     const Rng R = field_rng(d, f);
@@ -52,13 +50,21 @@ __global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ 
         else if (m >= R.is && m <= R.ie)
             v = divh<T>(dd1(px, d.sx, m, hx, d.rdx) - dd1(px, d.sx, m - 1, hx, d.rdx), T(2.0) * hx, 0.5 * d.rdx);
     }
-    stale[t] = v;
+    return v;
+}
+
+template <typename T>
+__global__ void k_stale(Dims d, const T* __restrict__ q0, const T* __restrict__ q1, const T* __restrict__ q2,
+                        const T* __restrict__ q3, T* __restrict__ stale)
+{
+    const int t = threadIdx.x;
+    if (t < 16) stale[t] = stale_value<T>(d, q0, q1, q2, q3, t);
 }
 
 // ------------------------------------------------- explicit terms + source
 
-// XCD-aware tile order of the cell grids (blocks of 64 k x 4 j, grid (k
-// tiles, j tiles, planes)).  Workgroups go to the 8 XCDs round-robin by
+// XCD-aware tile order of the cell grids (blocks of 128 k x 2 j or 64 k x
+// 4 j, grid (k tiles, j tiles, planes)).  Workgroups go to the 8 XCDs round-robin by
 // linear id and each XCD has its own L2.  In dispatch order a plane's
 // consecutive j tiles land on different XCDs, so every XCD fetches the
 // y-stencil halo rows of its tiles itself.  This hands XCD x the x-th eighth
@@ -1001,11 +1007,17 @@ __global__ __launch_bounds__(64) void k_flux_rows(Dims d, const T* __restrict__ 
     }
 }
 
+// Pass 2 and the deficit's addition to the outlet face (2216-2223) in one
+// launch: block j - 1 (of ny) sums the row integrals over j in the
+// reference's order, forms the deficit and adds it to row j of the outlet
+// face; every block forms the same value, and block 0 also stores qin,
+// qout and the deficit.
 template <typename T>
-__global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __restrict__ u, double* __restrict__ scal,
-                                                      const double* __restrict__ part, int rk)
+__global__ __launch_bounds__(128) void k_flux_apply(Dims d, Phys ph, T* __restrict__ u, double* __restrict__ scal,
+                                                    const double* __restrict__ part, int rk)
 {
     __shared__ double qk_in[1056], qk_out[1056];
+    __shared__ double def;
     const double hy = d.dy;
     for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) { qk_in[j] = part[j]; qk_out[j] = part[1056 + j]; }
     __syncthreads();
@@ -1016,21 +1028,20 @@ __global__ __launch_bounds__(256) void k_flux_deficit(Dims d, Phys ph, T* __rest
             qout += 0.5 * hy * (qk_out[j - 1] + qk_out[j]);
         }
         const double deficit = (qin - qout) / ((ph.Y2 - ph.Y1) * (ph.Z2 - ph.Z1));
-        scal[S_RK + 4 * rk + 1] = qin;
-        scal[S_RK + 4 * rk + 2] = qout;
-        scal[S_RK + 4 * rk + 3] = deficit;
+        def = deficit;
+        if (blockIdx.x == 0) {
+            scal[S_RK + 4 * rk + 1] = qin;
+            scal[S_RK + 4 * rk + 2] = qout;
+            scal[S_RK + 4 * rk + 3] = deficit;
+        }
     }
-}
-
-// the deficit added to the outlet face (2219-2223), one point per thread
-template <typename T>
-__global__ void k_add_deficit(Dims d, T* __restrict__ u, const double* __restrict__ scal, int rk)
-{
-    const int k = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y;
-    if (k > d.nz) return;
-    const long o = IX(d, d.nx + 3, j, k);
-    u[o] = u[o] + T(scal[S_RK + 4 * rk + 3]);
+    __syncthreads();
+    const int j = 1 + blockIdx.x;
+    const T dv = T(def);
+    for (int k = 1 + threadIdx.x; k <= d.nz; k += blockDim.x) {
+        const long o = IX(d, d.nx + 3, j, k);
+        u[o] = u[o] + dv;
+    }
 }
 
 // velo_correction (2144-2169) and pressure_gradient (2964-3044) of one
@@ -1213,6 +1224,57 @@ __global__ __launch_bounds__(256) void k_bulk_final(const double* __restrict__ p
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s && bulk_better(sv[t + s], si[t + s], sv[t], si[t])) { sv[t] = sv[t + s]; si[t] = si[t + s]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        scal[S_RK + 4 * rk] = si[0] < 0 ? 0.0 : sv[0];
+        if (tval) tval[0] = tval[0] + alpha * scal[S_DT];
+    }
+}
+
+// bulk_velo in one block over the whole plane (ny nz <= BULK1_MAX): each
+// thread scans its points in increasing index, then a tree over the block;
+// the winner (largest |u|, first index among ties) is the same in any order.
+// With q0 set, block 1 computes the 16 stale values (k_stale) meanwhile.
+constexpr int BULK1_MAX = 65536;
+template <typename T>
+__global__ __launch_bounds__(1024) void k_bulk_stale(Dims d, const T* __restrict__ u, double* __restrict__ scal,
+                                                     int rk, double alpha, double* __restrict__ tval,
+                                                     const T* __restrict__ q0, const T* __restrict__ q1,
+                                                     const T* __restrict__ q2, const T* __restrict__ q3,
+                                                     T* __restrict__ stale)
+{
+    constexpr int NT = 1024, U = 8;
+    const int t = threadIdx.x;
+    if (blockIdx.x == 1) {
+        if (t < 16) stale[t] = stale_value<T>(d, q0, q1, q2, q3, t);
+        return;
+    }
+    __shared__ double sv[NT];
+    __shared__ long si[NT];
+    const int np = d.ny * d.nz;
+    const T* __restrict__ pl = u + IX(d, d.nx + 2, 1, 1);
+    double v = 0.0;
+    long id = -1;
+    auto take = [&](double x, long l) {
+        if (fabs(x) > 0.0 && bulk_better(x, l, v, id)) { v = x; id = l; }
+    };
+    int l = t;
+    for (; l + (U - 1) * NT < np; l += U * NT) {
+        T x[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int m = l + q * NT;
+            x[q] = pl[(long)(m / d.nz) * d.sy + m % d.nz];
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) take((double)x[q], l + q * NT);
+    }
+    for (; l < np; l += NT) take((double)pl[(long)(l / d.nz) * d.sy + l % d.nz], l);
+    sv[t] = v; si[t] = id;
+    __syncthreads();
+    for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
+        if (t < s2 && bulk_better(sv[t + s2], si[t + s2], sv[t], si[t])) { sv[t] = sv[t + s2]; si[t] = si[t + s2]; }
         __syncthreads();
     }
     if (t == 0) {
@@ -1917,7 +1979,7 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*, const double*);                     \
     template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
     template __global__ void k_flux_rows<T>(Dims, const T*, double*);                                          \
-    template __global__ void k_flux_deficit<T>(Dims, Phys, T*, double*, const double*, int);                   \
+    template __global__ void k_flux_apply<T>(Dims, Phys, T*, double*, const double*, int);                     \
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
     template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
@@ -3059,9 +3121,7 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
     PLLC_L(c, "velo_corr", (k_velo_corr<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[PHI], c.f[U_NEW], c.f[V_NEW],
                                                                    c.f[W_NEW], c.scal, T(g), T(r))));
     PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
-    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
-    PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,
-                                                                                          slot)));
+    PLLC_L(c, "flux", (k_flux_apply<T><<<c.d.ny, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
 }
 
 template <typename T>
@@ -3073,9 +3133,7 @@ void Ops<T>::correction_and_gradient(Ctx<T>& c, double g, double r, int slot, bo
                                                                c.f[W_NEW], c.f[U_PG], c.f[V_PG], c.f[W_PG], c.scal,
                                                                T(g), T(r), ghosts ? 1 : 0)));
     PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
-    PLLC_L(c, "flux", (k_flux_deficit<T><<<1, 256, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
-    PLLC_L(c, "flux", (k_add_deficit<T><<<dim3((c.d.nz + 63) / 64, c.d.ny), 64, 0, c.st>>>(c.d, c.f[U_NEW], c.scal,
-                                                                                          slot)));
+    PLLC_L(c, "flux", (k_flux_apply<T><<<c.d.ny, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
 }
 
 template <typename T>
@@ -3100,10 +3158,30 @@ void Ops<T>::cfl(Ctx<T>& c)
 template <typename T>
 void Ops<T>::bulk(Ctx<T>& c, int slot, double alpha, bool advance_t)
 {
-    const int nb = (int)(((long)c.d.ny * c.d.nz + 255) / 256);
+    const long np = (long)c.d.ny * c.d.nz;
+    if (np <= BULK1_MAX) {
+        PLLC_L(c, "bulk", (k_bulk_stale<T><<<1, 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
+                                                               advance_t ? c.tval : nullptr, nullptr, nullptr,
+                                                               nullptr, nullptr, nullptr)));
+        return;
+    }
+    const int nb = (int)((np + 255) / 256);
     PLLC_L(c, "bulk", (k_bulk_partial<T><<<nb, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.part)));
     PLLC_L(c, "bulk", (k_bulk_final<T><<<1, 256, 0, c.st>>>(c.part, nb, c.scal, slot, alpha,
                                                           advance_t ? c.tval : nullptr)));
+}
+
+template <typename T>
+void Ops<T>::bulk_stale(Ctx<T>& c, int slot, double alpha, bool advance_t, const T* qc, const T* qu, const T* qv,
+                        const T* qw)
+{
+    if ((long)c.d.ny * c.d.nz > BULK1_MAX) {
+        bulk(c, slot, alpha, advance_t);
+        stale(c, qc, qu, qv, qw);
+        return;
+    }
+    PLLC_L(c, "bulk", (k_bulk_stale<T><<<2, 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
+                                                           advance_t ? c.tval : nullptr, qc, qu, qv, qw, c.stale)));
 }
 
 template <typename T>

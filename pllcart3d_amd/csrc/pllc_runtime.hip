@@ -697,8 +697,7 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
     Ops<T>::cfl(c);
     for (int rk = 0; rk < 3; rk++) {
         const double g = kRK1[rk], r = kRK2[rk];
-        Ops<T>::bulk(c, rk, g + r, true);
-        Ops<T>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        Ops<T>::bulk_stale(c, rk, g + r, true, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
         Ops<T>::viscosity(c, c.f[C_NEW], mun);
