@@ -1062,12 +1062,16 @@ __global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, c
     const long o = IX(d, i, j, k);
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
     const T facv = T(scal[S_DT]) * (g + r) / T(ph.Re);
-    // the three components' operands are loaded before any update
+    // the three components' operands are loaded before any update, for
+    // every point (the loads the three share then issue once): a point
+    // whose component is no unknown reads at most two rows, planes or
+    // points below its own, inside the ghost ring or the previous plane,
+    // and stores nothing
     const bool dv = j >= 2, dw = k >= 2, du = i >= 4;
     CorrOps<T> cv, cw, cu;
-    if (dv) corr_load<T, FV>(d, mu, phi, v, pgv, o, cv);
-    if (dw) corr_load<T, FW>(d, mu, phi, w, pgw, o, cw);
-    if (du) corr_load<T, FU>(d, mu, phi, u, pgu, o, cu);
+    corr_load<T, FV>(d, mu, phi, v, pgv, o, cv);
+    corr_load<T, FW>(d, mu, phi, w, pgw, o, cw);
+    corr_load<T, FU>(d, mu, phi, u, pgu, o, cu);
     // with vghost, update_veloghost_points for the values just written
     if (dv) {
         const T val = corr_apply<T, FV>(d, ph, cv, v, pgv, o, fac, facv);

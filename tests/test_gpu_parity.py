@@ -144,6 +144,19 @@ def test_square_planes_track_oracle(oracle_mod, dct, n, monkeypatch):
         _compare_step(orc, sim, "%s 8x%dx%d step %d" % (dct, n, n, k + 1), pg_tol=1e-9, components=False)
 
 
+def test_wide_planes_track_oracle(oracle_mod):
+    """ny nz = 67600 > 65536: the bulk velocity takes its two-pass form
+    (per-block candidates, then one block), and the transforms the dense
+    contractions at a plane size that is no multiple of their tiles; whole
+    steps vs the oracle (pg held to 1e-9 at this resolution, as for the
+    fine square planes)."""
+    prm, orc, sim = _pair(oracle_mod, nx=4, ny=260, nz=260)
+    for k in range(2):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "4x260x260 step %d" % (k + 1), pg_tol=1e-9, components=False)
+
+
 def test_nmvp_steps_track_oracle(oracle_mod):
     """Non-monotonic viscosity law (Nmvp = 1): the slope parameter found at
     set-up and whole fused steps."""
