@@ -10,6 +10,7 @@
 #pragma once
 #include <algorithm>
 #include <map>
+#include <type_traits>
 #include <utility>
 
 #include "pllc_dev.h"
@@ -2640,6 +2641,13 @@ __device__ __forceinline__ void pchunk_put(U* __restrict__ red, int E, int LPB, 
 }
 
 template <typename T>
+__device__ __forceinline__ PChunk<double>& pick_chunk(PChunk<T>& a, PChunk<double>& b)
+{
+    if constexpr (std::is_same<T, double>::value) return a;
+    else return b;
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_pois_part(Dims d, const T* __restrict__ lam_y, const T* __restrict__ lam_z,
                                                    T* __restrict__ P, const double* __restrict__ lam00)
 {
@@ -2656,8 +2664,11 @@ __global__ __launch_bounds__(256) void k_pois_part(Dims d, const T* __restrict__
     const int x0 = (s * n) / PC;
     const int m = ((s + 1) * n) / PC - x0;
     T* __restrict__ p = P + (live ? l : 0);
+    // the (0,0) line's FP64 chunk: in FP64 the same object as every other
+    // line's (one set of registers for the two branches)
     PChunk<T> ch;
-    PChunk<double> ch0;
+    PChunk<double> chd;
+    PChunk<double>& ch0 = pick_chunk(ch, chd);
     if (live) {
         if (mean) {
             const double dx2i = 1.0 / (d.dx * d.dx);
