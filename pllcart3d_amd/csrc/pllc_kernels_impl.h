@@ -10,6 +10,7 @@
 #pragma once
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -2729,12 +2730,15 @@ const char* kernel_class_names =
     } while (0)
 
 // blocks of kernel f (nthreads threads, sm bytes of LDS) resident on the
-// device at once: the occupancy per CU times the CUs (cached per shape)
+// device at once: the occupancy per CU times the CUs (cached per shape,
+// behind a lock: handles may launch from several host threads)
 template <typename K>
 static int resident_blocks(K f, int nthreads, size_t sm)
 {
     static std::map<std::pair<int, size_t>, int> cache;
     static int cus = 0;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
     const auto key = std::make_pair(nthreads, sm);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cmath>
 #include <cstdio>
@@ -829,10 +830,17 @@ struct TuneKey {
         return std::tie(prec, nx, ny, nz) < std::tie(o.prec, o.nx, o.ny, o.nz);
     }
 };
+// the shapes pllc_sim_autotune found, shared by every handle (and host
+// thread) of the process
 std::map<TuneKey, std::array<int, 3>>& tune_cache()
 {
     static std::map<TuneKey, std::array<int, 3>> m;
     return m;
+}
+std::mutex& tune_mutex()
+{
+    static std::mutex mu;
+    return mu;
 }
 }  // namespace
 
@@ -844,8 +852,14 @@ static void set_lpb(pllc_sim* s, const int* lpb)
 
 static void apply_tuned(pllc_sim* s)
 {
-    auto it = tune_cache().find(TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz});
-    if (it != tune_cache().end()) set_lpb(s, it->second.data());
+    std::array<int, 3> lpb;
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        auto it = tune_cache().find(TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz});
+        if (it == tune_cache().end()) return;
+        lpb = it->second;
+    }
+    set_lpb(s, lpb.data());
 }
 
 extern "C" int pllc_sim_autotune(pllc_sim* s, int nsteps, int* lpb_out)
@@ -867,7 +881,10 @@ extern "C" int pllc_sim_autotune(pllc_sim* s, int nsteps, int* lpb_out)
         best[d] = lbest;
     }
     set_lpb(s, best);
-    tune_cache()[TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz}] = {best[0], best[1], best[2]};
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_cache()[TuneKey{s->prec, s->p.nx, s->p.ny, s->p.nz}] = {best[0], best[1], best[2]};
+    }
     if (lpb_out)
         for (int d = 0; d < 3; d++) lpb_out[d] = best[d];
     return 0;
