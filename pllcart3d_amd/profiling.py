@@ -31,8 +31,18 @@ def run(args):
     from pllcart3d_amd.configs import CONFIGS, config_params
     prm = config_params(args.config)
     cfg = CONFIGS[args.config]
+    c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
+    if args.tune:
+        # the sweeps' block shapes tuned on a throwaway handle first, as
+        # bench.py does; later handles of this grid and precision use them
+        t = P.Sim(prm, precision=args.precision)
+        t.init(c0)
+        t.autotune(3)
+        t.close()
     sim = P.Sim(prm, precision=args.precision)
-    sim.init(P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"]))
+    sim.init(c0)
+    # the 12 copies of UPDATE_OLD calibrate the counters and mark where the
+    # summarised dispatches start
     sim.op("UPDATE_OLD")
     sim.run(args.steps)
     sim.synchronize()
@@ -40,27 +50,47 @@ def run(args):
           flush=True)
 
 
-def _counters(db, cname):
+def _counters(db, cname, last):
+    """Per kernel, the counter's values of its last `last[kernel]` dispatches
+    (the profiled handle's, which come last in the process)."""
     con = sqlite3.connect(db)
+    rows = list(con.execute("select kernel_name, dispatch_id, value from counters_collection where counter_name=? "
+                            "order by dispatch_id", (cname,)))
     d = collections.defaultdict(list)
-    for name, v in con.execute("select kernel_name, value from counters_collection where counter_name=?",
-                               (cname,)):
+    for name, di, v in rows:
         d[_short(name)].append(v)
-    return d
+    return {k: v[-last[k]:] for k, v in d.items() if last.get(k)}
+
+
+def _trace_rows(db):
+    """(name, calls, total us, average us, %) per kernel over the profiled
+    handle's dispatches (the stream of the last dispatch; a tuning handle
+    runs on its own stream) from its first copy kernel on."""
+    con = sqlite3.connect(db)
+    rows = list(con.execute("select name, dispatch_id, stream_id, start, end from kernels order by dispatch_id"))
+    st = rows[-1][2]
+    rows = [r for r in rows if r[2] == st]
+    first = min((di for n, di, s_, a, b in rows if _short(n).startswith("k_copy")), default=0)
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for name, di, s_, a, b in rows:
+        if di >= first:
+            agg[name][0] += 1
+            agg[name][1] += (b - a) / 1e3
+    tot = sum(v[1] for v in agg.values()) or 1.0
+    return [(n, c, t, t / c, 100.0 * t / tot) for n, (c, t) in agg.items()]
 
 
 def summary(args):
     out = []
-    con = sqlite3.connect(args.trace)
-    # top_kernels durations are in microseconds
-    rows = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    rows = _trace_rows(args.trace)
     out.append("| kernel | calls | total ms | avg us | % |")
     out.append("|---|---:|---:|---:|---:|")
     for name, calls, tot, avg, pct in sorted(rows, key=lambda r: -r[2]):
         out.append("| `%s` | %d | %.3f | %.2f | %.1f |" % (_short(name)[:60], calls, tot / 1e3, avg, pct))
     if args.fetch and args.write:
-        f = _counters(args.fetch, "FETCH_SIZE")
-        w = _counters(args.write, "WRITE_SIZE")
+        last = {_short(n): c for n, c, t, a, p in rows}
+        f = _counters(args.fetch, "FETCH_SIZE", last)
+        w = _counters(args.write, "WRITE_SIZE", last)
         copy = [k for k in f if k.startswith("k_copy")]
         if not copy:
             raise SystemExit("no copy kernel in the counter passes")
@@ -92,6 +122,7 @@ def main():
     r.add_argument("--config", default="front_256x128x128")
     r.add_argument("--steps", type=int, default=2)
     r.add_argument("--precision", default="plain", choices=["plain", "mixed"])
+    r.add_argument("--tune", action="store_true", help="autotune the sweeps' block shapes first (as bench.py)")
     s = sub.add_parser("summary")
     s.add_argument("--trace", required=True)
     s.add_argument("--fetch")
