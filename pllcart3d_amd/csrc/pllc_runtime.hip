@@ -39,6 +39,18 @@ bool pllc_hip_ok(const char* where, hipError_t e)
 
 extern "C" const char* pllc_last_error(void) { return g_err.c_str(); }
 
+// Graph capture and the calls that use the legacy stream or synchronise the
+// device (synchronous copies and memsets, hipFree) exclude each other across
+// host threads: the runtime refuses a legacy-stream operation while any
+// stream of the process is being captured, and a capture in which another
+// thread synchronised the device fails.  Handles on different threads take
+// this lock only around those calls, not around their launches.
+static std::recursive_mutex& pllc_api_mutex()
+{
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 // --------------------------------------------------------- host tables
 
 // cos(pi*p/(2n)) with p reduced to the first quadrant
@@ -255,6 +267,7 @@ extern "C" int pllc_ic_fill(const pllc_params* p, int recipe, unsigned long long
 template <typename T>
 bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
 {
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     const Dims& d = c.d;
     const size_t npk = (size_t)d.nx * d.ny * d.nz;
     auto get = [&](void** p, size_t bytes) {
@@ -361,6 +374,7 @@ static void apply_tuned(pllc_sim* s);
 
 static void graph_drop(pllc_sim* s)
 {
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     if (s->gx) (void)hipGraphExecDestroy(s->gx);
     s->gx = nullptr;
     s->gtab.clear();
@@ -393,6 +407,7 @@ static bool create_t(pllc_sim* s)
 {
     Ctx<T>& c = ctx<T>(s);
     const char* W = "pllc_sim_create";
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     pllc_fill_dims(&s->p, c.d, c.ph);
     c.st = s->st;
     const size_t n = npts(s);
@@ -436,6 +451,7 @@ extern "C" void pllc_sim_destroy(pllc_sim* s)
 {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     graph_drop(s);
     for (void* p : s->owned) (void)hipFree(p);
     for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
@@ -487,6 +503,7 @@ static void normalize(pllc_sim* s, Ctx<T>& c)
 template <typename T>
 static bool h2d(pllc_sim* s, T* dst, const double* src, size_t n, const char* W)
 {
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     if (sizeof(T) == 8) return pllc_hip_ok(W, hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice));
     std::vector<T> t(n);
     for (size_t e = 0; e < n; e++) t[e] = T(src[e]);
@@ -498,6 +515,7 @@ template <typename T>
 static bool d2h(pllc_sim* s, double* dst, const T* src, size_t n, const char* W)
 {
     if (!pllc_hip_ok(W, hipStreamSynchronize(s->st))) return false;
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     if (sizeof(T) == 8) return pllc_hip_ok(W, hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
     std::vector<T> t(n);
     if (!pllc_hip_ok(W, hipMemcpy(t.data(), src, n * sizeof(T), hipMemcpyDeviceToHost))) return false;
@@ -662,6 +680,7 @@ static int shift_t(pllc_sim* s, Ctx<T>& c, double Ampli, double Lamb, int Subhar
         h[(size_t)g.NX + k] = meshdx;
     }
     double* dev = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     if (!pllc_hip_ok(W, hipMalloc((void**)&dev, h.size() * 8))) return -1;
     int rc = -1;
     if (pllc_hip_ok(W, hipMemcpyAsync(dev, h.data(), h.size() * 8, hipMemcpyHostToDevice, s->st))) {
@@ -739,6 +758,7 @@ static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
     std::vector<void*> tab = ptr_table(c);
     if (s->gx && tab != s->gtab) graph_drop(s);
     if (!s->gx) {
+        std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
         if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
         fused_step<T>(s, c);
         fused_step<T>(s, c);
