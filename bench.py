@@ -156,7 +156,7 @@ def dct_flops_per_launch(prm):
 def roofline(kernels, cells, bytes_per_value, prm):
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     tmb = MEASURED_TRAFFIC_MB.get(bytes_per_value, {}).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
-    traffic = {dom: tmb * 1e6} if tmb else None
+    traffic = {dom: round(tmb * 1e6)} if tmb else None
     avg_s = kernels[dom]["avg_ms"] * 1e-3
     if dom == "dct" and not dct_is_fast(prm):
         # the dense contractions are matrix-core work; the fast transform
