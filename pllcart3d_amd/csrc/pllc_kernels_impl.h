@@ -215,7 +215,11 @@ __device__ __forceinline__ void rhs_mom_point(const Dims& d, const Phys& ph, con
     for (int m = 0; m < 4; m++) { av[m] = pa[o + ao[m]]; bv[m] = pb[o + bo[m]]; }
     const T mc0 = mu.cell[o - sF], mc1 = mu.cell[o];
     const T ma0 = mA[o], ma1 = mA[o + sA], mb0 = mB[o], mb1 = mB[o + sB];
-    const T cn0 = F == FW ? T(0) : cn[o], cn1 = F == FW ? T(0) : cn[o - sF];
+    // c enters only the body force fre (c) of u and v; with fre = 0 that
+    // term is a signed zero and the source the same bits without it, so c
+    // is not read (the flag is uniform over the launch)
+    const bool force = F != FW && ph.fre[F == FU ? 0 : 1] != 0.0;
+    const T cn0 = force ? cn[o] : T(0), cn1 = force ? cn[o - sF] : T(0);
     const T pgv = pg[o], nm1 = Nm1[o];
 
     // convec_term: upwind derivatives along x, y, z
@@ -286,7 +290,7 @@ __device__ __forceinline__ void rhs_mom_point(const Dims& d, const Phys& ph, con
     const T visc = alpha * (axx + axy + axz);
     const T Re = T(ph.Re);
     T bodyforce;
-    if (F == FW) bodyforce = alpha * (-sdh<T, FAST>(pgv, Re, ph.rRe));
+    if (!force) bodyforce = alpha * (-sdh<T, FAST>(pgv, Re, ph.rRe));
     else bodyforce = alpha * (-T(ph.fre[fi]) * (T(0.5) * (cn0 + cn1)) - sdh<T, FAST>(pgv, Re, ph.rRe));
     S[o] = dt * (conv + visc + bodyforce);
 }

@@ -144,6 +144,16 @@ def test_square_planes_track_oracle(oracle_mod, dct, n, monkeypatch):
         _compare_step(orc, sim, "%s 8x%dx%d step %d" % (dct, n, n, k + 1), pg_tol=1e-9, components=False)
 
 
+def test_body_force_steps_track_oracle(oracle_mod):
+    """Fx, Fy != 0: the body force -F/Re c of u and v (the only place c
+    enters the momentum sources); whole steps vs the oracle."""
+    prm, orc, sim = _pair(oracle_mod, nx=24, ny=12, nz=10, Fx=0.7, Fy=-0.4)
+    for k in range(3):
+        orc.run(1)
+        sim.run(1)
+        _compare_step(orc, sim, "body force step %d" % (k + 1))
+
+
 def test_wide_planes_track_oracle(oracle_mod):
     """ny nz = 67600 > 65536: the bulk velocity takes its two-pass form
     (per-block candidates, then one block), and the transforms the dense
