@@ -110,7 +110,9 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
     cy7[3] = cx7[3];
     cz7[3] = cx7[3];
     const T u0 = un[o], u1 = un[o + sx], v0 = vn[o], v1 = vn[o + sy], w0 = wn[o], w1 = wn[o + 1];
-    const T nm1 = Nm1[o];
+    // r = 0 (first substage): r N_nm1 is a signed zero, so N_nm1 is not read
+    // (the source keeps its value; at most the sign of a zero source differs)
+    const T nm1 = r != T(0) ? Nm1[o] : T(0);
 
     T D[6];
     const T uval = T(0.5) * (u1 + u0);
@@ -220,7 +222,7 @@ __device__ __forceinline__ void rhs_mom_point(const Dims& d, const Phys& ph, con
     // is not read (the flag is uniform over the launch)
     const bool force = F != FW && ph.fre[F == FU ? 0 : 1] != 0.0;
     const T cn0 = force ? cn[o] : T(0), cn1 = force ? cn[o - sF] : T(0);
-    const T pgv = pg[o], nm1 = Nm1[o];
+    const T pgv = pg[o], nm1 = r != T(0) ? Nm1[o] : T(0);   // as in k_rhs_c
 
     // convec_term: upwind derivatives along x, y, z
     const T alpha = g + r;
