@@ -138,10 +138,10 @@ PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
 # and profiles/step_mixed_256x128x128.md (MB, per kernel class; the DCT is
 # the mean of the forward and inverse plane transforms)
 MEASURED_TRAFFIC_MB = {
-    8.0: {"sweep_z": 188.2, "sweep_x": 260.4, "sweep_y": 189.8, "dct": 79.3, "corr_pg": 595.3,
-          "rhs_c": 255.5, "rhs_u": 393.7, "rhs_v": 395.9, "rhs_w": 360.3, "div": 138.8, "visc": 175.4},
-    4.0: {"sweep_z": 93.9, "sweep_x": 133.5, "sweep_y": 94.4, "dct": 34.0, "corr_pg": 299.2,
-          "rhs_c": 128.4, "rhs_u": 198.1, "rhs_v": 199.2, "rhs_w": 181.2, "div": 69.8, "visc": 88.0},
+    8.0: {"sweep_z": 188.2, "sweep_x": 260.7, "sweep_y": 192.9, "dct": 79.3, "corr_pg": 595.5,
+          "rhs_c": 244.3, "rhs_u": 348.3, "rhs_v": 348.8, "rhs_w": 349.2, "div": 138.8, "visc": 175.4},
+    4.0: {"sweep_z": 93.9, "sweep_x": 132.5, "sweep_y": 95.8, "dct": 34.0, "corr_pg": 299.1,
+          "rhs_c": 122.8, "rhs_u": 175.2, "rhs_v": 175.4, "rhs_w": 175.5, "div": 69.8, "visc": 88.0},
 }
 
 
