@@ -82,6 +82,7 @@ class Oracle:
 
     def __init__(self, params):
         self.L = load()
+        self.nmvp = params.Nmvp
         self.h = self.L.orc_create(C.byref(params))
         d = (C.c_int * 3)()
         self.n = self.L.orc_dims(self.h, d)
@@ -101,6 +102,11 @@ class Oracle:
 
     def init(self, c_full=None):
         self.L.orc_init(self.h, None if c_full is None else dp(np.ascontiguousarray(c_full)))
+        # Nmvp = 1: the reference exits when the slope parameter's bisection
+        # does not converge in 500 iterations (pllcart3d.c:812-817); the C
+        # restatement returns cm = -1 then
+        if self.nmvp == 1 and not self.L.orc_cm(self.h) > 0.0:
+            raise RuntimeError("oracle init: slope parameter did not converge in 500 iterations")
 
     def op(self, name, dt=0.0, rk=0, vbulk=0.0):
         return self.L.orc_op(self.h, OP_ID[name], dt, rk, vbulk)

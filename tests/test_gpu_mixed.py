@@ -43,6 +43,25 @@ def test_mixed_tracks_fp64_oracle(oracle_mod, shape):
         assert errs[f] <= tol, (f, errs[f])
 
 
+def test_mixed_nmvp_tracks_fp64_oracle(oracle_mod):
+    """The non-monotonic viscosity law (Nmvp = 1: slope parameter found on
+    the host in FP64, the law evaluated in FP32 by the viscosity kernel)
+    against the FP64 oracle after 10 steps, to the bars above."""
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1, Nmvp=1, Mum=1.5, Lambda=0.0)
+    c0 = P.ic_fill(prm, P.IC_NOISE, 3, 0.05)
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    sim = P.Sim(prm, "mixed")
+    sim.init(c0)
+    for n in range(10):
+        orc.run(1)
+        sim.run(1)
+    errs = {f: _err(sim, orc, f) for f in TOL}
+    print("[mixed nmvp] after 10 steps: %s" % " ".join("%s=%.2e" % kv for kv in errs.items()), flush=True)
+    for f, tol in TOL.items():
+        assert errs[f] <= tol, (f, errs[f])
+
+
 @pytest.mark.parametrize("n", [128, 256])
 def test_mixed_square_planes_track_fp64_oracle(oracle_mod, n):
     """ny = nz = n (the fast transform in FP32: one plane per block at 128,
