@@ -459,6 +459,22 @@ def test_slope_parameter_matches_analytic_root(O, M, Mum, Lam):
     assert abs(mu.max() - mum) < 1e-9 * mum
 
 
+def test_slope_parameter_without_root_fails(O):
+    """A Lambda that lambda(cm) never reaches in [0.1, 0.9]: the bisection
+    gives up after 500 iterations, where the reference writes "Lambda did not
+    converge" and exits (pllcart3d.c:812-817); the restatement returns -1
+    and the oracle's init raises."""
+    import pllcart3d_amd as P
+    n = 2001
+    c1d = np.linspace(0.0, 1.0, n)
+    L = O.load()
+    assert L.orc_slope_parameter(dp(c1d), n, 2.5, 1.5, 1e3) == -1.0
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1, Nmvp=1, Mum=1.5, Lambda=1e3)
+    orc = O.Oracle(prm)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        orc.init(P.ic_fill(prm, P.IC_NOISE, 3, 0.05))
+
+
 # ------------------------------------------------------ wave perturbation
 
 def test_spline_is_the_natural_cubic_spline(O):
