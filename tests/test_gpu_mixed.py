@@ -43,6 +43,26 @@ def test_mixed_tracks_fp64_oracle(oracle_mod, shape):
         assert errs[f] <= tol, (f, errs[f])
 
 
+def test_mixed_thomas_tracks_fp64_oracle(oracle_mod, monkeypatch):
+    """PLLC_LINE_SOLVER=thomas (one thread per line, the reference's
+    arithmetic in the line solves and explicit stencils) in FP32 against the
+    FP64 oracle after 10 steps, to the bars above."""
+    monkeypatch.setenv("PLLC_LINE_SOLVER", "thomas")
+    prm = P.make_params(32, 16, 16, Vratio=2.0, Pe=500.0, Inf_thick=0.05)
+    c0 = P.ic_fill(prm, P.IC_NOISE, 1, 0.01)
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    sim = P.Sim(prm, "mixed")
+    sim.init(c0)
+    for n in range(10):
+        orc.run(1)
+        sim.run(1)
+    errs = {f: _err(sim, orc, f) for f in TOL}
+    print("[mixed thomas] after 10 steps: %s" % " ".join("%s=%.2e" % kv for kv in errs.items()), flush=True)
+    for f, tol in TOL.items():
+        assert errs[f] <= tol, (f, errs[f])
+
+
 def test_mixed_nmvp_tracks_fp64_oracle(oracle_mod):
     """The non-monotonic viscosity law (Nmvp = 1: slope parameter found on
     the host in FP64, the law evaluated in FP32 by the viscosity kernel)
