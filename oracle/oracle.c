@@ -1181,6 +1181,19 @@ static void update_old(orc_sim *s)
 /* ------------------------------------------------------------------ set-up */
 
 /* pllcart3d.c:75-154 with Initial == 1 and Nmvp == 0 */
+/* The momentum coefficient tables from the stored mu_old, as A_umom/
+ * A_vmom/A_wmom (pllcart3d.c:1443-1536) leave them after set-up
+ * (pllcart3d.c:131-133): for a state written into the fields directly. */
+void orc_rebuild_A(orc_sim *s)
+{
+    double *muo[4] = {s->f[ORC_MUO_CELL], s->f[ORC_MUO_XY], s->f[ORC_MUO_XZ], s->f[ORC_MUO_YZ]};
+    build_A(s, 0, muo); build_A(s, 1, muo); build_A(s, 2, muo);
+}
+
+/* Set the non-monotonic law's slope parameter (for a state written into
+ * the fields directly; orc_init finds it from the initial c). */
+void orc_set_cm(orc_sim *s, double cm) { s->cm = cm; }
+
 void orc_init(orc_sim *s, const double *c_full)
 {
     const rng_t g = s->g, R = s->ru;

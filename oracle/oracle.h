@@ -70,6 +70,10 @@ double orc_slope_parameter(const double *c1d, int n, double M, double Mum, doubl
 /* Set-up as in pllcart3d.c:75-154.  c_full (nullable) replaces the erf IC
  * of initial_condition (pllcart3d.c:501-508) on every cell incl. ghosts. */
 void orc_init(orc_sim *s, const double *c_full);
+/* Rebuild the u, v, w coefficient tables from MUO_* (after writing fields) */
+void orc_rebuild_A(orc_sim *s);
+/* Set the slope parameter cm the Nmvp = 1 law uses (for a written state) */
+void orc_set_cm(orc_sim *s, double cm);
 /* One operator of the substage body (pllcart3d.c:169-194).  rk selects
  * RK1[rk]/RK2[rk].  Returns the value for BULK_VELO / FIND_CFL, else 0. */
 double orc_op(orc_sim *s, int op, double dt, int rk, double vbulk);

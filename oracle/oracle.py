@@ -53,6 +53,8 @@ def load():
         L.orc_slope_parameter.restype = C.c_double
         L.orc_slope_parameter.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double]
         L.orc_init.argtypes = [C.c_void_p, dp]
+        L.orc_rebuild_A.argtypes = [C.c_void_p]
+        L.orc_set_cm.argtypes = [C.c_void_p, C.c_double]
         L.orc_op.restype = C.c_double
         L.orc_op.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]
         L.orc_run.restype = C.c_double
@@ -107,6 +109,14 @@ class Oracle:
         # restatement returns cm = -1 then
         if self.nmvp == 1 and not self.L.orc_cm(self.h) > 0.0:
             raise RuntimeError("oracle init: slope parameter did not converge in 500 iterations")
+
+    def rebuild_A(self):
+        """Coefficient tables of u, v, w from MUO_* (after writing fields)."""
+        self.L.orc_rebuild_A(self.h)
+
+    def set_cm(self, cm):
+        """Slope parameter of the Nmvp = 1 law for a state written directly."""
+        self.L.orc_set_cm(self.h, cm)
 
     def op(self, name, dt=0.0, rk=0, vbulk=0.0):
         return self.L.orc_op(self.h, OP_ID[name], dt, rk, vbulk)
