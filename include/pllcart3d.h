@@ -304,6 +304,49 @@ int pllc_sim_time_kernel(pllc_sim *s, const char *name, int nsteps, double *avg_
 /* names of the kernel classes known to pllc_sim_time_kernel, ';'-separated */
 const char *pllc_kernel_names(void);
 
+/* ==================================== learned closure (new) ========= */
+
+/* A 3-layer 3x3x3 convolutional network (16 channels, BF16 operands on the
+ * matrix cores, FP32 accumulation) that corrects concentration's explicit
+ * advective term N (Hc_op, pllcart3d.c:1539-1644) on a coarse grid of
+ * nx x ny x nz cells.  No counterpart in the reference.  Inputs per cell:
+ * c, the cell-centred u, v, w and mu_old.cell, each (value - shift) * scale;
+ * output y, the correction being out_scale * y (zero within 4 cells of the
+ * inlet and the outlet).  Parameters: pllc_cnn_nparams() values, W1
+ * [27][16][16], b1 [16], W2 [27][16][16], b2 [16], W3 [27][16][16] (output
+ * channel 0 used), b3 [16]; offset (oi, oj, ok) in {-1,0,1}^3 at index
+ * 9 (oi+1) + 3 (oj+1) + (ok+1), zero padding outside the grid. */
+typedef struct pllc_cnn pllc_cnn;
+pllc_cnn *pllc_cnn_create(int nx, int ny, int nz);
+void pllc_cnn_destroy(pllc_cnn *m);
+int pllc_cnn_nparams(void);
+int pllc_cnn_set_params(pllc_cnn *m, const double *params);   /* also resets Adam's moments */
+int pllc_cnn_get_params(pllc_cnn *m, double *params);
+int pllc_cnn_set_norm(pllc_cnn *m, const double *shift, const double *scale, double out_scale);
+/* forward pass on host inputs x [cells][5] (shifted and scaled), y [cells] */
+int pllc_cnn_forward(pllc_cnn *m, const float *x, double *y);
+/* a training sample kept on the device: inputs [cells][5], targets [cells]
+ * (in units of out_scale), mask [cells] (1: the cell is in the loss);
+ * returns its index */
+int pllc_cnn_add_sample(pllc_cnn *m, const float *x, const float *target, const unsigned char *mask);
+int pllc_cnn_num_samples(const pllc_cnn *m);
+/* masked mean-square loss and its gradient (pllc_cnn_nparams() values) on
+ * one sample */
+int pllc_cnn_loss_grad(pllc_cnn *m, int sample, double *loss, double *grad);
+/* nsteps Adam steps, step s on sample order[s % norder]; losses (nullable)
+ * receives each step's loss before its update */
+int pllc_cnn_train(pllc_cnn *m, int nsteps, const int *order, int norder, double lr, double *losses);
+/* the simulation adds the network's correction to N in every concentration
+ * update (NULL: none); the network's grid must equal the simulation's */
+int pllc_sim_set_closure(pllc_sim *s, pllc_cnn *m);
+/* nsteps steps with the closure, checked after every pair of steps: c within
+ * [-tol_bound, 1 + tol_bound] and |dt sum(correction)| <= tol_mass |sum(c)|
+ * over the interior cells; a pair that fails is run again from its start
+ * without the correction (plain WENO); *fallbacks counts those steps */
+int pllc_sim_run_guarded(pllc_sim *s, int nsteps, double tol_bound, double tol_mass, int *fallbacks);
+/* the guard's test on stats = {min c, max c, sum c, sum correction}: 1 pass */
+int pllc_closure_guard_ok(const double *stats, double dt, double tol_bound, double tol_mass);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,23 @@ def load():
     L.pllc_sim_time_steps.argtypes = [vp, C.c_int, C.c_int, dp]
     L.pllc_sim_time_kernel.argtypes = [vp, C.c_char_p, C.c_int, dp, C.POINTER(C.c_int)]
     L.pllc_kernel_names.restype = C.c_char_p
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int)
+    L.pllc_cnn_create.restype = vp
+    L.pllc_cnn_create.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.pllc_cnn_destroy.argtypes = [vp]
+    L.pllc_cnn_nparams.argtypes = []
+    L.pllc_cnn_set_params.argtypes = [vp, dp]
+    L.pllc_cnn_get_params.argtypes = [vp, dp]
+    L.pllc_cnn_set_norm.argtypes = [vp, dp, dp, C.c_double]
+    L.pllc_cnn_forward.argtypes = [vp, fp, dp]
+    L.pllc_cnn_add_sample.argtypes = [vp, fp, fp, C.POINTER(C.c_ubyte)]
+    L.pllc_cnn_num_samples.argtypes = [vp]
+    L.pllc_cnn_loss_grad.argtypes = [vp, C.c_int, dp, dp]
+    L.pllc_cnn_train.argtypes = [vp, C.c_int, ip, C.c_int, C.c_double, dp]
+    L.pllc_sim_set_closure.argtypes = [vp, vp]
+    L.pllc_sim_run_guarded.argtypes = [vp, C.c_int, C.c_double, C.c_double, ip]
+    L.pllc_closure_guard_ok.argtypes = [dp, C.c_double, C.c_double, C.c_double]
     _lib = L
     return L
 

@@ -22,6 +22,7 @@ HIP_SOURCES = [
     ("pllc_kernels_f32.hip", ["-ffp-contract=on", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("pllc_runtime.hip", ["-ffp-contract=off"]),
     ("pllc_abi.hip", ["-ffp-contract=off"]),
+    ("pllc_closure.hip", ["-ffp-contract=on"]),
 ]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -8,10 +8,12 @@
 // mirror, host code moves data with pllc_to_device / pllc_to_host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -66,6 +68,13 @@ struct DropIn {
     // the Visc set of the last A_umom/A_vmom/A_wmom (or momentum) call on a
     // Velo: the coefficients pressure_gradient applies (pllcart3d.c:2988)
     std::unordered_map<const void*, const Visc*> coef;
+    // staging of the kernels on host data (pll_tridiag, tridiag_1rhs,
+    // calculate_weno): a device buffer and a pinned host buffer, kept
+    // between calls and grown only when a call needs more than any before
+    double* line_dev = nullptr;
+    size_t line_dev_n = 0;
+    double* line_host = nullptr;
+    size_t line_host_n = 0;
 };
 
 DropIn& din()
@@ -800,12 +809,36 @@ extern "C" void update_old_visc(Visc* mu_old, Visc* mu_new)
 // ==================================================== kernels on host data
 
 namespace {
+// a device buffer owned by one call (DCT entry points, diagnostics)
 struct DevBuf {
     double* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
     bool get(size_t n, const char* W) { return pllc_hip_ok(W, hipMalloc((void**)&p, (n ? n : 1) * 8)); }
 };
 }  // namespace
+
+// device and pinned host staging of at least n doubles each (kept in the
+// drop-in context; a call allocates only when it needs more than any call
+// before it)
+static bool staging(DropIn& d, size_t n, bool host, const char* W)
+{
+    n = n ? n : 1;
+    if (d.line_dev_n < n) {
+        if (d.line_dev) (void)hipFree(d.line_dev);
+        d.line_dev = nullptr;
+        d.line_dev_n = 0;
+        if (!pllc_hip_ok(W, hipMalloc((void**)&d.line_dev, n * 8))) return false;
+        d.line_dev_n = n;
+    }
+    if (host && d.line_host_n < n) {
+        if (d.line_host) (void)hipHostFree(d.line_host);
+        d.line_host = nullptr;
+        d.line_host_n = 0;
+        if (!pllc_hip_ok(W, hipHostMalloc((void**)&d.line_host, n * 8, hipHostMallocDefault))) return false;
+        d.line_host_n = n;
+    }
+    return true;
+}
 
 // calculate_weno (3455-3507) on a host D1 array: the device evaluation of
 // the same formula used by the stencil kernels
@@ -814,46 +847,60 @@ extern "C" double calculate_weno(double* d1, int kk, int sign)
     const char* W = "calculate_weno";
     DropIn& d = din();
     if (!d.have_stream) return NAN;
-    double v[5];
-    for (int m = 0; m < 5; m++) v[m] = d1[kk + sign * m];
-    DevBuf b;
-    if (!b.get(8, W)) return NAN;
-    int* ik = (int*)(b.p + 6);
-    const int hk[2] = {0, 1};
-    double out = NAN;
-    if (!pllc_hip_ok(W, hipMemcpyAsync(b.p, v, 5 * 8, hipMemcpyHostToDevice, d.st))) return NAN;
-    if (!pllc_hip_ok(W, hipMemcpyAsync(ik, hk, 8, hipMemcpyHostToDevice, d.st))) return NAN;
-    pllc::weno_batch(b.p, ik, ik + 1, b.p + 5, 1, d.st);
-    if (!pllc_hip_ok(W, hipMemcpyAsync(&out, b.p + 5, 8, hipMemcpyDeviceToHost, d.st))) return NAN;
+    if (!staging(d, 8, true, W)) return NAN;
+    double* h = d.line_host;
+    for (int m = 0; m < 5; m++) h[m] = d1[kk + sign * m];
+    int* hk = (int*)(h + 6);
+    hk[0] = 0;
+    hk[1] = 1;
+    int* ik = (int*)(d.line_dev + 6);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(d.line_dev, h, 8 * 8, hipMemcpyHostToDevice, d.st))) return NAN;
+    pllc::weno_batch(d.line_dev, ik, ik + 1, d.line_dev + 5, 1, d.st);
+    if (!pllc_hip_ok(W, hipMemcpyAsync(h + 5, d.line_dev + 5, 8, hipMemcpyDeviceToHost, d.st))) return NAN;
     if (!pllc_hip_ok(W, hipStreamSynchronize(d.st))) return NAN;
-    return out;
+    return h[5];
 }
 
+// nsys lines of n rows (host pointers per line) solved on the device: the
+// lines are staged row-major, [array][row][line], so the kernel's loads and
+// stores of one row are contiguous across the lines of a wave
 static int lines_on_device(int nsys, int n, double* const* a, double* const* b, double* const* c, double* const* r,
                            int mode, const char* W)
 {
     DropIn& d = din();
-    if (!d.have_stream || nsys <= 0 || n <= 0) return -1;
+    if (!d.have_stream) return -1;
+    if (nsys <= 0 || n <= 0) return 0;
     const size_t L = (size_t)nsys * n;
-    std::vector<double> h(4 * L);
-    for (int s = 0; s < nsys; s++) {
-        memcpy(&h[0 * L + (size_t)s * n], a[s], n * 8);
-        memcpy(&h[1 * L + (size_t)s * n], b[s], n * 8);
-        memcpy(&h[2 * L + (size_t)s * n], c[s], n * 8);
-        memcpy(&h[3 * L + (size_t)s * n], r[s], n * 8);
+    if (!staging(d, 5 * L, true, W)) return -1;
+    double* h = d.line_host;
+    double* const* src[4] = {a, b, c, r};
+    // blocks of 64 lines: each line is read sequentially, each row of a
+    // block written as 64 consecutive doubles
+    for (int s0 = 0; s0 < nsys; s0 += 64) {
+        const int s1 = std::min(nsys, s0 + 64);
+        for (int k = 0; k < 4; k++)
+            for (int i = 0; i < n; i++) {
+                double* row = h + k * L + (size_t)i * nsys;
+                for (int s = s0; s < s1; s++) row[s] = src[k][s][i];
+            }
     }
-    DevBuf buf;
-    if (!buf.get(5 * L, W)) return -1;
-    if (!pllc_hip_ok(W, hipMemcpyAsync(buf.p, h.data(), 4 * L * 8, hipMemcpyHostToDevice, d.st))) return -1;
-    pllc::thomas_lines(nsys, n, buf.p, buf.p + L, buf.p + 2 * L, buf.p + 3 * L, buf.p + 4 * L, mode, d.st);
-    if (!pllc_hip_ok(W, hipMemcpyAsync(h.data(), buf.p, 4 * L * 8, hipMemcpyDeviceToHost, d.st))) return -1;
+    double* g = d.line_dev;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(g, h, 4 * L * 8, hipMemcpyHostToDevice, d.st))) return -1;
+    pllc::thomas_lines(nsys, n, 1, nsys, g, g + L, g + 2 * L, g + 3 * L, g + 4 * L, mode, d.st);
+    if (!pllc_hip_ok(W, hipGetLastError())) return -1;
+    // mode 1 returns b, c and r; mode 0 only r
+    const int k0 = mode == 1 ? 1 : 3;
+    if (!pllc_hip_ok(W, hipMemcpyAsync(h + k0 * L, g + k0 * L, (4 - k0) * L * 8, hipMemcpyDeviceToHost, d.st)))
+        return -1;
     if (!pllc_hip_ok(W, hipStreamSynchronize(d.st))) return -1;
-    for (int s = 0; s < nsys; s++) {
-        if (mode == 1) {
-            memcpy(b[s], &h[1 * L + (size_t)s * n], n * 8);
-            memcpy(c[s], &h[2 * L + (size_t)s * n], n * 8);
-        }
-        memcpy(r[s], &h[3 * L + (size_t)s * n], n * 8);
+    double* const* dst[4] = {a, b, c, r};
+    for (int s0 = 0; s0 < nsys; s0 += 64) {
+        const int s1 = std::min(nsys, s0 + 64);
+        for (int k = k0; k < 4; k++)
+            for (int i = 0; i < n; i++) {
+                const double* row = h + k * L + (size_t)i * nsys;
+                for (int s = s0; s < s1; s++) dst[k][s][i] = row[s];
+            }
     }
     return 0;
 }
@@ -891,7 +938,7 @@ extern "C" void plan_dct_3d(int ndim1, int ndim2, int num_transforms)
 {
     const char* W = "plan_dct_3d";
     DctPlan& p = plan();
-    for (double* q : {p.f1, p.f2, p.i1, p.i2}) if (q) hipFree(q);
+    for (double* q : {p.f1, p.f2, p.i1, p.i2}) if (q) (void)hipFree(q);
     p = DctPlan();
     p.n1 = ndim1; p.n2 = ndim2; p.nt = num_transforms;
     std::vector<double> h((size_t)std::max(ndim1, ndim2) * std::max(ndim1, ndim2));

@@ -51,6 +51,9 @@ struct Ctx {
     double* scal = nullptr;   // S_SIZE device scalars
     double* part = nullptr;   // reduction partials
     double* tval = nullptr;   // device copy of tval
+    // additive correction of concentration's explicit term N (the learned
+    // closure's output, one field); null: none
+    T* ncorr = nullptr;
     long npart = 0;
     // ADI line solver: 0 partitioned chunks (k_sweep_part*), 1 one thread
     // per line with the reference's Thomas arithmetic (k_sweep*)
@@ -114,8 +117,9 @@ struct Ops {
 // on y; cp and fp are N-value scratch
 void spline_line(int N, double delta, const double* x, const double* xx, double* y, double* cp, double* fp,
                  hipStream_t st);
-void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
-                  hipStream_t st);
+// element (line l, row i) at l * ls + i * rs in each of a, b, c, r, cc
+void thomas_lines(int nsys, int n, long ls, long rs, const double* a, double* b, double* c, double* r, double* cc,
+                  int mode, hipStream_t st);
 void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int n, hipStream_t st);
 // C (M x N) = A (M x K) * B (K x N), batched with strides, FP64 (fdct_3d)
 void gemm_nn(int M, int N, int K, const double* A, long sA, const double* B, long sB, double* C, long sC, int batch,

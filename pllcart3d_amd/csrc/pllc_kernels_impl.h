@@ -88,7 +88,8 @@ template <typename T, bool FAST>
 __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restrict__ cold, const T* __restrict__ un,
                                                const T* __restrict__ vn, const T* __restrict__ wn,
                                                const T* __restrict__ Nm1, T* __restrict__ N, T* __restrict__ S,
-                                               const T* __restrict__ stale, const double* __restrict__ scal, T g, T r)
+                                               const T* __restrict__ stale, const T* __restrict__ ncorr,
+                                               const double* __restrict__ scal, T g, T r)
 {
     const int2 bt = xcd_tile();
     const int k = bt.x * blockDim.x + threadIdx.x + 1;
@@ -129,7 +130,10 @@ __global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restr
 #pragma unroll
     for (int m = 0; m < 6; m++) D[m] = sdh<T, FAST>(cz7[m + 1] - cz7[m], T(d.dz), d.rdz);
     const T czd = upwind_d<T, FAST>(D, k, R.ks, R.ke, wval >= T(0.0) ? 1 : -1, T(d.dz), d.rdz, stale[2], stale[3]);
-    const T nval = -(wval * czd + convx + convy);
+    T nval = -(wval * czd + convx + convy);
+    // the learned closure's correction of N (no counterpart in the
+    // reference; ncorr is null without a closure)
+    if (ncorr) nval += ncorr[o];
     N[o] = nval;
 
     T ax, bx, cx, ay, by, cy, az, bz, cz;
@@ -1909,37 +1913,51 @@ void diag_wmax(const Dims& d, const double* w, double* part, int nb, hipStream_t
 #endif  // PLLC_F64_TU
 
 #ifdef PLLC_F64_TU
-// batched single-line Thomas for the Tds entry points: lines stored
-// line-major [line][row]; mode 0 = tridiag_1rhs (b kept, solution in r),
-// mode 1 = pll_tridiag (b, c overwritten as in pll_tridiag.c:21-47).
-__global__ void k_thomas_lines(int nsys, int n, const double* __restrict__ a, double* __restrict__ b,
-                               double* __restrict__ c, double* __restrict__ r, double* __restrict__ cc, int mode)
+// batched single-line Thomas for the Tds entry points.  Element (line l,
+// row i) of each array sits at l * ls + i * rs: the drop-in stages lines
+// row-major (ls = 1, rs = nsys), so the 64 lanes of a wave read 64
+// consecutive doubles of one row; line-major staging (ls = n, rs = 1) is
+// accepted too.  mode 0 = tridiag_1rhs (b kept, solution in r, factors in
+// cc), mode 1 = pll_tridiag (b, c overwritten as in pll_tridiag.c:21-47,
+// cc unused).
+__global__ void k_thomas_lines(int nsys, int n, long ls, long rs, const double* __restrict__ a,
+                               double* __restrict__ b, double* __restrict__ c, double* __restrict__ r,
+                               double* __restrict__ cc, int mode)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nsys) return;
-    const long o = (long)l * n;
+    const long o = (long)l * ls;
     const double zero = 0.0;
     if (mode == 0) {
         double bb = b[o];
         cc[o] = c[o] / bb;
         r[o] = r[o] / bb;
         for (int i = 1; i < n; i++) {
-            bb = b[o + i] - cc[o + i - 1] * a[o + i];
-            cc[o + i] = c[o + i] / bb;
-            r[o + i] = (r[o + i] - r[o + i - 1] * a[o + i]) / bb;
+            const long p = o + i * rs, q = p - rs;
+            bb = b[p] - cc[q] * a[p];
+            cc[p] = c[p] / bb;
+            r[p] = (r[p] - r[q] * a[p]) / bb;
         }
-        for (int i = n - 2; i >= 0; i--) r[o + i] = r[o + i] - r[o + i + 1] * cc[o + i];
+        for (int i = n - 2; i >= 0; i--) {
+            const long p = o + i * rs;
+            r[p] = r[p] - r[p + rs] * cc[p];
+        }
     } else {
         b[o] = b[o] - zero * a[o];
         c[o] = c[o] / b[o];
         r[o] = (r[o] - zero * a[o]) / b[o];
         for (int i = 1; i < n; i++) {
-            b[o + i] = (b[o + i] - c[o + i - 1] * a[o + i]);
-            c[o + i] = c[o + i] / b[o + i];
-            r[o + i] = (r[o + i] - r[o + i - 1] * a[o + i]) / b[o + i];
+            const long p = o + i * rs, q = p - rs;
+            b[p] = (b[p] - c[q] * a[p]);
+            c[p] = c[p] / b[p];
+            r[p] = (r[p] - r[q] * a[p]) / b[p];
         }
-        r[o + n - 1] = r[o + n - 1] - c[o + n - 1] * zero;
-        for (int i = n - 2; i >= 0; i--) r[o + i] = r[o + i] - c[o + i] * r[o + i + 1];
+        const long e = o + (long)(n - 1) * rs;
+        r[e] = r[e] - c[e] * zero;
+        for (int i = n - 2; i >= 0; i--) {
+            const long p = o + i * rs;
+            r[p] = r[p] - c[p] * r[p + rs];
+        }
     }
 }
 
@@ -1959,7 +1977,7 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
 
 #define PLLC_INST_RHS(T, FAST)                                                                                 \
     template __global__ void k_rhs_c<T, FAST>(Dims, Phys, const T*, const T*, const T*, const T*, const T*, T*,  \
-                                              T*, const T*, const double*, T, T);                               \
+                                              T*, const T*, const T*, const double*, T, T);                     \
     template __global__ void k_rhs_mom<T, FU, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
                                                     const T*, const T*, const T*, T*, T*, const T*,             \
                                                     const double*, T, T);                                       \
@@ -2930,11 +2948,11 @@ void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un,
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     if (fast_stencils(c))
         PLLC_L(c, "rhs_c", (k_rhs_c<T, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
-                                                                     c.f[C_N], c.f[F_SRC], c.stale + 0, c.scal, T(g),
+                                                                     c.f[C_N], c.f[F_SRC], c.stale + 0, c.ncorr, c.scal, T(g),
                                                                      T(r))));
     else
         PLLC_L(c, "rhs_c", (k_rhs_c<T, false><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
-                                                                      c.f[C_N], c.f[F_SRC], c.stale + 0, c.scal, T(g),
+                                                                      c.f[C_N], c.f[F_SRC], c.stale + 0, c.ncorr, c.scal, T(g),
                                                                       T(r))));
     const int F[1] = {FC};
     T* S[1] = {c.f[F_SRC]};
@@ -3248,10 +3266,10 @@ void spline_line(int N, double delta, const double* x, const double* xx, double*
     k_spline_lines<double><<<1, 64, 0, st>>>(N, 1, 1, 0, 0, 1, delta, x, xx, nullptr, y, cp, fp);
 }
 
-void thomas_lines(int nsys, int n, const double* a, double* b, double* c, double* r, double* cc, int mode,
-                  hipStream_t st)
+void thomas_lines(int nsys, int n, long ls, long rs, const double* a, double* b, double* c, double* r, double* cc,
+                  int mode, hipStream_t st)
 {
-    k_thomas_lines<<<(nsys + 255) / 256, 256, 0, st>>>(nsys, n, a, b, c, r, cc, mode);
+    k_thomas_lines<<<(nsys + 255) / 256, 256, 0, st>>>(nsys, n, ls, rs, a, b, c, r, cc, mode);
 }
 
 void weno_batch(const double* d1, const int* kk, const int* sg, double* out, int n, hipStream_t st)

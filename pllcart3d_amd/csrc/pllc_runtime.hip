@@ -368,6 +368,12 @@ struct pllc_sim {
     bool copy_next = false;
     hipGraphExec_t gx = nullptr;
     std::vector<void*> gtab;
+    // learned closure: the network whose correction of N each substage adds
+    // (null: none); snapshot buffers of the guarded run (NSTATE fields, the
+    // device scalars and tval) and the guard's reduction scratch
+    pllc_cnn* cnn = nullptr;
+    std::vector<void*> snap;
+    double* gscratch = nullptr;
 };
 
 static void apply_tuned(pllc_sim* s);
@@ -629,6 +635,9 @@ static double op_t(pllc_sim* s, Ctx<T>& c, int op, double dt, int rk, double vbu
     T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
     switch (op) {
     case PLLC_OP_CONCENTRATION:
+        if (s->cnn && c.ncorr)
+            pllc_cnn_correct<T>(s->cnn, c.d, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.f[MUO_CELL], c.ncorr,
+                                c.st);
         Ops<T>::stale(c, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         Ops<T>::concentration(c, g, r, rk, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW]);
         break;
@@ -718,6 +727,11 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
     for (int rk = 0; rk < 3; rk++) {
         const double g = kRK1[rk], r = kRK2[rk];
         Ops<T>::bulk_stale(c, rk, g + r, true, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
+        // the learned closure's correction of N from the substage's start
+        // state (c, u, v, w and mu_old.cell)
+        if (s->cnn && c.ncorr)
+            pllc_cnn_correct<T>(s->cnn, c.d, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.f[MUO_CELL], c.ncorr,
+                                c.st);
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
         Ops<T>::viscosity(c, c.f[C_NEW], mun);
@@ -833,6 +847,125 @@ extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* 
     if (!pllc_hip_ok("pllc_sim_time_steps", hipEventElapsedTime(&f, s->e0, s->e1))) return -1;
     if (ms) *ms = f;
     return 0;
+}
+
+// ------------------------------------------------------- learned closure
+//
+// A simulation with a closure adds the network's correction to the explicit
+// advective term N of every concentration update (no counterpart in the
+// reference).  The guarded run checks, after every pair of steps, that c
+// stays within [-tol_bound, 1 + tol_bound] and that the mass the correction
+// injects in a step, |dt sum(ncorr)|, stays within tol_mass of the mass
+// sum(c) over the interior cells; when either is violated the pair is run
+// again from its start without the correction (plain WENO).
+
+extern "C" int pllc_closure_guard_ok(const double* stats, double dt, double tol_bound, double tol_mass)
+{
+    const double lo = stats[0], hi = stats[1], mass = stats[2], corr = stats[3];
+    if (!std::isfinite(lo) || !std::isfinite(hi) || !std::isfinite(mass) || !std::isfinite(corr)) return 0;
+    if (lo < -tol_bound || hi > 1.0 + tol_bound) return 0;
+    if (fabs(dt * corr) > tol_mass * fabs(mass)) return 0;
+    return 1;
+}
+
+template <typename T>
+static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m)
+{
+    const char* W = "pllc_sim_set_closure";
+    graph_drop(s);
+    s->cnn = nullptr;
+    if (!m) {
+        c.ncorr = nullptr;
+        return 0;
+    }
+    if (!pllc_cnn_dims_match(m, c.d)) {
+        pllc_set_error(W, "network grid differs from the simulation's");
+        return -1;
+    }
+    if (!c.ncorr) {
+        void* p = nullptr;
+        if (!pllc_hip_ok(W, hipMalloc(&p, npts(s) * sizeof(T)))) return -1;
+        s->owned.push_back(p);
+        if (!pllc_hip_ok(W, hipMemset(p, 0, npts(s) * sizeof(T)))) return -1;
+        c.ncorr = (T*)p;
+    }
+    s->cnn = m;
+    return 0;
+}
+
+extern "C" int pllc_sim_set_closure(pllc_sim* s, pllc_cnn* m)
+{
+    return s->prec ? set_closure_t<float>(s, s->cf, m) : set_closure_t<double>(s, s->cd, m);
+}
+
+template <typename T>
+static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, double tol_bound, double tol_mass, int* fallbacks)
+{
+    const char* W = "pllc_sim_run_guarded";
+    const size_t n = npts(s);
+    if (s->snap.empty()) {
+        for (int f = 0; f < NSTATE + 1; f++) {
+            void* p = nullptr;
+            const size_t bytes = f < NSTATE ? n * sizeof(T) : (S_SIZE + 1) * 8;
+            if (!pllc_hip_ok(W, hipMalloc(&p, bytes))) return -1;
+            s->owned.push_back(p);
+            s->snap.push_back(p);
+        }
+        if (!pllc_hip_ok(W, hipMalloc((void**)&s->gscratch, 4 * 256 * 8))) return -1;
+        s->owned.push_back(s->gscratch);
+    }
+    int fb = 0;
+    while (nsteps > 0) {
+        const int k = nsteps >= 2 ? 2 : 1;
+        // snapshot: the pointer table, the fields, the device scalars, tval
+        T* tab[NFIELDS];
+        std::copy(c.f, c.f + NFIELDS, tab);
+        const bool stale0 = s->new_stale, copy0 = s->copy_next;
+        for (int f = 0; f < NSTATE; f++)
+            if (!pllc_hip_ok(W, hipMemcpyAsync(s->snap[f], c.f[f], n * sizeof(T), hipMemcpyDeviceToDevice, s->st)))
+                return -1;
+        double* sd = (double*)s->snap[NSTATE];
+        if (!pllc_hip_ok(W, hipMemcpyAsync(sd, c.scal, S_SIZE * 8, hipMemcpyDeviceToDevice, s->st)) ||
+            !pllc_hip_ok(W, hipMemcpyAsync(sd + S_SIZE, c.tval, 8, hipMemcpyDeviceToDevice, s->st)))
+            return -1;
+        steps(s, k);
+        double st[4], dt = 0.0;
+        if (!pllc_guard_stats<T>(c.d, c.f[C_OLD], c.ncorr, s->gscratch, st, s->st, W)) return -1;
+        if (!pllc_hip_ok(W, hipMemcpy(&dt, c.scal + S_DT, 8, hipMemcpyDeviceToHost))) return -1;
+        if (!pllc_closure_guard_ok(st, dt, tol_bound, tol_mass)) {
+            // restore and run the pair with plain WENO
+            std::copy(tab, tab + NFIELDS, c.f);
+            s->new_stale = stale0;
+            s->copy_next = copy0;
+            for (int f = 0; f < NSTATE; f++)
+                if (!pllc_hip_ok(W, hipMemcpyAsync(c.f[f], s->snap[f], n * sizeof(T), hipMemcpyDeviceToDevice, s->st)))
+                    return -1;
+            if (!pllc_hip_ok(W, hipMemcpyAsync(c.scal, sd, S_SIZE * 8, hipMemcpyDeviceToDevice, s->st)) ||
+                !pllc_hip_ok(W, hipMemcpyAsync(c.tval, sd + S_SIZE, 8, hipMemcpyDeviceToDevice, s->st)))
+                return -1;
+            pllc_cnn* m = s->cnn;
+            T* nc = c.ncorr;
+            s->cnn = nullptr;
+            c.ncorr = nullptr;
+            for (int q = 0; q < k; q++) step_any(s);
+            s->cnn = m;
+            c.ncorr = nc;
+            fb += k;
+        }
+        nsteps -= k;
+    }
+    if (fallbacks) *fallbacks = fb;
+    return finish(s, W);
+}
+
+extern "C" int pllc_sim_run_guarded(pllc_sim* s, int nsteps, double tol_bound, double tol_mass, int* fallbacks)
+{
+    if (!s->cnn) {
+        pllc_set_error("pllc_sim_run_guarded", "no closure set");
+        return -1;
+    }
+    return s->prec ? run_guarded_t<float>(s, s->cf, nsteps, tol_bound, tol_mass, fallbacks)
+                   : run_guarded_t<double>(s, s->cd, nsteps, tol_bound, tol_mass, fallbacks);
 }
 
 // ------------------------------------------------------------ autotuner

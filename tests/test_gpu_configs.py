@@ -1,0 +1,163 @@
+"""The plain FP64 path at the run configurations' own sizes and states, vs
+the FP64 oracle (OpenMP on the host's cores), through the C-ABI.
+
+Bar (north_star's): after every step, relative L2 <= 1e-11 in c, in the
+velocity (u, v, w as one vector field) and in phi.  Each velocity
+component on its own and the pressure gradient (one vector field) are held
+to ten times the rounding floor the oracle itself shows for the same case
+(tests/rounding_floor.py: the oracle against itself with FFT-ordered
+(y, z) transforms, and with every state value moved by one ulp after each
+step), never below 1e-11.
+
+  * front_256x128x128: 256x128x128, R=3, Pe=2000, front width 0.02 displaced
+    by 16x16 seeded modes (seed 2), the benchmark's workload; STEPS_256
+    steps (default 30; PLLC_CFG_STEPS overrides, 1000 is the workload's
+    full run).
+  * noise_64x32x32: 64x32x32, R=2, Pe=500, width 0.05, +-0.01 noise (seed
+    1), all of its 200 steps.
+  * planes that vary in z: ny = nz = 128 and 256 with 24 x-cells and a
+    16x16-mode front, so the (y, z) transforms and the z-sweeps see 3D data.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import pllcart3d_amd as P
+from pllcart3d_amd.configs import CONFIGS, config_params, ensemble_members
+from tests.conftest import rel_l2
+from tests.rounding_floor import bar
+
+pytestmark = pytest.mark.gpu
+
+STEPS_256 = int(os.environ.get("PLLC_CFG_STEPS", "30"))
+VEC = {"c": ["C_OLD"], "velocity": ["U_OLD", "V_OLD", "W_OLD"], "phi": ["PHI"],
+       "pg": ["U_PG", "V_PG", "W_PG"]}
+COMPONENTS = ["U_OLD", "V_OLD", "W_OLD"]
+
+
+@pytest.fixture(scope="module")
+def threads(oracle_mod):
+    n = min(16, len(os.sched_getaffinity(0)))
+    oracle_mod.set_threads(n)
+    return n
+
+
+def _errors(sim, orc):
+    e = {}
+    for k, parts in VEC.items():
+        a = np.concatenate([sim.download(p).ravel() for p in parts])
+        b = np.concatenate([orc.field(p).ravel() for p in parts])
+        e[k] = rel_l2(a, b)
+    for f in COMPONENTS:
+        e[f] = rel_l2(sim.download(f), orc.field(f))
+    return e
+
+
+def _check(e, what, case):
+    line = " ".join("%s %.2e" % kv for kv in e.items())
+    print("[%s] %s" % (what, line), flush=True)
+    for q, v in e.items():
+        assert v <= bar(case, q), (what, q, v, bar(case, q))
+
+
+def _config_pair(oracle_mod, name):
+    c = CONFIGS[name]
+    prm = config_params(name)
+    c0 = P.ic_fill(prm, c["ic"], c["seed"], c["amp"])
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    sim = P.Sim(prm)
+    sim.init(c0)
+    return prm, orc, sim
+
+
+def _track(orc, sim, nsteps, tag, case, every=1):
+    worst = {}
+    t0 = time.time()
+    for n in range(nsteps):
+        _, dto = orc.run(1)
+        _, dtg = sim.run(1)
+        assert abs(dtg - dto) <= 1e-12 * dto, (n, dtg, dto)
+        if (n + 1) % every == 0 or n + 1 == nsteps:
+            e = _errors(sim, orc)
+            for k, v in e.items():
+                worst[k] = max(worst.get(k, 0.0), v)
+            _check(e, "%s step %d (%.0f s)" % (tag, n + 1, time.time() - t0), case)
+    print("[%s] worst over %d steps: %s" % (tag, nsteps, " ".join("%s %.2e" % kv for kv in worst.items())),
+          flush=True)
+    return worst
+
+
+def test_front_256x128x128_tracks_oracle(oracle_mod, threads):
+    """The benchmark's workload, plain FP64, compared after every step."""
+    print("[front_256x128x128] %d steps, oracle on %d threads" % (STEPS_256, threads), flush=True)
+    prm, orc, sim = _config_pair(oracle_mod, "front_256x128x128")
+    e = _errors(sim, orc)
+    assert max(e.values()) == 0.0, e
+    # the state is three-dimensional: w is not rounding noise around zero
+    w = orc.field("W_OLD")
+    print("[front_256x128x128] |w| max %.3e |u| max %.3e" % (np.abs(w).max(), np.abs(orc.field("U_OLD")).max()))
+    _track(orc, sim, STEPS_256, "front_256x128x128", "front_256x128x128")
+    assert np.abs(orc.field("W_OLD")).max() > 1e-3
+
+
+def test_noise_64x32x32_all_200_steps(oracle_mod, threads):
+    """The 64x32x32 case in plain FP64 for all of its 200 steps, compared
+    after every step."""
+    prm, orc, sim = _config_pair(oracle_mod, "noise_64x32x32")
+    _track(orc, sim, CONFIGS["noise_64x32x32"]["steps"], "noise_64x32x32", "noise_64x32x32")
+
+
+@pytest.mark.parametrize("n", [128, 256])
+@pytest.mark.parametrize("dct", ["fast", "gemm"])
+def test_z_varying_square_planes_track_oracle(oracle_mod, threads, dct, n, monkeypatch):
+    """ny = nz = n with a 16x16-mode front over 24 x-cells (front width 0.3,
+    so several cells lie inside it and every (y, z) plane varies in both
+    directions): the fast plane transform (n = 128 one block per plane,
+    n = 256 line passes) or, with PLLC_DCT=gemm, the dense contractions on
+    the matrix cores; whole steps against the oracle."""
+    if dct == "gemm":
+        monkeypatch.setenv("PLLC_DCT", "gemm")
+    prm = P.make_params(24, n, n, Vratio=3.0, Pe=1000.0, Inf_thick=0.3)
+    c0 = P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.05)
+    core = c0[3:-3, 1:-1, 1:-1]
+    assert np.abs(np.diff(core, axis=2)).max() > 1e-3 and np.abs(np.diff(core, axis=1)).max() > 1e-3
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    sim = P.Sim(prm)
+    sim.init(c0)
+    _track(orc, sim, 4, "%s 24x%dx%d" % (dct, n, n), "modes_24x%dx%d" % (n, n))
+    w = orc.field("W_OLD")[3:-3, 1:-1, 1:-1]
+    assert np.abs(w).max() > 1e-4
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_ensemble_members_at_workload_shapes_equal_standalone(precision):
+    """Three members of the 64-member ensemble, one of each grid shape
+    (64x32x32, 128x48x48, 192x64x64) with their drawn R and Pe, advanced
+    together, equal the same simulations run alone in every bit."""
+    members, seen = [], set()
+    for p, ic, seed, amp in ensemble_members(64):
+        key = (p.nx, p.ny, p.nz)
+        if key not in seen:
+            seen.add(key)
+            members.append((p, ic, seed, amp))
+    assert len(members) == 3
+    ens = P.Ensemble([m[0] for m in members], precision)
+    alone = []
+    for (p, ic, seed, amp), m in zip(members, ens.members):
+        c0 = P.ic_fill(p, ic, seed, amp)
+        m.init(c0)
+        s = P.Sim(p, precision)
+        s.init(c0)
+        alone.append(s)
+    ens.run(6)
+    for s in alone:
+        s.run(6)
+    for i, (m, s) in enumerate(zip(ens.members, alone)):
+        for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG"):
+            assert np.array_equal(m.download(f), s.download(f)), (precision, i, f)
+    print("[ensemble] %s members %s x 6 steps equal" % (precision, sorted(seen)), flush=True)
+    ens.close()
