@@ -311,8 +311,9 @@ const char *pllc_kernel_names(void);
  * advective term N (Hc_op, pllcart3d.c:1539-1644) on a coarse grid of
  * nx x ny x nz cells.  No counterpart in the reference.  Inputs per cell:
  * c, the cell-centred u, v, w and mu_old.cell, each (value - shift) * scale;
- * output y, the correction being out_scale * y (zero within 4 cells of the
- * inlet and the outlet).  Parameters: pllc_cnn_nparams() values, W1
+ * output y, the correction being out_scale * (y - mean y) (the mean over the
+ * corrected cells, so that it adds no mass; zero within 4 cells of the inlet
+ * and the outlet).  Parameters: pllc_cnn_nparams() values, W1
  * [27][16][16], b1 [16], W2 [27][16][16], b2 [16], W3 [27][16][16] (output
  * channel 0 used), b3 [16]; offset (oi, oj, ok) in {-1,0,1}^3 at index
  * 9 (oi+1) + 3 (oj+1) + (ok+1), zero padding outside the grid. */
@@ -337,13 +338,17 @@ int pllc_cnn_loss_grad(pllc_cnn *m, int sample, double *loss, double *grad);
  * receives each step's loss before its update */
 int pllc_cnn_train(pllc_cnn *m, int nsteps, const int *order, int norder, double lr, double *losses);
 /* the simulation adds the network's correction to N in every concentration
- * update (NULL: none); the network's grid must equal the simulation's */
-int pllc_sim_set_closure(pllc_sim *s, pllc_cnn *m);
-/* nsteps steps with the closure, checked after every pair of steps: c within
- * [-tol_bound, 1 + tol_bound] and |dt sum(correction)| <= tol_mass |sum(c)|
- * over the interior cells; a pair that fails is run again from its start
- * without the correction (plain WENO); *fallbacks counts those steps */
-int pllc_sim_run_guarded(pllc_sim *s, int nsteps, double tol_bound, double tol_mass, int *fallbacks);
+ * update (NULL: none); the network's grid must equal the simulation's; with
+ * once_per_step the correction of a step's first substage is kept for its
+ * other two */
+int pllc_sim_set_closure(pllc_sim *s, pllc_cnn *m, int once_per_step);
+/* nsteps steps with the closure, checked after every check_every steps: c
+ * within [-tol_bound, 1 + tol_bound] and |dt sum(correction)| <= tol_mass
+ * |sum(c)| over the interior cells; steps that fail are run again from their
+ * start without the correction (plain WENO); *fallbacks counts those steps;
+ * worst (nullable) receives the largest bound excess and mass ratio seen */
+int pllc_sim_run_guarded(pllc_sim *s, int nsteps, int check_every, double tol_bound, double tol_mass,
+                         int *fallbacks, double *worst);
 /* the guard's test on stats = {min c, max c, sum c, sum correction}: 1 pass */
 int pllc_closure_guard_ok(const double *stats, double dt, double tol_bound, double tol_mass);
 

@@ -98,8 +98,8 @@ def load():
     L.pllc_cnn_num_samples.argtypes = [vp]
     L.pllc_cnn_loss_grad.argtypes = [vp, C.c_int, dp, dp]
     L.pllc_cnn_train.argtypes = [vp, C.c_int, ip, C.c_int, C.c_double, dp]
-    L.pllc_sim_set_closure.argtypes = [vp, vp]
-    L.pllc_sim_run_guarded.argtypes = [vp, C.c_int, C.c_double, C.c_double, ip]
+    L.pllc_sim_set_closure.argtypes = [vp, vp, C.c_int]
+    L.pllc_sim_run_guarded.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, ip, dp]
     L.pllc_closure_guard_ok.argtypes = [dp, C.c_double, C.c_double, C.c_double]
     _lib = L
     return L

@@ -1,14 +1,14 @@
 """The learned closure of the coarse concentration update (no counterpart in
 the reference).
 
-The coarse solver misses what the fine grid resolves: box-filtering a fine
-run onto cells twice as large and evaluating the coarse advective term there
-gives N_coarse(filtered state), while the fine run's own term, filtered,
-is filter(N_fine).  Their difference is the target; a 3-layer 3x3x3
-convolutional network (libpllcart3d_amd's BF16 kernels, trained on the GPU)
-predicts it from the coarse cells' c, u, v, w and mu, and the coarse run
-adds the prediction to N in every concentration update (Hc_op's explicit
-term, pllcart3d.c:1539-1644).  A guarded run falls back to plain WENO where
+The coarse solver misses what the fine grid resolves: a fine run's state,
+box-filtered onto cells twice as large and advanced one step by the coarse
+solver, lands away from the filtered fine state one step later.  That
+difference per unit time is the target; a 3-layer 3x3x3 convolutional
+network (libpllcart3d_amd's BF16 kernels, trained on the GPU) predicts it
+from the coarse cells' c, u, v, w and mu, and the coarse run adds the
+prediction to N in every concentration update (Hc_op's explicit term,
+pllcart3d.c:1539-1644), so that a step adds dt times it to c.  A guarded run falls back to plain WENO where
 c leaves [0, 1] or the correction injects mass (pllc_sim_run_guarded).  The
 mixing-zone length, from the y-z average of c along x (the profile
 conc_average writes to cavgyz.bin, pllcart3d.c:3691-3724), judges the
@@ -234,31 +234,55 @@ def loss_mask(cp):
     return m
 
 
-def sample_from_fine(fine, fp, coarse, dt):
-    """(raw features, target) of the fine simulation's current state.
-    fine: a plain Sim on fp; coarse: a mixed Sim on coarse_params(fp) used as
-    the coarse operator (its state is overwritten)."""
+def filtered_state(fine, fp):
+    """The fine simulation's current state box-filtered onto the coarse grid:
+    c (ghosted, coarse_c0's ghost rules), the face velocities and their
+    pressure gradients (ghost rows left to the coarse ghost routines)."""
     cp = coarse_params(fp)
-    vb = fine.op("BULK_VELO")
-    fine.op("CONCENTRATION", dt, 0, vb)     # N of the current state into C_N
-    fc, fu, fv, fw, fN = (fine.download(f) for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "C_N"))
-    cc = coarse_c0(fp, fc)
-    cc[3:3 + cp.nx, 1:1 + cp.ny, 1:1 + cp.nz] = filter_cells(fc, fp)
-    cu = _full(cp, filter_faces(fu, fp, 0))
-    cv = _full(cp, filter_faces(fv, fp, 1))
-    cw = _full(cp, filter_faces(fw, fp, 2))
-    for f, a in (("C_OLD", cc), ("C_NEW", cc), ("U_NEW", cu), ("V_NEW", cv), ("W_NEW", cw)):
-        coarse.upload(f, a)
+    f = {n: fine.download(n) for n in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "U_PG", "V_PG", "W_PG")}
+    st = {"C": coarse_c0(fp, f["C_OLD"])}
+    for q, ax in (("U", 0), ("V", 1), ("W", 2)):
+        st[q] = _full(cp, filter_faces(f[q + "_OLD"], fp, ax))
+        st[q + "PG"] = _full(cp, filter_faces(f[q + "_PG"], fp, ax))
+    return st
+
+
+def load_coarse(coarse, st):
+    """A coarse simulation set to a filtered state (old = new, the
+    viscosity of c, the velocity ghosts of update_velo_boundary)."""
+    for n, q in (("C_OLD", "C"), ("C_NEW", "C"), ("U_NEW", "U"), ("V_NEW", "V"), ("W_NEW", "W"),
+                 ("U_PG", "UPG"), ("V_PG", "VPG"), ("W_PG", "WPG")):
+        coarse.upload(n, st[q])
     coarse.op("VISCOSITY")
     coarse.op("VELO_BOUNDARY")
     coarse.op("UPDATE_OLD")
-    vbc = coarse.op("BULK_VELO")
-    coarse.op("CONCENTRATION", dt, 0, vbc)
-    cN = coarse.download("C_N")[3:3 + cp.nx, 1:1 + cp.ny, 1:1 + cp.nz]
+
+
+def sample_one_step(fine, fp):
+    """(raw features, target) at the fine simulation's current state: the
+    coarse solver's one-step error.  The filtered state is advanced one step
+    on the coarse grid (mixed precision, its Timestep set to the fine step's
+    dt, which the coarse CFL limit does not bind); the fine run advances the
+    same dt; the target is (filter(c_fine after) - c_coarse after) / dt, the
+    rate the correction of N must add for the coarse step to land on the
+    filtered fine one.  Advances `fine` by one step."""
+    cp = coarse_params(fp)
+    st = filtered_state(fine, fp)
+    _, dt = fine.run(1)
+    fc1 = fine.download("C_OLD")
+    cpx = coarse_params(fp)
+    cpx.Timestep = dt
+    coarse = Sim(cpx, "mixed")
+    coarse.init(st["C"])
+    load_coarse(coarse, st)
     mu = coarse.download("MUO_CELL")
-    u, v, w = (coarse.download(f) for f in ("U_OLD", "V_OLD", "W_OLD"))
-    x = features(cc, u, v, w, mu, cp)
-    t = filter_cells(fN, fp) - cN
+    _, dtc = coarse.run(1)
+    cc1 = coarse.download("C_OLD")
+    coarse.close()
+    if abs(dtc - dt) > 1e-9 * dt:
+        raise RuntimeError("coarse step dt %.9g differs from the fine step's %.9g" % (dtc, dt))
+    x = features(st["C"], st["U"], st["V"], st["W"], mu, cp)
+    t = (filter_cells(fc1, fp) - cc1[3:3 + cp.nx, 1:1 + cp.ny, 1:1 + cp.nz]) / dt
     return x, t
 
 
@@ -270,9 +294,12 @@ def yz_average(c, p):
 
 
 def mixing_length(c, p, lo=0.05, hi=0.95):
-    """Distance along x between the first points where the y-z averaged c
-    reaches lo and hi (linear interpolation between cell centres)."""
+    """Distance along x between the first points where the y-z averaged
+    concentration of the displaced fluid reaches lo and hi (linear
+    interpolation between cell centres)."""
     prof = yz_average(c, p)
+    if prof[0] > prof[-1]:
+        prof = 1.0 - prof   # the displacing fluid (c = 1 here) enters at X1
     dx = (p.X2 - p.X1) / p.nx
     xc = p.X1 + (np.arange(p.nx) + 0.5) * dx
 
@@ -288,19 +315,23 @@ def mixing_length(c, p, lo=0.05, hi=0.95):
 
 
 
-def set_closure(sim, net):
-    """The simulation adds net's correction to N from now on (None: off)."""
-    if load().pllc_sim_set_closure(sim.h, net.h if net is not None else None) != 0:
+def set_closure(sim, net, once_per_step=False):
+    """The simulation adds net's correction to N from now on (None: off);
+    once_per_step keeps the correction of a step's first substage for its
+    other two."""
+    if load().pllc_sim_set_closure(sim.h, net.h if net is not None else None, int(once_per_step)) != 0:
         raise RuntimeError("pllc_sim_set_closure: " + last_error())
 
 
-def run_guarded(sim, nsteps, tol_bound=1e-2, tol_mass=1e-4):
-    """nsteps guarded steps with the simulation's closure; returns the
-    number of steps that fell back to plain WENO."""
+def run_guarded(sim, nsteps, check_every=2, tol_bound=1e-2, tol_mass=1e-4):
+    """nsteps guarded steps with the simulation's closure; returns the number
+    of steps that fell back to plain WENO and the largest bound excess and
+    mass ratio seen."""
     fb = C.c_int(0)
-    if load().pllc_sim_run_guarded(sim.h, nsteps, tol_bound, tol_mass, C.byref(fb)) != 0:
+    worst = np.zeros(2)
+    if load().pllc_sim_run_guarded(sim.h, nsteps, check_every, tol_bound, tol_mass, C.byref(fb), _dp(worst)) != 0:
         raise RuntimeError("pllc_sim_run_guarded: " + last_error())
-    return fb.value
+    return fb.value, worst
 
 
 def sim_time(sim):
@@ -328,20 +359,17 @@ def collect(fine_shape, runs, steps, every, warmup, log=print):
         c0 = ic_fill(fp, IC_MULTIMODE, seed, 0.01)
         fine = Sim(fp, "plain")
         fine.init(c0)
-        coarse = Sim(coarse_params(fp), "mixed")
-        coarse.init(coarse_c0(fp, c0))
         done = 0
         for target_step in range(warmup, steps + 1, every):
             fine.run(target_step - done)
-            done = target_step
-            x, t = sample_from_fine(fine, fp, coarse, 1e-3)
+            x, t = sample_one_step(fine, fp)
+            done = target_step + 1
             X.append(x.astype(np.float32))
             T.append(t.astype(np.float32))
         rms = float(np.sqrt(np.mean(T[-1].astype(np.float64) ** 2)))
         log("[closure data] R=%.2f Pe=%.0f seed %d: samples up to step %d, last target rms %.3e"
             % (R, Pe, seed, done, rms))
         fine.close()
-        coarse.close()
     return X, T
 
 
@@ -367,3 +395,78 @@ def mixing_length_at(sim, p, t_end):
         if t1 >= t_end:
             return L0 + (L1 - L0) * (t_end - t0) / (t1 - t0)
         t0, L0 = t1, L1
+
+
+def train(X, T, mask, shape, steps=3000, seed=1, log=print):
+    """A network for a coarse grid of `shape` trained with Adam on the
+    samples (X, T): normalisation from the samples, then `steps` steps at
+    learning rate 1e-3 and steps // 3 at 3e-4, samples in seeded random
+    order.  Returns (network, (shift, scale, out_scale), losses)."""
+    shift, scale, osc = normalisation(X, T, mask)
+    net = Network(*shape)
+    net.set_norm(shift, scale, osc)
+    idx = [net.add_sample((x - shift) * scale, t / osc, mask) for x, t in zip(X, T)]
+    net.set_params(flat_from_dict(init_params(seed)))
+    rng = np.random.default_rng(seed)
+    losses = []
+    for n, lr in ((steps, 1e-3), (steps // 3, 3e-4)):
+        order = rng.permutation(np.repeat(idx, 1 + n // len(idx)))[:n]
+        losses.append(net.train(n, order, lr))
+    losses = np.concatenate(losses)
+    log("[closure train] %d steps: loss %.4f -> %.4f" % (len(losses), losses[:50].mean(), losses[-50:].mean()))
+    return net, (shift, scale, osc), losses
+
+
+def validation_gain(net, norm, X, T, mask):
+    """Mean masked loss of the trained network and of no correction (a zero
+    output) on held-out samples; the gain is 1 - trained / none."""
+    shift, scale, osc = norm
+    idx = [net.add_sample((x - shift) * scale, t / osc, mask) for x, t in zip(X, T)]
+    trained = float(np.mean([net.loss(i) for i in idx]))
+    m = mask.astype(bool)
+    none = float(np.mean([np.mean((t[m] / osc) ** 2) for t in T]))
+    return trained, none, 1.0 - trained / none
+
+
+def coarse_run(fp, c0_fine, t_end, net=None, timestep=None, guard_every=10, tol_bound=1e-2, tol_mass=1e-4,
+               chunk=20):
+    """The coarse (2x) mixed-precision run of the fine run (fp, c0_fine) up to
+    t_end, with the closure `net` (guarded, correction once per step) or
+    without it; returns the mixing-zone length at t_end, the wall time (s),
+    the steps, the fallback steps and the guard's worst bound excess and mass
+    ratio."""
+    cp = coarse_params(fp)
+    if timestep is not None:
+        cp.Timestep = timestep
+
+    def make():
+        s = Sim(cp, "mixed")
+        s.init(coarse_c0(fp, c0_fine))
+        if net is not None:
+            set_closure(s, net, True)
+        return s
+
+    warm = make()   # first use: graph capture and kernel loading
+    warm.run(2)
+    warm.synchronize()
+    warm.close()
+    s = make()
+    t0 = time.perf_counter()
+    n = nfb = 0
+    worst = np.zeros(2)
+    while True:
+        t, dt = s.run(0)
+        if t + 3 * max(dt, 1e-6) >= t_end:
+            break
+        if net is not None:
+            f, w = run_guarded(s, chunk, guard_every, tol_bound, tol_mass)
+            nfb += f
+            worst = np.maximum(worst, w)
+        else:
+            s.run(chunk)
+        n += chunk
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    L = mixing_length_at(s, cp, t_end)
+    s.close()
+    return L, wall, n, nfb, worst

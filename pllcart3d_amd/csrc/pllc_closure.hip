@@ -8,9 +8,10 @@
 //   y = conv(a2, W3) + b3 (one output channel),
 // zero padding outside the grid.  x holds c, the cell-centred u, v, w and
 // mu_old.cell, each shifted and scaled by fixed constants (channels 5..15
-// are zero).  The correction added to N is out_scale * y, set to zero within
-// XMARGIN cells of the inlet and outlet (where the ENO boundary rule and the
-// outflow condition replace the WENO stencil).
+// are zero).  The correction added to N is out_scale * (y - mean y), the mean
+// taken over the corrected cells so that the correction adds no mass, and it
+// is zero within XMARGIN cells of the inlet and outlet (where the ENO
+// boundary rule and the outflow condition replace the WENO stencil).
 //
 // Arithmetic: BF16 operands on the matrix cores (v_mfma_f32_16x16x16_bf16),
 // FP32 accumulation, activations and gradients stored in BF16, weights kept
@@ -71,7 +72,9 @@ enum { EPI_TANH = 0, EPI_OUT0 = 1, EPI_DTANH = 2 };
 // channels 4 (l / 16) .. 4 (l / 16) + 3; the B operand is W[o], packed on
 // the device so that lane l reads its four values W[o][4 (l / 16) + q][l % 16]
 // as 8 contiguous bytes: Wp[(o * 64 + l) * 4 + q].  The accumulator gives
-// lane l the cells 4 (l / 16) + q, channel l % 16.
+// lane l the cells 4 (l / 16) + q, channel l % 16.  Each wave keeps the 27
+// B operands in registers and takes segments in turn (grid-stride); per
+// segment all 27 A loads are issued before the 27 MFMAs.
 __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __restrict__ in,
                                               const unsigned short* __restrict__ Wp, const float* __restrict__ bias,
                                               int epi, const unsigned short* __restrict__ aprev,
@@ -79,37 +82,43 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
 {
     const int lane = threadIdx.x & 63;
     const int segk = (g.nz + 15) >> 4;
-    const long seg = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (seg >= (long)g.nx * g.ny * segk) return;   // whole waves
-    const int k0 = (int)(seg % segk) * 16;
-    const long row = seg / segk;
-    const int j = (int)(row % g.ny), i = (int)(row / g.ny);
+    const long nseg = (long)g.nx * g.ny * segk;
+    const long nwave = (long)gridDim.x * 4;
     const int m = lane & 15, kq = lane >> 4;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const bf16x4 zero = {0, 0, 0, 0};
-#pragma unroll 3
-    for (int o = 0; o < NOFF; o++) {
-        const int ii = i + o / 9 - 1, jj = j + (o / 3) % 3 - 1, kk = k0 + m + o % 3 - 1;
-        bf16x4 a = zero;
-        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
-            a = *(const bf16x4*)(in + vox(g, ii, jj, kk) * CH + 4 * kq);
-        const bf16x4 b = *(const bf16x4*)(Wp + ((long)o * 64 + lane) * 4);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    const float bc = bias ? bias[m] : 0.f;
+    bf16x4 b[NOFF];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int k = k0 + 4 * kq + q;
-        if (k >= g.nz) continue;
-        const long v = vox(g, i, j, k);
-        const float z = acc[q] + bc;
-        if (epi == EPI_TANH) {
-            out_bf[v * CH + m] = f2bf(tanhf(z));
-        } else if (epi == EPI_OUT0) {
-            if (m == 0) out_f[v] = z;
-        } else {
-            const float a = bf2f(aprev[v * CH + m]);
-            out_bf[v * CH + m] = f2bf(z * (1.f - a * a));
+    for (int o = 0; o < NOFF; o++) b[o] = *(const bf16x4*)(Wp + ((long)o * 64 + lane) * 4);
+    const float bc = bias ? bias[m] : 0.f;
+    for (long seg = (long)blockIdx.x * 4 + (threadIdx.x >> 6); seg < nseg; seg += nwave) {
+        const int k0 = (int)(seg % segk) * 16;
+        const long row = seg / segk;
+        const int j = (int)(row % g.ny), i = (int)(row / g.ny);
+        bf16x4 a[NOFF];
+#pragma unroll
+        for (int o = 0; o < NOFF; o++) {
+            const int ii = i + o / 9 - 1, jj = j + (o / 3) % 3 - 1, kk = k0 + m + o % 3 - 1;
+            a[o] = zero;
+            if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
+                a[o] = *(const bf16x4*)(in + vox(g, ii, jj, kk) * CH + 4 * kq);
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int o = 0; o < NOFF; o++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[o], b[o], acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + 4 * kq + q;
+            if (k >= g.nz) continue;
+            const long v = vox(g, i, j, k);
+            const float z = acc[q] + bc;
+            if (epi == EPI_TANH) {
+                out_bf[v * CH + m] = f2bf(tanhf(z));
+            } else if (epi == EPI_OUT0) {
+                if (m == 0) out_f[v] = z;
+            } else {
+                const float av = bf2f(aprev[v * CH + m]);
+                out_bf[v * CH + m] = f2bf(z * (1.f - av * av));
+            }
         }
     }
 }
@@ -265,20 +274,52 @@ __global__ __launch_bounds__(256) void k_gather(pllc::Dims d, const T* __restric
     for (int q = NIN; q < CH; q++) dst[q] = 0;
 }
 
-// The correction field on the simulation's ghosted layout:
-// ncorr[i][j][k] = out_scale * y[p] on interior cells at least XMARGIN cells
-// from the inlet and the outlet, 0 on every other interior cell.
-template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(pllc::Dims d, const float* __restrict__ y, float out_scale,
-                                                 T* __restrict__ ncorr)
+// Sum of y over the corrected cells (at least XMARGIN cells from the inlet
+// and the outlet), per block into part[block] (FP64)
+__global__ __launch_bounds__(256) void k_ysum(G3 g, const float* __restrict__ y, double* __restrict__ part)
 {
+    __shared__ double s[256];
+    const long n = (long)g.nx * g.ny * g.nz, plane = (long)g.ny * g.nz;
+    double sm = 0.0;
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
+        const long i = p / plane;
+        if (i >= XMARGIN && i < g.nx - XMARGIN) sm += (double)y[p];
+    }
+    s[threadIdx.x] = sm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// The correction field on the simulation's ghosted layout:
+// ncorr[i][j][k] = out_scale * (y[p] - mean) on interior cells at least
+// XMARGIN cells from the inlet and the outlet, 0 on every other interior
+// cell; mean is the average of y over those cells (from nb block sums), so
+// the correction adds no mass.  out_scale is nrm[2 NIN] (device memory, so a
+// captured graph sees later changes).
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(pllc::Dims d, const float* __restrict__ y,
+                                                 const float* __restrict__ nrm, const double* __restrict__ part,
+                                                 int nb, T* __restrict__ ncorr)
+{
+    __shared__ double smean;
+    if (threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int b = 0; b < nb; b++) sm += part[b];
+        const long cells = (long)(d.nx - 2 * XMARGIN) * d.ny * d.nz;
+        smean = cells > 0 ? sm / (double)cells : 0.0;
+    }
+    __syncthreads();
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     const long n = (long)d.nx * d.ny * d.nz;
     if (p >= n) return;
     const int k = (int)(p % d.nz), j = (int)(p / d.nz % d.ny), i = (int)(p / ((long)d.nz * d.ny));
     const long o = (long)(i + 3) * d.sx + (long)(j + 1) * d.sy + (k + 1);
     const bool in = i >= XMARGIN && i < d.nx - XMARGIN;
-    ncorr[o] = in ? (T)(out_scale * y[p]) : (T)0;
+    ncorr[o] = in ? (T)(nrm[2 * NIN] * (float)((double)y[p] - smean)) : (T)0;
 }
 
 // FP32 [vox][NIN] host inputs (already shifted and scaled) -> BF16 [vox][16]
@@ -375,9 +416,9 @@ struct pllc_cnn {
     float* y = nullptr;
     float* part = nullptr;         // weight-gradient partials
     double* lpart = nullptr;
-    float* nrm = nullptr;          // 2 * NIN input shifts and scales
-    float hnrm[2 * NIN] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1};
-    float out_scale = 1.f;
+    float* nrm = nullptr;          // 2 * NIN input shifts and scales, then out_scale
+    float hnrm[2 * NIN + 1] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
+    double* ypart = nullptr;       // block sums of y (the mean the correction removes)
     long adam_t = 0;
     std::vector<pllc::Sample> samples;
     std::vector<void*> owned;
@@ -387,6 +428,7 @@ namespace {
 
 constexpr int WG_BLOCKS = 256;
 constexpr int LOSS_BLOCKS = 256;
+constexpr int YSUM_BLOCKS = 64;
 
 bool dmalloc(pllc_cnn* m, void** p, size_t bytes, const char* W)
 {
@@ -395,7 +437,13 @@ bool dmalloc(pllc_cnn* m, void** p, size_t bytes, const char* W)
     return true;
 }
 
-int conv_blocks(const G3& g) { return (int)(((long)g.nx * g.ny * ((g.nz + 15) / 16) + 3) / 4); }
+// blocks of the grid-stride convolution: one segment per wave up to 2048
+// blocks (8 per CU), more segments per wave beyond
+int conv_blocks(const G3& g)
+{
+    const long b = ((long)g.nx * g.ny * ((g.nz + 15) / 16) + 3) / 4;
+    return (int)std::min<long>(b, 2048);
+}
 
 void pack_all(pllc_cnn* m)
 {
@@ -480,7 +528,8 @@ pllc_cnn* pllc_cnn_create(int nx, int ny, int nz)
          dmalloc(m, (void**)&m->dz, act, W) && dmalloc(m, (void**)&m->dz2, act, W) &&
          dmalloc(m, (void**)&m->y, (size_t)m->n * 4, W) &&
          dmalloc(m, (void**)&m->part, (size_t)WG_BLOCKS * (NOFF * 256 + CH) * 4, W) &&
-         dmalloc(m, (void**)&m->lpart, LOSS_BLOCKS * 8, W) && dmalloc(m, (void**)&m->nrm, 2 * NIN * 4, W);
+         dmalloc(m, (void**)&m->lpart, LOSS_BLOCKS * 8, W) && dmalloc(m, (void**)&m->nrm, (2 * NIN + 1) * 4, W) &&
+         dmalloc(m, (void**)&m->ypart, YSUM_BLOCKS * 8, W);
     for (int l = 0; ok && l < 3; l++) {
         ok = dmalloc(m, (void**)&m->Wf[l], WSZ * 2, W);
         if (ok && l > 0) ok = dmalloc(m, (void**)&m->Wb[l], WSZ * 2, W);
@@ -560,7 +609,7 @@ int pllc_cnn_set_norm(pllc_cnn* m, const double* shift, const double* scale, dou
         m->hnrm[q] = (float)shift[q];
         m->hnrm[NIN + q] = (float)scale[q];
     }
-    m->out_scale = (float)out_scale;
+    m->hnrm[2 * NIN] = (float)out_scale;
     if (!pllc_hip_ok("pllc_cnn_set_norm", hipMemcpyAsync(m->nrm, m->hnrm, sizeof m->hnrm, hipMemcpyHostToDevice, m->st)))
         return -1;
     return pllc_hip_ok("pllc_cnn_set_norm", hipStreamSynchronize(m->st)) ? 0 : -1;
@@ -688,7 +737,8 @@ void pllc_cnn_correct(pllc_cnn* m, const pllc::Dims& d, const T* c, const T* u, 
     m->st = st;
     k_gather<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, c, u, v, w, mu, m->nrm, m->x);
     forward(m);
-    k_scatter<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, m->y, m->out_scale, ncorr);
+    k_ysum<<<YSUM_BLOCKS, 256, 0, st>>>(m->g, m->y, m->ypart);
+    k_scatter<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, m->y, m->nrm, m->ypart, YSUM_BLOCKS, ncorr);
     m->st = keep;
 }
 template void pllc_cnn_correct<float>(pllc_cnn*, const pllc::Dims&, const float*, const float*, const float*,

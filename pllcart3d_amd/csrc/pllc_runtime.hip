@@ -372,6 +372,7 @@ struct pllc_sim {
     // (null: none); snapshot buffers of the guarded run (NSTATE fields, the
     // device scalars and tval) and the guard's reduction scratch
     pllc_cnn* cnn = nullptr;
+    bool cnn_once = false;        // the correction of substage 0 held for the step's three
     std::vector<void*> snap;
     double* gscratch = nullptr;
 };
@@ -729,7 +730,7 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::bulk_stale(c, rk, g + r, true, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         // the learned closure's correction of N from the substage's start
         // state (c, u, v, w and mu_old.cell)
-        if (s->cnn && c.ncorr)
+        if (s->cnn && c.ncorr && (rk == 0 || !s->cnn_once))
             pllc_cnn_correct<T>(s->cnn, c.d, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.f[MUO_CELL], c.ncorr,
                                 c.st);
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
@@ -853,11 +854,13 @@ extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* 
 //
 // A simulation with a closure adds the network's correction to the explicit
 // advective term N of every concentration update (no counterpart in the
-// reference).  The guarded run checks, after every pair of steps, that c
-// stays within [-tol_bound, 1 + tol_bound] and that the mass the correction
-// injects in a step, |dt sum(ncorr)|, stays within tol_mass of the mass
-// sum(c) over the interior cells; when either is violated the pair is run
-// again from its start without the correction (plain WENO).
+// reference); with once_per_step the correction of a step's first substage
+// is kept for its other two.  The guarded run checks, after every
+// check_every steps, that c stays within [-tol_bound, 1 + tol_bound] and that
+// the mass the correction injects in a step, |dt sum(ncorr)|, stays within
+// tol_mass of the mass sum(c) over the interior cells; when either is
+// violated those steps are run again from their start without the
+// correction (plain WENO).
 
 extern "C" int pllc_closure_guard_ok(const double* stats, double dt, double tol_bound, double tol_mass)
 {
@@ -869,7 +872,7 @@ extern "C" int pllc_closure_guard_ok(const double* stats, double dt, double tol_
 }
 
 template <typename T>
-static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m)
+static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m, int once_per_step)
 {
     const char* W = "pllc_sim_set_closure";
     graph_drop(s);
@@ -890,16 +893,19 @@ static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m)
         c.ncorr = (T*)p;
     }
     s->cnn = m;
+    s->cnn_once = once_per_step != 0;
     return 0;
 }
 
-extern "C" int pllc_sim_set_closure(pllc_sim* s, pllc_cnn* m)
+extern "C" int pllc_sim_set_closure(pllc_sim* s, pllc_cnn* m, int once_per_step)
 {
-    return s->prec ? set_closure_t<float>(s, s->cf, m) : set_closure_t<double>(s, s->cd, m);
+    return s->prec ? set_closure_t<float>(s, s->cf, m, once_per_step)
+                   : set_closure_t<double>(s, s->cd, m, once_per_step);
 }
 
 template <typename T>
-static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, double tol_bound, double tol_mass, int* fallbacks)
+static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, int check_every, double tol_bound, double tol_mass,
+                         int* fallbacks, double* worst)
 {
     const char* W = "pllc_sim_run_guarded";
     const size_t n = npts(s);
@@ -915,8 +921,10 @@ static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, double tol_bound, d
         s->owned.push_back(s->gscratch);
     }
     int fb = 0;
+    double wb = 0.0, wm = 0.0;
+    const int every = check_every < 1 ? 1 : check_every;
     while (nsteps > 0) {
-        const int k = nsteps >= 2 ? 2 : 1;
+        const int k = nsteps >= every ? every : nsteps;
         // snapshot: the pointer table, the fields, the device scalars, tval
         T* tab[NFIELDS];
         std::copy(c.f, c.f + NFIELDS, tab);
@@ -932,6 +940,8 @@ static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, double tol_bound, d
         double st[4], dt = 0.0;
         if (!pllc_guard_stats<T>(c.d, c.f[C_OLD], c.ncorr, s->gscratch, st, s->st, W)) return -1;
         if (!pllc_hip_ok(W, hipMemcpy(&dt, c.scal + S_DT, 8, hipMemcpyDeviceToHost))) return -1;
+        wb = std::max(wb, std::max(-st[0], st[1] - 1.0));
+        wm = std::max(wm, fabs(dt * st[3]) / std::max(fabs(st[2]), 1e-300));
         if (!pllc_closure_guard_ok(st, dt, tol_bound, tol_mass)) {
             // restore and run the pair with plain WENO
             std::copy(tab, tab + NFIELDS, c.f);
@@ -955,17 +965,22 @@ static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, double tol_bound, d
         nsteps -= k;
     }
     if (fallbacks) *fallbacks = fb;
+    if (worst) {
+        worst[0] = wb;
+        worst[1] = wm;
+    }
     return finish(s, W);
 }
 
-extern "C" int pllc_sim_run_guarded(pllc_sim* s, int nsteps, double tol_bound, double tol_mass, int* fallbacks)
+extern "C" int pllc_sim_run_guarded(pllc_sim* s, int nsteps, int check_every, double tol_bound, double tol_mass,
+                                    int* fallbacks, double* worst)
 {
     if (!s->cnn) {
         pllc_set_error("pllc_sim_run_guarded", "no closure set");
         return -1;
     }
-    return s->prec ? run_guarded_t<float>(s, s->cf, nsteps, tol_bound, tol_mass, fallbacks)
-                   : run_guarded_t<double>(s, s->cd, nsteps, tol_bound, tol_mass, fallbacks);
+    return s->prec ? run_guarded_t<float>(s, s->cf, nsteps, check_every, tol_bound, tol_mass, fallbacks, worst)
+                   : run_guarded_t<double>(s, s->cd, nsteps, check_every, tol_bound, tol_mass, fallbacks, worst);
 }
 
 // ------------------------------------------------------------ autotuner
@@ -1055,6 +1070,14 @@ struct pllc_ensemble {
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<hipEvent_t> done;
+    // one graph holding two fused steps of every member (each member a
+    // branch forked from and joined to the ensemble stream), replayed while
+    // every member's pointer table equals the captured one; PLLC_ENSEMBLE_GRAPH=0
+    // replays the members' own graphs round-robin instead
+    bool one_graph = true;
+    hipGraphExec_t gx = nullptr;
+    std::vector<std::vector<void*>> gtab;
+    hipEvent_t fork = nullptr, after = nullptr;
 };
 
 extern "C" void pllc_ensemble_destroy(pllc_ensemble* e)
@@ -1062,6 +1085,9 @@ extern "C" void pllc_ensemble_destroy(pllc_ensemble* e)
     if (!e) return;
     for (pllc_sim* s : e->m) pllc_sim_destroy(s);
     for (hipEvent_t v : e->done) (void)hipEventDestroy(v);
+    if (e->gx) (void)hipGraphExecDestroy(e->gx);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->after) (void)hipEventDestroy(e->after);
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->st) (void)hipStreamDestroy(e->st);
@@ -1074,7 +1100,11 @@ extern "C" pllc_ensemble* pllc_ensemble_create(const pllc_params* p, int n, int 
     if (!p || n < 1) { pllc_set_error(W, "no members"); return nullptr; }
     pllc_ensemble* e = new pllc_ensemble();
     bool ok = pllc_hip_ok(W, hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking)) &&
-              pllc_hip_ok(W, hipEventCreate(&e->e0)) && pllc_hip_ok(W, hipEventCreate(&e->e1));
+              pllc_hip_ok(W, hipEventCreate(&e->e0)) && pllc_hip_ok(W, hipEventCreate(&e->e1)) &&
+              pllc_hip_ok(W, hipEventCreateWithFlags(&e->fork, hipEventDisableTiming)) &&
+              pllc_hip_ok(W, hipEventCreateWithFlags(&e->after, hipEventDisableTiming));
+    const char* og = getenv("PLLC_ENSEMBLE_GRAPH");
+    e->one_graph = !(og && strcmp(og, "0") == 0);
     for (int i = 0; ok && i < n; i++) {
         pllc_sim* s = pllc_sim_create2(&p[i], precision);
         hipEvent_t v = nullptr;
@@ -1094,11 +1124,76 @@ extern "C" pllc_sim* pllc_ensemble_member(pllc_ensemble* e, int i)
     return e->m[i];
 }
 
+// two fused steps of every member through the ensemble's one graph; false
+// (nothing launched) when that is not possible
+static bool ensemble_pair_graph(pllc_ensemble* e)
+{
+    if (!e->one_graph) return false;
+    std::vector<std::vector<void*>> tab;
+    for (pllc_sim* s : e->m) {
+        if (!s->use_graph || s->copy_next || s->cd.hook || s->cf.hook || s->cnn) return false;
+        tab.push_back(s->prec ? ptr_table(s->cf) : ptr_table(s->cd));
+    }
+    if (e->gx && tab != e->gtab) {
+        (void)hipGraphExecDestroy(e->gx);
+        e->gx = nullptr;
+    }
+    if (!e->gx) {
+        std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
+        if (hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            e->one_graph = false;
+            return false;
+        }
+        bool ok = hipEventRecord(e->fork, e->st) == hipSuccess;
+        for (size_t i = 0; ok && i < e->m.size(); i++) {
+            pllc_sim* s = e->m[i];
+            ok = hipStreamWaitEvent(s->st, e->fork, 0) == hipSuccess;
+            if (!ok) break;
+            step_any(s);
+            step_any(s);   // six swaps: the member's table is back to tab[i]
+            ok = hipEventRecord(e->done[i], s->st) == hipSuccess &&
+                 hipStreamWaitEvent(e->st, e->done[i], 0) == hipSuccess;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->st, &g);
+        (void)hipGetLastError();
+        hipGraphExec_t gx = nullptr;
+        if (ok && ec == hipSuccess && g && hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) == hipSuccess) {
+            e->gx = gx;
+            e->gtab = tab;
+        } else {
+            (void)hipGetLastError();
+            e->one_graph = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (!e->gx) return false;
+    }
+    // the members' streams start after whatever they have queued
+    for (size_t i = 0; i < e->m.size(); i++) {
+        if (hipEventRecord(e->done[i], e->m[i]->st) != hipSuccess ||
+            hipStreamWaitEvent(e->st, e->done[i], 0) != hipSuccess)
+            return false;
+    }
+    if (hipGraphLaunch(e->gx, e->st) != hipSuccess) {
+        (void)hipGetLastError();
+        e->one_graph = false;
+        return false;
+    }
+    // later work on a member's own stream waits for the graph
+    if (hipEventRecord(e->after, e->st) != hipSuccess) return false;
+    for (pllc_sim* s : e->m) {
+        if (hipStreamWaitEvent(s->st, e->after, 0) != hipSuccess) return false;
+        s->new_stale = true;
+    }
+    return true;
+}
+
 static void ensemble_steps(pllc_ensemble* e, int nsteps)
 {
     for (int done = 0; done < nsteps;) {
         const int k = nsteps - done >= 2 ? 2 : 1;
-        for (pllc_sim* s : e->m) steps(s, k);
+        if (!(k == 2 && ensemble_pair_graph(e)))
+            for (pllc_sim* s : e->m) steps(s, k);
         done += k;
     }
 }

@@ -197,3 +197,138 @@ def test_spline_entry_point_matches_oracle(oracle_mod):
         ref = np.zeros(N)
         O.orc_spline(N, dx, oracle_mod.dp(x), oracle_mod.dp(xx), oracle_mod.dp(y), oracle_mod.dp(ref))
         assert np.array_equal(got, ref), (shift, np.abs(got - ref).max())
+
+
+def find_max_record(u, v, w, cn, co, cx):
+    """find_max (pllcart3d.c:3539-3603) restated on whole arrays: per field
+    the value of largest magnitude, the last index in i-j-k order winning a
+    tie (the `<=` comparison from a running maximum 0); the first cell in
+    i-j-k order with a NaN in c_old ('a'), c_new ('b') or N ('c')."""
+    out = []
+    for f in (u, v, w, cn):
+        flat = f.ravel()
+        with np.errstate(invalid="ignore"):
+            m = np.nanmax(np.abs(flat))   # a NaN never passes the comparison
+            idx = int(np.nonzero(np.abs(flat) == m)[0][-1])
+        out.append((np.unravel_index(idx, f.shape), flat[idx]))
+    bad = np.isnan(co) | np.isnan(cn) | np.isnan(cx)
+    nan = None
+    if bad.any():
+        p = int(np.nonzero(bad.ravel())[0][0])
+        tag = "a" if np.isnan(co.ravel()[p]) else ("b" if np.isnan(cn.ravel()[p]) else "c")
+        nan = (tag, np.unravel_index(p, co.shape))
+    return out, nan
+
+
+def _max_blocks(text):
+    """The (i, j, k, value) rows of every `max` block of an Outscreen."""
+    lines = text.splitlines()
+    blocks = []
+    for n, ln in enumerate(lines):
+        if ln == "max":
+            rows = []
+            for r in lines[n + 1:n + 5]:
+                i, j, k, val = r.split()
+                rows.append(((int(i), int(j), int(k)), float(val)))
+            blocks.append(rows)
+    return blocks
+
+
+MIDFIELD_RECORDS = ["C_OLD", "C_NM1", "U_OLD", "U_NEW", "U_NM1", "U_PG", "V_OLD", "V_NEW", "V_NM1", "V_PG",
+                    "W_OLD", "W_NEW", "W_NM1", "W_PG"]
+
+
+def write_restart(d, orc, tstep, tval):
+    """middata and midfield000.bin of the oracle's state in the reference's
+    formats (midfield, pllcart3d.c:3121-3178: per (i, j) the 14 rows of
+    nz_max values in MIDFIELD_RECORDS order)."""
+    with open(d / "middata", "w") as f:
+        f.write("%d\n%d\n%f\n%f\n%f\n%f\n" % (tstep, 1, tval, 100.0, 100.0, 100.0))
+    rec = np.stack([orc.field(n) for n in MIDFIELD_RECORDS], axis=2)   # [i][j][14][k]
+    rec.tofile(d / "midfield000.bin")
+
+
+def test_driver_diagnostic_values(tmp_path, oracle_mod):
+    """find_max's printed indices and values after every step, and
+    find_wmax's records (written with Initial = 0 and Wavpert = 1), from a
+    restart of a z-varying state (a 16x16-mode front after 2 oracle steps),
+    against the oracle restarted the same way: find_max restated on the
+    oracle's arrays, find_wmax = max(0, largest interior w) as float32."""
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
+    rst = oracle_mod.Oracle(prm)
+    rst.init(P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.05))
+    t, _ = rst.run(2)
+    t = float("%f" % t)   # middata keeps tval to 6 decimals
+    write_restart(tmp_path, rst, 2, t)
+    nsteps = 3
+    _run(tmp_path, nsteps, {"Initial": 0, "Wavpert": 1, "Niter_print": 1, "Out_mid": 100.0,
+                            "Out_inter": 100.0})
+    blocks = _max_blocks((tmp_path / "Outscreen").read_text())
+    wrec = np.fromfile(tmp_path / "wmax.bin", dtype=np.float32).reshape(-1, 2)
+    assert len(blocks) == nsteps and len(wrec) == nsteps, (len(blocks), len(wrec))
+    # the restarted run starts with every array midfield does not save zero
+    for f in ("C_NEW", "C_N", "U_N", "V_N", "W_N", "PHI", "DIV"):
+        rst.field(f)[...] = 0.0
+    for s in range(nsteps):
+        t, _ = rst.run(1, t)
+        rec, nan = find_max_record(*(rst.field(f) for f in ("U_NEW", "V_NEW", "W_NEW", "C_NEW", "C_OLD", "C_N")))
+        assert nan is None
+        fields = [rst.field(f) for f in ("U_NEW", "V_NEW", "W_NEW", "C_NEW")]
+        for (ix, val), (gix, gval), f in zip(rec, blocks[s], fields):
+            # the index is the oracle's unless the two differ only by a near
+            # tie (values within 1e-12 of each other, which rounding may
+            # order either way)
+            if tuple(int(x) for x in ix) != gix:
+                assert abs(abs(f[gix]) - abs(val)) <= 1e-12 * abs(val), (s, ix, gix, f[gix], val)
+            assert abs(gval - val) <= 5e-7 + 1e-12 * abs(val), (s, gval, val)
+        w = rst.field("W_NEW")[3:-3, 1:-1, 1:-1]
+        wm = max(0.0, float(w.max()))
+        print("[diag] step %d max rows %s wmax %.6e (file %.6e) t %.6f" % (s + 1, blocks[s], wm, wrec[s, 0], t),
+              flush=True)
+        assert wm > 1e-6
+        assert abs(float(wrec[s, 0]) - wm) <= 1.2e-7 * wm + 1e-12, (s, wrec[s, 0], wm)
+        assert abs(float(wrec[s, 1]) - t) <= 1.2e-7 * t, (s, wrec[s, 1], t)
+
+
+def test_driver_find_max_nan_guard(tmp_path, oracle_mod):
+    """A restart file with a NaN in one cell of c_old: after the first step
+    the NaN has spread along the cell's line solves; find_max names the
+    first NaN cell in i-j-k order ('a': c_old) as the oracle's state does,
+    and the driver exits with status 1 (pllcart3d.c:3568-3588)."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    _run(a, 2, {"Out_mid": 0.003})
+    (b / "middata").write_bytes((a / "middata").read_bytes())
+    # midfield (pllcart3d.c:3121-3178) writes, for every (i, j), 14 rows of
+    # nz_max values, c_old first
+    raw = np.fromfile(a / "midfield000.bin").reshape(30, 14, 14, 12).copy()
+    raw[10, 5, 0, 4] = np.nan
+    raw.tofile(b / "midfield000.bin")
+    vals = dict(PARAMS)
+    vals.update({"Initial": 0, "Niter_print": 1, "Out_mid": 100.0})
+    with open(b / "param.input", "w") as f:
+        for name, _ in PARAMS:
+            f.write("//%s\n%r\n" % (name, vals[name]))
+    exe = os.path.join(os.path.dirname(P.__file__), "run3_amd")
+    r = subprocess.run([exe, "2"], cwd=b, capture_output=True, text=True, timeout=120)
+    text = (b / "Outscreen").read_text()
+    line = [ln for ln in text.splitlines() if ln.startswith("nan in")]
+    print("[nan guard] rc %d: %s" % (r.returncode, line), flush=True)
+    assert r.returncode == 1 and len(line) == 1
+    tag, i, j, k = line[0].split()[2:6]
+    assert tag == "a"
+    # the oracle from the same restart state, one step
+    prm = P.make_params(24, 12, 10, Vratio=3.0, Pe=300.0, Inf_thick=0.1)
+    rst = oracle_mod.Oracle(prm)
+    rst.init()
+    t, _ = rst.run(2)
+    t = float("%f" % t)   # middata keeps tval to 6 decimals
+    for f in ("C_NEW", "C_N", "U_N", "V_N", "W_N", "PHI", "DIV"):
+        rst.field(f)[...] = 0.0
+    rst.field("C_OLD")[10, 5, 4] = np.nan
+    with np.errstate(invalid="ignore"):
+        rst.run(1, t)
+        _, nan = find_max_record(*(rst.field(f) for f in ("U_NEW", "V_NEW", "W_NEW", "C_NEW", "C_OLD", "C_N")))
+    assert nan is not None and nan[0] == "a"
+    assert tuple(int(x) for x in nan[1]) == (int(i), int(j), int(k)), (nan, line)
