@@ -183,6 +183,17 @@ void conc_average(Grid *grid, double ***cn, double tval);              /* decl:4
    rhs holds the solution and bcof/ccof the eliminated factors */
 void pll_tridiag(Tds *arr, int N, int n);
 
+/* pll_tridiag without allocation: no counterpart in the reference, for
+   callers that cannot allocate (code under graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  The lines pass through a static
+   device array of 4 * PLLC_NOALLOC_MAX_ROWS doubles (32 MiB, held by every
+   GPU the library runs on until the process ends), as many systems at a time
+   as fit; a system longer than PLLC_NOALLOC_MAX_ROWS rows is refused (-1,
+   pllc_last_error).  Calls that overlap on one GPU share the array and race.
+   Same results as pll_tridiag, bit for bit; returns 0 on success. */
+#define PLLC_NOALLOC_MAX_ROWS 1048576
+int pll_tridiag_noalloc(Tds *arr, int N, int n);
+
 /* fftw_3d.h:3-5: 2D REDFT10 / REDFT01 over the first two indices of
    [ndim1][ndim2][num_transforms] arrays; idct divides by 4*ndim1*ndim2 */
 void plan_dct_3d(int ndim1, int ndim2, int num_transforms);

@@ -920,6 +920,89 @@ extern "C" void pll_tridiag(Tds* arr, int N, int n)
     lines_on_device(N, n, a.data(), b.data(), c.data(), r.data(), 1, "pll_tridiag");
 }
 
+// ------------------------------------------- pll_tridiag without allocation
+//
+// pll_tridiag_noalloc stages the lines in `scratch`, a device array of
+// 4 x PLLC_NOALLOC_MAX_ROWS doubles that every GPU holds for the life of the
+// process, instead of the buffers pll_tridiag allocates; it calls no
+// allocator (device or host) and uses the null stream.  Its device address
+// is taken once per device (hipGetSymbolAddress) and used for every copy and
+// launch: the host-side name `scratch` is not a device address.  Calls that
+// overlap on one GPU share the array and race.  Every value the kernel reads
+// is copied in first, so nothing needs clearing between calls.
+
+static __device__ __align__(256) double scratch[4 * (size_t)PLLC_NOALLOC_MAX_ROWS];
+
+namespace {
+constexpr int kNoallocDevices = 64;
+double* g_noalloc_ptr[kNoallocDevices] = {};
+std::mutex g_noalloc_mutex;
+}  // namespace
+
+// the device address of `scratch` on the current device (looked up on the
+// first call with that device current)
+static double* noalloc_scratch(const char* W)
+{
+    int dev = -1;
+    if (!pllc_hip_ok(W, hipGetDevice(&dev))) return nullptr;
+    if (dev < 0 || dev >= kNoallocDevices) {
+        pllc_set_error(W, "device index beyond 63");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        if (!pllc_hip_ok(W, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)))) return nullptr;
+        g_noalloc_ptr[dev] = (double*)p;
+    }
+    return g_noalloc_ptr[dev];
+}
+
+extern "C" int pll_tridiag_noalloc(Tds* arr, int N, int n)
+{
+    const char* W = "pll_tridiag_noalloc";
+    if (N < 0 || n < 0) {
+        pllc_set_error(W, "negative size");
+        return -1;
+    }
+    if ((long)n > PLLC_NOALLOC_MAX_ROWS) {
+        pllc_set_error(W, "a system of n rows exceeds PLLC_NOALLOC_MAX_ROWS (1048576) rows");
+        return -1;
+    }
+    if (N == 0 || n == 0) return 0;
+    double* g = noalloc_scratch(W);
+    if (!g) return -1;
+    // systems per pass: as many whole lines as one quarter of scratch holds,
+    // line-major (line s at s * n, row i at + i) in each of a, b, c, r
+    const int G = (int)std::min<long>((long)N, PLLC_NOALLOC_MAX_ROWS / n);
+    const size_t L = (size_t)G * n;
+    const hipStream_t st = nullptr;
+    for (int s0 = 0; s0 < N; s0 += G) {
+        const int ns = std::min(G, N - s0);
+        for (int s = 0; s < ns; s++) {
+            const Tds& t = arr[s0 + s];
+            const double* src[4] = {t.acof, t.bcof, t.ccof, t.rhs};
+            for (int k = 0; k < 4; k++)
+                if (!pllc_hip_ok(W, hipMemcpyAsync(g + k * L + (size_t)s * n, src[k], (size_t)n * 8,
+                                                   hipMemcpyHostToDevice, st)))
+                    return -1;
+        }
+        pllc::thomas_lines(ns, n, n, 1, g, g + L, g + 2 * L, g + 3 * L, nullptr, 1, st);
+        if (!pllc_hip_ok(W, hipGetLastError())) return -1;
+        // b, c and r come back (pll_tridiag.c:21-47 overwrites all three)
+        for (int s = 0; s < ns; s++) {
+            const Tds& t = arr[s0 + s];
+            double* dst[3] = {t.bcof, t.ccof, t.rhs};
+            for (int k = 0; k < 3; k++)
+                if (!pllc_hip_ok(W, hipMemcpyAsync(dst[k], g + (k + 1) * L + (size_t)s * n, (size_t)n * 8,
+                                                   hipMemcpyDeviceToHost, st)))
+                    return -1;
+        }
+        if (!pllc_hip_ok(W, hipStreamSynchronize(st))) return -1;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------------------- DCT
 namespace {
 struct DctPlan {
