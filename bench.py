@@ -35,6 +35,12 @@ TOUCHES = {
 SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
 
 
+# the learned closure's set-up: target horizon (coarse steps), the scale of
+# the applied correction, and the coarse run's Timestep cap (the fine run's)
+CLOSURE_HORIZON = 4
+CLOSURE_ALPHA = 0.5
+CLOSURE_TIMESTEP = 2e-3
+
 # initial-condition recipes, as the workload names them
 IC_NAMES = {0: "erf front", 1: "noise front", 2: "16x16-mode front"}
 
@@ -66,6 +72,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 plain/mixed line")
     ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure line")
     return ap.parse_args()
 
 
@@ -239,6 +246,62 @@ def large_line(a):
     return out
 
 
+def closure_line(a):
+    """The learned closure (cfg: coarse 128x64x64 FP32 run with the 3-layer
+    3x3x3 CNN, R=3, Pe=2000, judged against the fine 256x128x128 plain FP64
+    run): training data from 4 box-filtered fine runs with R ~ U[2,4],
+    Pe ~ U[1000,3000] (seeds 500..503; held-out run seed 600), training on
+    the GPU with the library's BF16 kernels, then the fine run of the
+    256x128x128 workload for the workload's 1000 steps against the coarse
+    run without and with the closure up to the same time: mixing-zone
+    lengths, their error against the fine run's, and the cost ratio (fine
+    GPU time over coarse wall time)."""
+    import numpy as np
+    import pllcart3d_amd as P
+    from pllcart3d_amd import closure as CL
+    from pllcart3d_amd.configs import CONFIGS, config_params
+    t0 = time.perf_counter()
+    fine_shape = (256, 128, 128)
+    fp = config_params("front_256x128x128")
+    cfg = CONFIGS["front_256x128x128"]
+    cp = CL.coarse_params(fp)
+    mask = CL.loss_mask(cp)
+    quiet = lambda *x: None
+    X, T = CL.collect(fine_shape, CL.draw_training_runs(4, 500), 600, 60, 60, quiet, CLOSURE_HORIZON)
+    Xv, Tv = CL.collect(fine_shape, CL.draw_training_runs(1, 600), 600, 120, 120, quiet, CLOSURE_HORIZON)
+    t1 = time.perf_counter()
+    net, norm, losses = CL.train(X, T, mask, (cp.nx, cp.ny, cp.nz), 3000, 1, quiet)
+    t2 = time.perf_counter()
+    trained, none, gain = CL.validation_gain(net, norm, Xv, Tv, mask)
+    shift, scale, osc = norm
+    net.set_norm(shift, scale, osc * CLOSURE_ALPHA)
+    c0 = P.ic_fill(fp, cfg["ic"], cfg["seed"], cfg["amp"])
+    fine = P.Sim(fp, "plain")
+    fine.init(c0)
+    ms_f = fine.time_steps(0, cfg["steps"])
+    t_end = CL.sim_time(fine)
+    Lf = CL.mixing_length(fine.download("C_OLD"), fp)
+    fine.close()
+    out = {"workload": "coarse 128x64x64 mixed + 3-layer 3x3x3 CNN (16 ch, BF16 MFMA) vs fine 256x128x128 "
+                       "plain FP64, R=3 Pe=2000, 16x16-mode front, to the fine run's time after %d steps"
+                       % cfg["steps"],
+           "training": {"fine_runs": 4, "samples": len(X), "horizon_steps": CLOSURE_HORIZON,
+                        "adam_steps": len(losses), "loss_first": float(losses[:50].mean()),
+                        "loss_last": float(losses[-50:].mean()), "data_s": round(t1 - t0, 1),
+                        "train_s": round(t2 - t1, 1)},
+           "heldout": {"loss_trained": trained, "loss_no_correction": none, "gain": gain},
+           "fine": {"steps": cfg["steps"], "t": t_end, "gpu_ms": ms_f, "mixing_length": Lf},
+           "correction_scale": CLOSURE_ALPHA}
+    for name, n in (("coarse_plain", None), ("coarse_closure", net)):
+        L, wall, steps, fb, worst = CL.coarse_run(fp, c0, t_end, n, CLOSURE_TIMESTEP, tol_bound=2e-2)
+        out[name] = {"steps": steps, "wall_ms": wall * 1e3, "cost_ratio": ms_f / (wall * 1e3), "mixing_length": L,
+                     "err_pct": 100.0 * (L - Lf) / Lf, "fallback_steps": fb,
+                     "guard_worst_bound_excess": float(worst[0])}
+    net.close()
+    out["seconds"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -289,6 +352,9 @@ def main():
     large = None
     if not a.no_large:
         large = large_line(a)
+    closure = None
+    if not a.no_closure:
+        closure = closure_line(a)
     step_ab = 2256.0 * cells   # whole-step algorithmic bytes (282 field touches x 8 B)
     cpu = None
     if not a.no_cpu:
@@ -311,6 +377,7 @@ def main():
         "kernel_gbs": hbm_table(kernels, cells, 8.0, prm) if kernels else None,
         "ensemble": ens,
         "large": large,
+        "closure": closure,
         "sweep_lines_per_block": {k: {"x": v[0], "y": v[1], "z": v[2]} for k, v in tuned.items()},
         "kernels": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels.items()},
         "kernels_mixed": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in kernels_m.items()},

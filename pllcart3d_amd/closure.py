@@ -20,7 +20,7 @@ import time
 import numpy as np
 
 from . import Sim, ic_fill
-from ._lib import IC_MULTIMODE, Params, last_error, load
+from ._lib import FIELDS, IC_MULTIMODE, Params, last_error, load
 from .configs import make_params
 
 NIN = 5
@@ -258,31 +258,38 @@ def load_coarse(coarse, st):
     coarse.op("UPDATE_OLD")
 
 
-def sample_one_step(fine, fp):
+def sample_one_step(fine, fp, horizon=1):
     """(raw features, target) at the fine simulation's current state: the
-    coarse solver's one-step error.  The filtered state is advanced one step
-    on the coarse grid (mixed precision, its Timestep set to the fine step's
-    dt, which the coarse CFL limit does not bind); the fine run advances the
-    same dt; the target is (filter(c_fine after) - c_coarse after) / dt, the
-    rate the correction of N must add for the coarse step to land on the
-    filtered fine one.  Advances `fine` by one step."""
+    coarse solver's error over `horizon` steps.  The filtered state is
+    advanced on the coarse grid (mixed precision) step by step with the fine
+    run's dt (the coarse Timestep set to each fine step's dt, which the
+    coarse CFL limit does not bind); the target is
+    (filter(c_fine after) - c_coarse after) / (time advanced), the rate the
+    correction of N must add for the coarse run to land on the filtered fine
+    one.  Advances `fine` by `horizon` steps."""
     cp = coarse_params(fp)
     st = filtered_state(fine, fp)
-    _, dt = fine.run(1)
+    dts = [fine.run(1)[1] for _ in range(horizon)]
     fc1 = fine.download("C_OLD")
-    cpx = coarse_params(fp)
-    cpx.Timestep = dt
-    coarse = Sim(cpx, "mixed")
-    coarse.init(st["C"])
-    load_coarse(coarse, st)
-    mu = coarse.download("MUO_CELL")
-    _, dtc = coarse.run(1)
-    cc1 = coarse.download("C_OLD")
-    coarse.close()
-    if abs(dtc - dt) > 1e-9 * dt:
-        raise RuntimeError("coarse step dt %.9g differs from the fine step's %.9g" % (dtc, dt))
+    cc = None
+    for n, dt in enumerate(dts):
+        cpx = coarse_params(fp)
+        cpx.Timestep = dt
+        coarse = Sim(cpx, "mixed")
+        coarse.init(st["C"])
+        if n == 0:
+            load_coarse(coarse, st)
+            mu = coarse.download("MUO_CELL")
+        else:
+            for f, a in cc.items():
+                coarse.upload(f, a)
+        _, dtc = coarse.run(1)
+        if abs(dtc - dt) > 1e-9 * dt:
+            raise RuntimeError("coarse step dt %.9g differs from the fine step's %.9g" % (dtc, dt))
+        cc = {f: coarse.download(f) for f in FIELDS}
+        coarse.close()
     x = features(st["C"], st["U"], st["V"], st["W"], mu, cp)
-    t = (filter_cells(fc1, fp) - cc1[3:3 + cp.nx, 1:1 + cp.ny, 1:1 + cp.nz]) / dt
+    t = (filter_cells(fc1, fp) - cc["C_OLD"][3:3 + cp.nx, 1:1 + cp.ny, 1:1 + cp.nz]) / sum(dts)
     return x, t
 
 
@@ -350,9 +357,10 @@ def draw_training_runs(n, seed0=500):
     return out
 
 
-def collect(fine_shape, runs, steps, every, warmup, log=print):
+def collect(fine_shape, runs, steps, every, warmup, log=print, horizon=1):
     """(features, targets) of box-filtered fine runs: one sample every
-    `every` steps from step `warmup` to `steps` of each run."""
+    `every` steps from step `warmup` to `steps` of each run, each target the
+    coarse solver's error over `horizon` steps."""
     X, T = [], []
     for R, Pe, seed in runs:
         fp = make_params(*fine_shape, Vratio=R, Pe=Pe, Inf_thick=0.02)
@@ -362,8 +370,8 @@ def collect(fine_shape, runs, steps, every, warmup, log=print):
         done = 0
         for target_step in range(warmup, steps + 1, every):
             fine.run(target_step - done)
-            x, t = sample_one_step(fine, fp)
-            done = target_step + 1
+            x, t = sample_one_step(fine, fp, horizon)
+            done = target_step + horizon
             X.append(x.astype(np.float32))
             T.append(t.astype(np.float32))
         rms = float(np.sqrt(np.mean(T[-1].astype(np.float64) ** 2)))

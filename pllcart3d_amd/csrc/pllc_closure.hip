@@ -94,13 +94,18 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
         const int k0 = (int)(seg % segk) * 16;
         const long row = seg / segk;
         const int j = (int)(row % g.ny), i = (int)(row / g.ny);
+        // every lane loads for every offset (a cell outside the grid reads
+        // the segment's own clamped cell and is zeroed after), so the 27
+        // loads issue back to back without branches
+        const long self = vox(g, i, j, min(k0 + m, g.nz - 1));
         bf16x4 a[NOFF];
 #pragma unroll
         for (int o = 0; o < NOFF; o++) {
             const int ii = i + o / 9 - 1, jj = j + (o / 3) % 3 - 1, kk = k0 + m + o % 3 - 1;
-            a[o] = zero;
-            if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
-                a[o] = *(const bf16x4*)(in + vox(g, ii, jj, kk) * CH + 4 * kq);
+            const bool ok = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
+            const long v = ok ? vox(g, ii, jj, kk) : self;
+            const bf16x4 t = *(const bf16x4*)(in + v * CH + 4 * kq);
+            a[o] = ok ? t : zero;
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -294,25 +299,38 @@ __global__ __launch_bounds__(256) void k_ysum(G3 g, const float* __restrict__ y,
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// the mean of y over the corrected cells from nb block sums, into mean[0]
+// (one block)
+__global__ __launch_bounds__(256) void k_ymean(G3 g, const double* __restrict__ part, int nb,
+                                               double* __restrict__ mean)
+{
+    __shared__ double s[256];
+    double sm = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) sm += part[b];
+    s[threadIdx.x] = sm;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long cells = (long)(g.nx - 2 * XMARGIN) * g.ny * g.nz;
+        mean[0] = cells > 0 ? s[0] / (double)cells : 0.0;
+    }
+}
+
 // The correction field on the simulation's ghosted layout:
 // ncorr[i][j][k] = out_scale * (y[p] - mean) on interior cells at least
 // XMARGIN cells from the inlet and the outlet, 0 on every other interior
-// cell; mean is the average of y over those cells (from nb block sums), so
-// the correction adds no mass.  out_scale is nrm[2 NIN] (device memory, so a
+// cell; mean is the average of y over those cells (k_ymean), so the
+// correction adds no mass.  out_scale is nrm[2 NIN] (device memory, so a
 // captured graph sees later changes).
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(pllc::Dims d, const float* __restrict__ y,
-                                                 const float* __restrict__ nrm, const double* __restrict__ part,
-                                                 int nb, T* __restrict__ ncorr)
+                                                 const float* __restrict__ nrm, const double* __restrict__ mean,
+                                                 T* __restrict__ ncorr)
 {
-    __shared__ double smean;
-    if (threadIdx.x == 0) {
-        double sm = 0.0;
-        for (int b = 0; b < nb; b++) sm += part[b];
-        const long cells = (long)(d.nx - 2 * XMARGIN) * d.ny * d.nz;
-        smean = cells > 0 ? sm / (double)cells : 0.0;
-    }
-    __syncthreads();
+    const double smean = mean[0];
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     const long n = (long)d.nx * d.ny * d.nz;
     if (p >= n) return;
@@ -418,7 +436,7 @@ struct pllc_cnn {
     double* lpart = nullptr;
     float* nrm = nullptr;          // 2 * NIN input shifts and scales, then out_scale
     float hnrm[2 * NIN + 1] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
-    double* ypart = nullptr;       // block sums of y (the mean the correction removes)
+    double* ypart = nullptr;       // block sums of y, then their mean (the mean the correction removes)
     long adam_t = 0;
     std::vector<pllc::Sample> samples;
     std::vector<void*> owned;
@@ -428,7 +446,7 @@ namespace {
 
 constexpr int WG_BLOCKS = 256;
 constexpr int LOSS_BLOCKS = 256;
-constexpr int YSUM_BLOCKS = 64;
+constexpr int YSUM_BLOCKS = 512;
 
 bool dmalloc(pllc_cnn* m, void** p, size_t bytes, const char* W)
 {
@@ -529,7 +547,7 @@ pllc_cnn* pllc_cnn_create(int nx, int ny, int nz)
          dmalloc(m, (void**)&m->y, (size_t)m->n * 4, W) &&
          dmalloc(m, (void**)&m->part, (size_t)WG_BLOCKS * (NOFF * 256 + CH) * 4, W) &&
          dmalloc(m, (void**)&m->lpart, LOSS_BLOCKS * 8, W) && dmalloc(m, (void**)&m->nrm, (2 * NIN + 1) * 4, W) &&
-         dmalloc(m, (void**)&m->ypart, YSUM_BLOCKS * 8, W);
+         dmalloc(m, (void**)&m->ypart, (YSUM_BLOCKS + 1) * 8, W);
     for (int l = 0; ok && l < 3; l++) {
         ok = dmalloc(m, (void**)&m->Wf[l], WSZ * 2, W);
         if (ok && l > 0) ok = dmalloc(m, (void**)&m->Wb[l], WSZ * 2, W);
@@ -738,7 +756,8 @@ void pllc_cnn_correct(pllc_cnn* m, const pllc::Dims& d, const T* c, const T* u, 
     k_gather<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, c, u, v, w, mu, m->nrm, m->x);
     forward(m);
     k_ysum<<<YSUM_BLOCKS, 256, 0, st>>>(m->g, m->y, m->ypart);
-    k_scatter<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, m->y, m->nrm, m->ypart, YSUM_BLOCKS, ncorr);
+    k_ymean<<<1, 256, 0, st>>>(m->g, m->ypart, YSUM_BLOCKS, m->ypart + YSUM_BLOCKS);
+    k_scatter<T><<<(unsigned)((m->n + 255) / 256), 256, 0, st>>>(d, m->y, m->nrm, m->ypart + YSUM_BLOCKS, ncorr);
     m->st = keep;
 }
 template void pllc_cnn_correct<float>(pllc_cnn*, const pllc::Dims&, const float*, const float*, const float*,

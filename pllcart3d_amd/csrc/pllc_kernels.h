@@ -48,6 +48,10 @@ struct Ctx {
     // (cos, sin) pairs, see k_dct_plane / k_dct_lines; null when the dense
     // contractions are used
     T* tw = nullptr;
+    // BF16x3 transform (FP32, ny = nz = 128): the REDFT10 and REDFT01 matrices
+    // as hi and lo BF16 halves, [fwd y, fwd z, inv y, inv z] x [hi, lo] x
+    // [out][in]; null when not used
+    unsigned short* cbf = nullptr;
     double* scal = nullptr;   // S_SIZE device scalars
     double* part = nullptr;   // reduction partials
     double* tval = nullptr;   // device copy of tval
@@ -110,6 +114,9 @@ struct Ops {
     // its blocks the LDS it needs (n = 128: a whole plane of T; n = 256: 32
     // lines), with the kernels set up to use it
     static bool dct_fast_ready(int n);
+    // whether FP32 planes of n x n run the BF16x3 matrix-core transform
+    // (n = 128) with the kernels set up for it
+    static bool dct_mfma_ready(int n);
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)

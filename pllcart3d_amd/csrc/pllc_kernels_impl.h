@@ -1711,6 +1711,115 @@ __global__ __launch_bounds__(1024) void k_dct_plane(const T* in, T* out, const T
     for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[(e >> 7) * FD_P + (e & 127)];
 }
 
+// N = 128, FP32 data: the 2D transform of plane b as two dense contractions
+// on the matrix cores in BF16x3 (each FP32 operand split as hi + lo, hi =
+// bf16(x), lo = bf16(x - hi); a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi with
+// FP32 accumulation, about 16 significant bits per product): first
+// T = X Cz^T (z), then Y = Cy T (y), C the REDFT10 matrix (forward) or the
+// REDFT01 matrix (inverse), stored as hi and lo BF16 arrays [out][in]
+// (cb: fwd y, fwd z, inv y, inv z, each hi then lo).  The plane sits in LDS
+// (row pitch DM_P); 8 waves, wave w owning rows 16 w .. 16 w + 15 of T and
+// of Y.  v_mfma_f32_16x16x16_bf16: A 16 x 16 (lane l: row l % 16, columns
+// 4 (l / 16) .. +3), B 16 x 16 (lane l: rows 4 (l / 16) .. +3, column
+// l % 16), D (lane l: rows 4 (l / 16) .. +3, column l % 16).  Step one
+// reads only the wave's own rows of X, so T overwrites them in place; the
+// inverse writes phi normalised (and its face ghosts).  in may equal out.
+constexpr int DM_N = 128, DM_P = 132;
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short dm_bf4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float dm_f4;
+
+__device__ __forceinline__ unsigned short dm_bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void dm_split(const float (&x)[4], dm_bf4& hi, dm_bf4& lo)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const unsigned short h = dm_bf(x[q]);
+        hi[q] = (short)h;
+        lo[q] = (short)dm_bf(x[q] - __uint_as_float((unsigned)h << 16));
+    }
+}
+
+template <bool INV>
+__global__ __launch_bounds__(512) void k_dct_bf16x3(const float* in, float* out,
+                                                    const unsigned short* __restrict__ cb, PhiOut<float> po)
+{
+    extern __shared__ __align__(16) unsigned char pllc_smem[];
+    float* pl = reinterpret_cast<float*>(pllc_smem);   // [128][DM_P]
+    constexpr int N = DM_N, NN = DM_N * DM_N;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long base = (long)b * NN;
+    for (int e = 4 * tid; e < NN; e += 4 * 512) {
+        const float4 v = *(const float4*)(in + base + e);
+        *(float4*)(pl + (e >> 7) * DM_P + (e & 127)) = v;
+    }
+    __syncthreads();
+    const int m = lane & 15, kq = lane >> 4;
+    const unsigned short* Cy = cb + (INV ? 4 : 0) * NN;   // hi, then lo at + NN
+    const unsigned short* Cz = cb + (INV ? 6 : 2) * NN;
+    {
+        // T[y][k] = sum_z X[y][z] Cz[k][z], rows y of this wave
+        dm_f4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc[t] = dm_f4{0.f, 0.f, 0.f, 0.f};
+        const float* xr = pl + (16 * w + m) * DM_P + 4 * kq;
+        for (int z0 = 0; z0 < N; z0 += 16) {
+            const float4 xv = *(const float4*)(xr + z0);
+            const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+            dm_bf4 ah, al;
+            dm_split(xa, ah, al);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const long ci = (long)(16 * t + m) * N + z0 + 4 * kq;
+                const dm_bf4 bh = *(const dm_bf4*)(Cz + ci), bl = *(const dm_bf4*)(Cz + NN + ci);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) pl[(16 * w + 4 * kq + q) * DM_P + 16 * t + m] = acc[t][q];
+    }
+    __syncthreads();
+    // Y[k'][k] = sum_y Cy[k'][y] T[y][k], rows k' of this wave
+    dm_f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = dm_f4{0.f, 0.f, 0.f, 0.f};
+    for (int y0 = 0; y0 < N; y0 += 16) {
+        const long ci = (long)(16 * w + m) * N + y0 + 4 * kq;
+        const dm_bf4 ah = *(const dm_bf4*)(Cy + ci), al = *(const dm_bf4*)(Cy + NN + ci);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            float tv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) tv[q] = pl[(y0 + 4 * kq + q) * DM_P + 16 * t + m];
+            dm_bf4 bh, bl;
+            dm_split(tv, bh, bl);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc[t], 0, 0, 0);
+        }
+    }
+    if (INV && po.phi) {
+        const float rn = 1.0f / float(4 * po.ny * po.nz);
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) phi_store(po, b, 16 * w + 4 * kq + q, 16 * t + m, acc[t][q] * rn);
+        return;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) out[base + (long)(16 * w + 4 * kq + q) * N + 16 * t + m] = acc[t][q];
+}
+
 // N = 256: a plane does not fit a block's LDS, so the 2D transform is two
 // launches of FL lines per block (8 FL threads, 8 per line; LDS [FL][N + 1]):
 // AX 2 transforms rows (z, contiguous): block B the packed rows FL B ..
@@ -3062,6 +3171,17 @@ static bool lds_optin(const void* f, int sm)
 }
 
 template <typename T>
+bool Ops<T>::dct_mfma_ready(int n)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (n != DM_N) return false;
+        const int sm = DM_N * DM_P * (int)sizeof(float);
+        return lds_optin((const void*)k_dct_bf16x3<false>, sm) && lds_optin((const void*)k_dct_bf16x3<true>, sm);
+    }
+    return false;
+}
+
+template <typename T>
 bool Ops<T>::dct_fast_ready(int n)
 {
     if (n == FD_N) {
@@ -3083,6 +3203,19 @@ template <typename T>
 void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool ghosts)
 {
     const int nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
+    if constexpr (sizeof(T) == 4) {
+        if (c.cbf && ny == DM_N && nz == DM_N) {
+            // 128 x 128 FP32 planes: BF16x3 contractions on the matrix cores
+            const size_t sm = (size_t)DM_N * DM_P * sizeof(float);
+            if (inverse) {
+                const PhiOut<float> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+                PLLC_L(c, "dct", (k_dct_bf16x3<true><<<nx, 512, sm, c.st>>>(in, out, c.cbf, po)));
+            } else {
+                PLLC_L(c, "dct", (k_dct_bf16x3<false><<<nx, 512, sm, c.st>>>(in, out, c.cbf, PhiOut<float>{})));
+            }
+            return;
+        }
+    }
     if (c.tw && ny == FD_N && nz == FD_N) {
         // 128 x 128 planes: the fast transform, one block per x-plane
         const size_t sm = (size_t)FD_N * FD_P * sizeof(T);

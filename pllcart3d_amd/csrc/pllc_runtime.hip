@@ -320,6 +320,33 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
     // keeps the dense contractions), when the device gives its blocks the LDS
     const char* dm = getenv("PLLC_DCT");
     const int n = d.ny;
+    // FP32 128 x 128 planes with PLLC_DCT=mfma: the BF16x3 matrix-core
+    // transform, its matrices split into hi and lo BF16 halves on the host
+    if (sizeof(T) == 4 && d.nz == n && dm && strcmp(dm, "mfma") == 0 && Ops<T>::dct_mfma_ready(n)) {
+        const size_t nn = (size_t)n * n;
+        std::vector<unsigned short> hb(8 * nn);
+        auto bf = [](float f) {
+            unsigned u;
+            memcpy(&u, &f, 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            return (unsigned short)(u >> 16);
+        };
+        const bool inv[4] = {false, false, true, true};
+        for (int q = 0; q < 4; q++) {
+            pllc_dct_matrix(n, inv[q], h.data());   // the y and z matrices are equal for ny = nz
+            for (size_t e = 0; e < nn; e++) {
+                const float x = (float)h[e];
+                const unsigned short hi = bf(x);
+                const unsigned hu = (unsigned)hi << 16;
+                float hf;
+                memcpy(&hf, &hu, 4);
+                hb[(2 * q) * nn + e] = hi;
+                hb[(2 * q + 1) * nn + e] = bf(x - hf);
+            }
+        }
+        if (!get((void**)&c.cbf, hb.size() * 2)) return false;
+        if (!pllc_hip_ok(where, hipMemcpy(c.cbf, hb.data(), hb.size() * 2, hipMemcpyHostToDevice))) return false;
+    }
     if (d.nz == n && (n == 128 || n == 256) && !(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_fast_ready(n)) {
         // (cos, sin) pairs: e^{-2 pi i k/M} (k < M = n/2), e^{-2 pi i k/n}
         // (k <= n/2), e^{-i pi k/(2n)} (k < n)

@@ -84,12 +84,15 @@ def test_mixed_nmvp_tracks_fp64_oracle(oracle_mod):
         assert errs[f] <= tol, (f, errs[f])
 
 
-@pytest.mark.parametrize("n", [128, 256])
-def test_mixed_square_planes_track_fp64_oracle(oracle_mod, n):
+@pytest.mark.parametrize("n,dct", [(128, "fft"), (256, "fft"), (128, "mfma")])
+def test_mixed_square_planes_track_fp64_oracle(oracle_mod, n, dct, monkeypatch):
     """ny = nz = n (the fast transform in FP32: one plane per block at 128,
-    lines at 256) on a 16x16-mode front over 24 x-cells, whose planes vary
+    lines at 256; with PLLC_DCT=mfma at 128 the BF16x3 contractions on the
+    matrix cores) on a 16x16-mode front over 24 x-cells, whose planes vary
     in y and z: c, the velocity vector and phi after 10 steps against the
     FP64 oracle, to the bars above."""
+    if dct == "mfma":
+        monkeypatch.setenv("PLLC_DCT", "mfma")
     prm = P.make_params(24, n, n, Vratio=3.0, Pe=1000.0, Inf_thick=0.3)
     c0 = P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.05)
     orc = oracle_mod.Oracle(prm)
@@ -100,6 +103,39 @@ def test_mixed_square_planes_track_fp64_oracle(oracle_mod, n):
         orc.run(1)
         sim.run(1)
     errs = {f: _err(sim, orc, f) for f in ("C_OLD", "U_OLD+V_OLD+W_OLD", "PHI")}
-    print("[mixed 24x%dx%d] after 10 steps: %s" % (n, n, " ".join("%s=%.2e" % kv for kv in errs.items())), flush=True)
+    print("[mixed 24x%dx%d %s] after 10 steps: %s" % (n, n, dct, " ".join("%s=%.2e" % kv for kv in errs.items())),
+          flush=True)
     for f, e in errs.items():
         assert e <= TOL[f], (f, e)
+
+
+def test_bf16x3_poisson_solve_vs_fp64_oracle(oracle_mod, monkeypatch):
+    """One phisolver_dct from a shared state: phi of the mixed path with the
+    BF16x3 matrix-core transform (PLLC_DCT=mfma) and with the FP32 fast
+    transform against the FP64 oracle's; both printed, the BF16x3 one held to
+    1e-4 relative L2 (its products keep about 16 significant bits)."""
+    prm = P.make_params(12, 128, 128, Vratio=3.0, Pe=1000.0, Inf_thick=0.3)
+    c0 = P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.05)
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    orc.run(2)
+    dt = min(orc.op("FIND_CFL"), orc.timestep)
+    vb = orc.op("BULK_VELO")
+    for op in ("CONCENTRATION", "VISCOSITY", "XMOM", "YMOM", "ZMOM", "VELO_BOUNDARY", "DIVERGENCE"):
+        orc.op(op, dt, 1, vb)
+    state = {f: orc.field(f).copy() for f in P.FIELDS}
+    orc.op("PHISOLVER", dt, 1, vb)
+    want = orc.field("PHI")[3:-3, 1:-1, 1:-1]
+    errs = {}
+    for dct in ("fft", "mfma"):
+        if dct == "mfma":
+            monkeypatch.setenv("PLLC_DCT", "mfma")
+        sim = P.Sim(prm, "mixed")
+        sim.init(c0)
+        for f, a in state.items():
+            sim.upload(f, a)
+        sim.op("PHISOLVER", dt, 1, vb)
+        errs[dct] = rel_l2(sim.download("PHI")[3:-3, 1:-1, 1:-1], want)
+        sim.close()
+    print("[poisson 12x128x128] phi vs FP64: fft %.2e bf16x3 %.2e" % (errs["fft"], errs["mfma"]), flush=True)
+    assert errs["mfma"] <= 1e-4 and errs["fft"] <= 1e-4
