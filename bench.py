@@ -141,15 +141,48 @@ PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
 
 
 # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated by the copy
-# kernel) of the 256x128x128 workload, from profiles/step_fp64_256x128x128.md
-# and profiles/step_mixed_256x128x128.md (MB, per kernel class; the DCT is
-# the mean of the forward and inverse plane transforms)
-MEASURED_TRAFFIC_MB = {
-    8.0: {"sweep_z": 188.2, "sweep_x": 260.7, "sweep_y": 192.9, "dct": 79.3, "corr_pg": 595.5,
-          "rhs_c": 244.3, "rhs_u": 348.3, "rhs_v": 348.8, "rhs_w": 349.2, "div": 138.8, "visc": 175.4},
-    4.0: {"sweep_z": 93.9, "sweep_x": 132.5, "sweep_y": 95.8, "dct": 34.0, "corr_pg": 299.1,
-          "rhs_c": 122.8, "rhs_u": 175.2, "rhs_v": 175.4, "rhs_w": 175.5, "div": 69.8, "visc": 88.0},
+# kernel) of the 256x128x128 workload, read from the committed rocprofv3
+# summaries (profiles/step_fp64_256x128x128.md, step_mixed_256x128x128.md:
+# their "HBM read MB | HBM write MB" table, per dispatch), per kernel class;
+# the DCT is the mean of the forward and inverse plane transforms
+PROFILES = os.path.join(ROOT, "profiles")
+CLASS_KERNELS = {
+    "sweep_x": ["k_sweep_part<{T}, 0>"], "sweep_y": ["k_sweep_part<{T}, 1>"],
+    "sweep_z": ["k_sweep_part_z<{T}>", "k_sweep_part_zt<{T}>"], "corr_pg": ["k_corr_pg<{T}>"],
+    "rhs_c": ["k_rhs_c<{T}, true>"], "rhs_u": ["k_rhs_mom<{T}, 1, true>"], "rhs_v": ["k_rhs_mom<{T}, 2, true>"],
+    "rhs_w": ["k_rhs_mom<{T}, 3, true>"], "div": ["k_div<{T}>"], "visc": ["k_visc<{T}>"],
+    "dct": ["k_dct_plane<{T}, false>", "k_dct_plane<{T}, true>"],
 }
+
+
+def measured_traffic_mb(bytes_per_value):
+    """{kernel class: HBM MB read + written per launch} from the profile
+    summary of the 256x128x128 workload in this precision ({} when absent)."""
+    name = "step_fp64_256x128x128.md" if bytes_per_value == 8.0 else "step_mixed_256x128x128.md"
+    T = "double" if bytes_per_value == 8.0 else "float"
+    rows = {}
+    try:
+        with open(os.path.join(PROFILES, name)) as f:
+            table = False
+            for line in f:
+                if "HBM read MB" in line:
+                    table = True
+                    continue
+                if table and line.startswith("| `"):
+                    cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                    rows[cells[0].strip("`")] = float(cells[2]) + float(cells[3])
+                elif table and not line.startswith("|"):
+                    table = False
+    except OSError:
+        return {}
+    out = {}
+    for cls, names in CLASS_KERNELS.items():
+        vals = [rows[n.format(T=T)] for n in names if n.format(T=T) in rows]
+        if cls == "dct" and len(vals) == 2:
+            out[cls] = round(sum(vals) / 2, 1)
+        elif cls != "dct" and vals:
+            out[cls] = vals[0]
+    return out
 
 
 def dct_flops_per_launch(prm):
@@ -162,7 +195,7 @@ def dct_flops_per_launch(prm):
 
 def roofline(kernels, cells, bytes_per_value, prm):
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    tmb = MEASURED_TRAFFIC_MB.get(bytes_per_value, {}).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
+    tmb = measured_traffic_mb(bytes_per_value).get(dom) if (prm.nx, prm.ny, prm.nz) == (256, 128, 128) else None
     traffic = {dom: round(tmb * 1e6)} if tmb else None
     avg_s = kernels[dom]["avg_ms"] * 1e-3
     if dom == "dct" and not dct_is_fast(prm):

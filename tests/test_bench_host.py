@@ -53,7 +53,7 @@ def test_roofline_fields_of_the_dominant_kernel():
     assert r["frac"] == pytest.approx(r["achieved"] / r["peak"], rel=1e-3)
     assert r["peak"] == 8000.0
     # measured HBM bytes per launch of that kernel at this workload
-    assert r["traffic"] == pytest.approx(bench.MEASURED_TRAFFIC_MB[8.0]["corr_pg"] * 1e6)
+    assert r["traffic"] == pytest.approx(bench.measured_traffic_mb(8.0)["corr_pg"] * 1e6)
     # the dense transform, when dominant, is priced against the matrix cores
     dense = {"dct": {"avg_ms": 1.0, "launches_per_step": 12.0, "ms_per_step": 12.0}}
     r2 = bench.roofline(dense, 64 * 32 * 32, 8.0, _prm(64, 32, 32))
@@ -74,3 +74,13 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+def test_traffic_comes_from_the_committed_profiles():
+    """roofline.traffic is read from the rocprofv3 summaries in profiles/
+    (read + write MB per dispatch of the kernel class)."""
+    t8 = bench.measured_traffic_mb(8.0)
+    t4 = bench.measured_traffic_mb(4.0)
+    assert t8["sweep_x"] > 100 and t4["sweep_x"] > 50
+    assert set(t8) >= {"sweep_x", "sweep_y", "sweep_z", "corr_pg", "rhs_c", "rhs_u", "dct"}
+    assert t8["sweep_x"] == pytest.approx(2 * t4["sweep_x"], rel=0.1)
