@@ -350,9 +350,9 @@ int pllc_cnn_loss_grad(pllc_cnn *m, int sample, double *loss, double *grad);
 int pllc_cnn_train(pllc_cnn *m, int nsteps, const int *order, int norder, double lr, double *losses);
 /* the simulation adds the network's correction to N in every concentration
  * update (NULL: none); the network's grid must equal the simulation's; with
- * once_per_step the correction of a step's first substage is kept for its
- * other two */
-int pllc_sim_set_closure(pllc_sim *s, pllc_cnn *m, int once_per_step);
+ * a refresh period every >= 1 the correction is made at the first substage
+ * of every every-th step and kept until the next, with 0 at every substage */
+int pllc_sim_set_closure(pllc_sim *s, pllc_cnn *m, int every);
 /* nsteps steps with the closure, checked after every check_every steps: c
  * within [-tol_bound, 1 + tol_bound] and |dt sum(correction)| <= tol_mass
  * |sum(c)| over the interior cells; steps that fail are run again from their

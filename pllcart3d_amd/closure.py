@@ -322,11 +322,11 @@ def mixing_length(c, p, lo=0.05, hi=0.95):
 
 
 
-def set_closure(sim, net, once_per_step=False):
+def set_closure(sim, net, every=0):
     """The simulation adds net's correction to N from now on (None: off);
-    once_per_step keeps the correction of a step's first substage for its
-    other two."""
-    if load().pllc_sim_set_closure(sim.h, net.h if net is not None else None, int(once_per_step)) != 0:
+    with every = k >= 1 the correction is made at the first substage of every
+    k-th step and kept until the next one (0: at every substage)."""
+    if load().pllc_sim_set_closure(sim.h, net.h if net is not None else None, int(every)) != 0:
         raise RuntimeError("pllc_sim_set_closure: " + last_error())
 
 
@@ -437,10 +437,10 @@ def validation_gain(net, norm, X, T, mask):
 
 
 def coarse_run(fp, c0_fine, t_end, net=None, timestep=None, guard_every=10, tol_bound=1e-2, tol_mass=1e-4,
-               chunk=20):
+               chunk=20, every=1):
     """The coarse (2x) mixed-precision run of the fine run (fp, c0_fine) up to
-    t_end, with the closure `net` (guarded, correction once per step) or
-    without it; returns the mixing-zone length at t_end, the wall time (s),
+    t_end, with the closure `net` (guarded, a new correction every `every`
+    steps) or without it; returns the mixing-zone length at t_end, the wall time (s),
     the steps, the fallback steps and the guard's worst bound excess and mass
     ratio."""
     cp = coarse_params(fp)
@@ -451,7 +451,7 @@ def coarse_run(fp, c0_fine, t_end, net=None, timestep=None, guard_every=10, tol_
         s = Sim(cp, "mixed")
         s.init(coarse_c0(fp, c0_fine))
         if net is not None:
-            set_closure(s, net, True)
+            set_closure(s, net, every)
         return s
 
     warm = make()   # first use: graph capture and kernel loading

@@ -66,58 +66,85 @@ enum { EPI_TANH = 0, EPI_OUT0 = 1, EPI_DTANH = 2 };
 //   EPI_OUT0  out_f[p] = . + bias[0] (channel 0 only)
 //   EPI_DTANH out_bf[p][co] = bf16(. * (1 - aprev[p][co]^2))  (the backward
 //             pass through the previous layer's tanh)
-// One wave: 16 consecutive cells along k of one (i, j) row (the rows of the
-// MFMA's A operand) times the 16 output channels.  Per offset o the A
-// operand is the 16 cells' input channels, lane l holding cell l % 16 and
-// channels 4 (l / 16) .. 4 (l / 16) + 3; the B operand is W[o], packed on
-// the device so that lane l reads its four values W[o][4 (l / 16) + q][l % 16]
-// as 8 contiguous bytes: Wp[(o * 64 + l) * 4 + q].  The accumulator gives
-// lane l the cells 4 (l / 16) + q, channel l % 16.  Each wave keeps the 27
-// B operands in registers and takes segments in turn (grid-stride); per
-// segment all 27 A loads are issued before the 27 MFMAs.
+// One block: the cells (i, j0 .. j0+7, k0 .. k0+31).  Its input window,
+// (i-1 .. i+1, j0-1 .. j0+8, k0-1 .. k0+32) with zeros outside the grid, is
+// staged in LDS once (32 KB), so each input cell is fetched about 4 times
+// (3 x-planes, 1.33 for the j and k halo) instead of once per offset.  One wave: 16 consecutive cells
+// along k of one (i, j) row (the rows of the MFMA's A operand) times the 16
+// output channels, four such segments per wave.  Per offset o the A operand
+// is the 16 cells' input channels, lane l holding cell l % 16 and channels
+// 4 (l / 16) .. 4 (l / 16) + 3 (8 contiguous bytes of the window); the B
+// operand is W[o], packed on the device so that lane l reads its four
+// values W[o][4 (l / 16) + q][l % 16] as 8 contiguous bytes:
+// Wp[(o * 64 + l) * 4 + q].  The accumulator gives lane l the cells
+// 4 (l / 16) + q, channel l % 16.  Each wave keeps the 27 B operands in
+// registers; per segment all 27 A reads are issued before the 27 MFMAs.
+constexpr int CT_J = 8, CT_K = 32, CW_J = CT_J + 2, CW_K = CT_K + 2;
+constexpr int CW_CELLS = 3 * CW_J * CW_K;
+
+__device__ __forceinline__ float fast_tanh(float z)
+{
+    // tanh z = 1 - 2 / (e^{2z} + 1), with the hardware exp (relative error
+    // ~1e-7, far below the BF16 rounding of the stored activation)
+    return 1.f - 2.f / (__expf(2.f * z) + 1.f);
+}
+
 __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __restrict__ in,
                                               const unsigned short* __restrict__ Wp, const float* __restrict__ bias,
                                               int epi, const unsigned short* __restrict__ aprev,
                                               unsigned short* __restrict__ out_bf, float* __restrict__ out_f)
 {
-    const int lane = threadIdx.x & 63;
-    const int segk = (g.nz + 15) >> 4;
-    const long nseg = (long)g.nx * g.ny * segk;
-    const long nwave = (long)gridDim.x * 4;
+    __shared__ uint4 win[CW_CELLS * 2];   // 16 BF16 channels = two uint4 per cell
+    const int tk = (g.nz + CT_K - 1) / CT_K, tj = (g.ny + CT_J - 1) / CT_J;
+    int bid = blockIdx.x;
+    const int k0 = (bid % tk) * CT_K;
+    bid /= tk;
+    const int j0 = (bid % tj) * CT_J;
+    const int i = bid / tj;
+    // 32-bit cell indices: the grid holds < 2^27 cells
+    for (int e = threadIdx.x; e < CW_CELLS * 2; e += 256) {
+        const int c = e >> 1;
+        const int pk = c % CW_K, r = c / CW_K;
+        const int pj = r % CW_J, pi = r / CW_J;
+        const int ii = i + pi - 1, jj = j0 + pj - 1, kk = k0 + pk - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
+            v = reinterpret_cast<const uint4*>(in + ((ii * g.ny + jj) * g.nz + kk) * CH)[e & 1];
+        win[e] = v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
-    const bf16x4 zero = {0, 0, 0, 0};
     bf16x4 b[NOFF];
 #pragma unroll
-    for (int o = 0; o < NOFF; o++) b[o] = *(const bf16x4*)(Wp + ((long)o * 64 + lane) * 4);
+    for (int o = 0; o < NOFF; o++) b[o] = *(const bf16x4*)(Wp + (o * 64 + lane) * 4);
     const float bc = bias ? bias[m] : 0.f;
-    for (long seg = (long)blockIdx.x * 4 + (threadIdx.x >> 6); seg < nseg; seg += nwave) {
-        const int k0 = (int)(seg % segk) * 16;
-        const long row = seg / segk;
-        const int j = (int)(row % g.ny), i = (int)(row / g.ny);
-        // every lane loads for every offset (a cell outside the grid reads
-        // the segment's own clamped cell and is zeroed after), so the 27
-        // loads issue back to back without branches
-        const long self = vox(g, i, j, min(k0 + m, g.nz - 1));
+    __syncthreads();
+    const unsigned short* ws = reinterpret_cast<const unsigned short*>(win);
+    for (int sgm = 0; sgm < 4; sgm++) {
+        // rows 2 wave and 2 wave + 1 of the tile, two segments each
+        const int rj = 2 * wave + (sgm >> 1), sk = (sgm & 1) * 16;
+        const int j = j0 + rj;
+        if (j >= g.ny || k0 + sk >= g.nz) continue;   // uniform over the wave
         bf16x4 a[NOFF];
 #pragma unroll
-        for (int o = 0; o < NOFF; o++) {
-            const int ii = i + o / 9 - 1, jj = j + (o / 3) % 3 - 1, kk = k0 + m + o % 3 - 1;
-            const bool ok = ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz;
-            const long v = ok ? vox(g, ii, jj, kk) : self;
-            const bf16x4 t = *(const bf16x4*)(in + v * CH + 4 * kq);
-            a[o] = ok ? t : zero;
+        for (int r = 0; r < 9; r++) {
+            // window row of offset (oi, oj) = (r / 3 - 1, r % 3 - 1)
+            const int wr = ((r / 3) * CW_J + rj + r % 3) * CW_K + sk + m;
+#pragma unroll
+            for (int dk = 0; dk < 3; dk++) a[3 * r + dk] = *(const bf16x4*)(ws + (wr + dk) * CH + 4 * kq);
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int o = 0; o < NOFF; o++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[o], b[o], acc, 0, 0, 0);
+        const int kb = k0 + sk + 4 * kq;
+        const int vb = (i * g.ny + j) * g.nz + kb;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int k = k0 + 4 * kq + q;
-            if (k >= g.nz) continue;
-            const long v = vox(g, i, j, k);
+            if (kb + q >= g.nz) continue;
+            const int v = vb + q;
             const float z = acc[q] + bc;
             if (epi == EPI_TANH) {
-                out_bf[v * CH + m] = f2bf(tanhf(z));
+                out_bf[v * CH + m] = f2bf(fast_tanh(z));
             } else if (epi == EPI_OUT0) {
                 if (m == 0) out_f[v] = z;
             } else {
@@ -261,10 +288,10 @@ __global__ __launch_bounds__(256) void k_gather(pllc::Dims d, const T* __restric
                                                 const T* __restrict__ mu, const float* __restrict__ nrm,
                                                 unsigned short* __restrict__ x)
 {
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    const long n = (long)d.nx * d.ny * d.nz;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int n = d.nx * d.ny * d.nz;
     if (p >= n) return;
-    const int k = (int)(p % d.nz), j = (int)(p / d.nz % d.ny), i = (int)(p / ((long)d.nz * d.ny));
+    const int k = p % d.nz, jk = p / d.nz, j = jk % d.ny, i = jk / d.ny;
     const long o = (long)(i + 3) * d.sx + (long)(j + 1) * d.sy + (k + 1);
     float f[NIN];
     f[0] = (float)c[o];
@@ -284,12 +311,11 @@ __global__ __launch_bounds__(256) void k_gather(pllc::Dims d, const T* __restric
 __global__ __launch_bounds__(256) void k_ysum(G3 g, const float* __restrict__ y, double* __restrict__ part)
 {
     __shared__ double s[256];
-    const long n = (long)g.nx * g.ny * g.nz, plane = (long)g.ny * g.nz;
+    // the corrected cells are the contiguous planes XMARGIN .. nx - XMARGIN - 1
+    const int plane = g.ny * g.nz;
+    const int p0 = XMARGIN * plane, p1 = (g.nx - XMARGIN) * plane;
     double sm = 0.0;
-    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
-        const long i = p / plane;
-        if (i >= XMARGIN && i < g.nx - XMARGIN) sm += (double)y[p];
-    }
+    for (int p = p0 + blockIdx.x * 256 + threadIdx.x; p < p1; p += gridDim.x * 256) sm += (double)y[p];
     s[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -331,10 +357,10 @@ __global__ __launch_bounds__(256) void k_scatter(pllc::Dims d, const float* __re
                                                  T* __restrict__ ncorr)
 {
     const double smean = mean[0];
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    const long n = (long)d.nx * d.ny * d.nz;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int n = d.nx * d.ny * d.nz;
     if (p >= n) return;
-    const int k = (int)(p % d.nz), j = (int)(p / d.nz % d.ny), i = (int)(p / ((long)d.nz * d.ny));
+    const int k = p % d.nz, jk = p / d.nz, j = jk % d.ny, i = jk / d.ny;
     const long o = (long)(i + 3) * d.sx + (long)(j + 1) * d.sy + (k + 1);
     const bool in = i >= XMARGIN && i < d.nx - XMARGIN;
     ncorr[o] = in ? (T)(nrm[2 * NIN] * (float)((double)y[p] - smean)) : (T)0;
@@ -357,8 +383,8 @@ __global__ __launch_bounds__(256) void k_guard_stats(pllc::Dims d, const T* __re
     __shared__ double smin[256], smax[256], ssum[256];
     const long n = (long)d.nx * d.ny * d.nz;
     double lo = INFINITY, hi = -INFINITY, sm = 0.0;
-    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
-        const int k = (int)(p % d.nz), j = (int)(p / d.nz % d.ny), i = (int)(p / ((long)d.nz * d.ny));
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < (int)n; p += gridDim.x * 256) {
+        const int k = p % d.nz, jk = p / d.nz, j = jk % d.ny, i = jk / d.ny;
         const double x = (double)c[(long)(i + 3) * d.sx + (long)(j + 1) * d.sy + (k + 1)];
         lo = x < lo ? x : lo;   // a NaN leaves lo, hi unchanged and makes sm NaN
         hi = x > hi ? x : hi;
@@ -391,8 +417,8 @@ __global__ __launch_bounds__(256) void k_corr_sum(pllc::Dims d, const T* __restr
     __shared__ double s[256];
     const long n = (long)d.nx * d.ny * d.nz;
     double sm = 0.0;
-    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) {
-        const int k = (int)(p % d.nz), j = (int)(p / d.nz % d.ny), i = (int)(p / ((long)d.nz * d.ny));
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < (int)n; p += gridDim.x * 256) {
+        const int k = p % d.nz, jk = p / d.nz, j = jk % d.ny, i = jk / d.ny;
         sm += (double)q[(long)(i + 3) * d.sx + (long)(j + 1) * d.sy + (k + 1)];
     }
     s[threadIdx.x] = sm;
@@ -455,12 +481,10 @@ bool dmalloc(pllc_cnn* m, void** p, size_t bytes, const char* W)
     return true;
 }
 
-// blocks of the grid-stride convolution: one segment per wave up to 2048
-// blocks (8 per CU), more segments per wave beyond
+// blocks of the convolution: one per tile of 8 x 32 cells of an x-plane
 int conv_blocks(const G3& g)
 {
-    const long b = ((long)g.nx * g.ny * ((g.nz + 15) / 16) + 3) / 4;
-    return (int)std::min<long>(b, 2048);
+    return g.nx * ((g.ny + CT_J - 1) / CT_J) * ((g.nz + CT_K - 1) / CT_K);
 }
 
 void pack_all(pllc_cnn* m)
@@ -532,6 +556,11 @@ pllc_cnn* pllc_cnn_create(int nx, int ny, int nz)
     const char* W = "pllc_cnn_create";
     if (nx < 1 || ny < 1 || nz < 1) {
         pllc_set_error(W, "bad grid");
+        return nullptr;
+    }
+    if ((long)nx * ny * nz >= (1L << 27)) {
+        // the kernels index cells x 16 channels in 32 bits
+        pllc_set_error(W, "grid too large (at most 2^27 - 1 cells)");
         return nullptr;
     }
     pllc_cnn* m = new pllc_cnn();

@@ -52,6 +52,9 @@ struct Ctx {
     // as hi and lo BF16 halves, [fwd y, fwd z, inv y, inv z] x [hi, lo] x
     // [out][in]; null when not used
     unsigned short* cbf = nullptr;
+    // small planes (ny, nz multiples of 16, at most 64): the dense transform
+    // in one launch per direction (k_dct_small) instead of two k_gemm
+    int dsmall = 0;
     double* scal = nullptr;   // S_SIZE device scalars
     double* part = nullptr;   // reduction partials
     double* tval = nullptr;   // device copy of tval
@@ -117,6 +120,9 @@ struct Ops {
     // whether FP32 planes of n x n run the BF16x3 matrix-core transform
     // (n = 128) with the kernels set up for it
     static bool dct_mfma_ready(int n);
+    // whether planes of ny x nz (multiples of 16, at most 64) run both dense
+    // contractions in one launch (k_dct_small) with the kernels set up for it
+    static bool dct_small_ready(int ny, int nz);
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)

@@ -1466,6 +1466,74 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
         }
 }
 
+// Dense (y, z) transforms of small planes (ny and nz multiples of 16, at
+// most 64: the coarse and ensemble grids) in one launch.  Block (x, g) makes
+// the columns kz = 16 g .. 16 g + 15 of x-plane x: it stages the plane in
+// LDS, contracts it along z into Q[y][kz] = sum_m P[y][m] Cz[kz][m] (its 16
+// columns, every y), then along y into out[ky][kz] = sum_m Cy[ky][m] Q[m][kz]
+// (with phi set: phi, normalised, with its face ghosts).  Wave w makes the
+// 16-row tile w of both (ny <= 64); the Cz and Cy operands of its tiles are
+// loaded into registers at the start.  Every output tile is the MFMA
+// sequence of k_gemm's (a zero accumulator, k ascending in steps of 4), so
+// the result equals the two k_gemm launches bit for bit, without Q's round
+// trip through HBM.
+constexpr int DS_MAX = 64;
+
+template <typename T, bool INV>
+__global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __restrict__ in, T* __restrict__ out,
+                                                   const T* __restrict__ Cz, const T* __restrict__ Cy, PhiOut<T> po)
+{
+    extern __shared__ unsigned char pllc_smem[];
+    typedef typename Mma<T>::acc acc_t;
+    constexpr int KM = DS_MAX / 4;             // k steps of 4 at most
+    const int pz = nz + 1;
+    T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][nz + 1]
+    T* Q = P + ny * pz;                        // [ny][17]: the block's 16 columns
+    const int ng = nz >> 4;
+    const int x = blockIdx.x / ng, z0 = (blockIdx.x % ng) * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ty = ny >> 4;
+    // this lane's B operands of the z contraction (column z0 + li of Cz^T)
+    // and A operands of the y contraction (its first tile's rows of Cy)
+    T bz[KM], ay[KM];
+#pragma unroll
+    for (int q = 0; q < KM; q++) {
+        bz[q] = 4 * q < nz ? Cz[(long)(z0 + li) * nz + 4 * q + lk] : T(0);
+        ay[q] = (4 * q < ny && wave < ty) ? Cy[(long)(16 * wave + li) * ny + 4 * q + lk] : T(0);
+    }
+    const long plane = (long)ny * nz;
+    const T* __restrict__ src = in + x * plane;
+    for (int e = threadIdx.x; e < ny * nz; e += 256) {
+        const int y = e / nz;
+        P[y * pz + e - y * nz] = src[e];
+    }
+    __syncthreads();
+    // ny <= 64: at most four row tiles, one per wave
+    const int y0 = 16 * wave;
+    if (wave < ty) {
+        acc_t acc = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < KM; q++)
+            if (4 * q < nz) acc = Mma<T>::run(P[(y0 + li) * pz + 4 * q + lk], bz[q], acc);
+#pragma unroll
+        for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * 17 + li] = acc[r];
+    }
+    __syncthreads();
+    if (wave >= ty) return;
+    const T norm = T(4 * po.ny * po.nz);
+    acc_t acc = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int q = 0; q < KM; q++)
+        if (4 * q < ny) acc = Mma<T>::run(ay[q], Q[(4 * q + lk) * 17 + li], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
+        if (INV && po.phi) phi_store(po, x, gm, gn, fdiv<T>(acc[r], norm));
+        else out[x * plane + (long)gm * nz + gn] = acc[r];
+    }
+}
+
 // ------------------------------------------------- fast DCT of planes
 //
 // The (y, z) transforms of phisolver_dct for ny = nz = N, N = 128 or 256
@@ -1547,11 +1615,28 @@ __device__ __forceinline__ void dft16(Cx<T> (&a)[16], const T* __restrict__ tw, 
     }
 }
 
+// Position of real e of a line in LDS.  With SW (the 128 x 128 plane in
+// LDS) bits 1..3 of e are XORed with bits 4..6: a permutation that keeps
+// each complex value's two reals together and puts the values that the 8
+// threads of a line touch at a stride of 16 reals (the transposed reads of
+// fft_line) in different banks; without SW the line is in natural order.
+template <bool SW>
+__device__ __forceinline__ int lp(int e) { return SW ? (e ^ (((e >> 4) & 7) << 1)) : e; }
+
 // complex value c of a line stored as reals (2c, 2c+1) at stride st
-template <typename T>
-__device__ __forceinline__ Cx<T> lcx(const T* L, int st, int c) { return {L[2 * c * st], L[(2 * c + 1) * st]}; }
-template <typename T>
-__device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v) { L[2 * c * st] = v.r; L[(2 * c + 1) * st] = v.i; }
+template <typename T, bool SW = false>
+__device__ __forceinline__ Cx<T> lcx(const T* L, int st, int c)
+{
+    const int e = lp<SW>(2 * c);
+    return {L[e * st], L[(e + 1) * st]};
+}
+template <typename T, bool SW = false>
+__device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v)
+{
+    const int e = lp<SW>(2 * c);
+    L[e * st] = v.r;
+    L[(e + 1) * st] = v.i;
+}
 
 // M-point DFT (M = 64 or 128) of the line's complex values, in place
 // through LDS, natural order in and out; thread t of the line's 8.  Four
@@ -1559,14 +1644,14 @@ __device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v) { L[2 * c * st
 // over n1, twiddles by e^{-2 pi i t k1/M} and stores at 8 k1 + t; then the
 // 8-point DFTs over the contiguous 8 k1 .. 8 k1 + 7 (P/8 of them per thread)
 // give X[k1 + P k2].  Every thread of the block calls it (block barriers).
-template <typename T, bool INV, int M>
+template <typename T, bool INV, int M, bool SW = false>
 __device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restrict__ tw)
 {
     static_assert(M == 64 || M == 128, "fft_line: M is 64 or 128");
     constexpr int P = M / 8;
     Cx<T> a[P];
 #pragma unroll
-    for (int n1 = 0; n1 < P; n1++) a[n1] = lcx(L, st, 8 * n1 + t);
+    for (int n1 = 0; n1 < P; n1++) a[n1] = lcx<T, SW>(L, st, 8 * n1 + t);
     if constexpr (P == 8) dft8<T, INV>(a);
     else dft16<T, INV>(a, tw, M / 16);
 #pragma unroll
@@ -1576,25 +1661,25 @@ __device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restric
     }
     __syncthreads();
 #pragma unroll
-    for (int k1 = 0; k1 < P; k1++) scx(L, st, 8 * k1 + t, a[k1]);
+    for (int k1 = 0; k1 < P; k1++) scx<T, SW>(L, st, 8 * k1 + t, a[k1]);
     __syncthreads();
     Cx<T> b[P / 8][8];
 #pragma unroll
     for (int g = 0; g < P / 8; g++) {
 #pragma unroll
-        for (int m2 = 0; m2 < 8; m2++) b[g][m2] = lcx(L, st, 8 * (t + 8 * g) + m2);
+        for (int m2 = 0; m2 < 8; m2++) b[g][m2] = lcx<T, SW>(L, st, 8 * (t + 8 * g) + m2);
         dft8<T, INV>(b[g]);
     }
     __syncthreads();
 #pragma unroll
     for (int g = 0; g < P / 8; g++)
 #pragma unroll
-        for (int k2 = 0; k2 < 8; k2++) scx(L, st, t + 8 * g + P * k2, b[g][k2]);
+        for (int k2 = 0; k2 < 8; k2++) scx<T, SW>(L, st, t + 8 * g + P * k2, b[g][k2]);
     __syncthreads();
 }
 
 // REDFT10 of one line of N reals at stride st, in place
-template <typename T, int N>
+template <typename T, int N, bool SW = false>
 __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __restrict__ tw)
 {
     constexpr int M = N / 2, V = M / 8;
@@ -1603,14 +1688,14 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
 #pragma unroll
     for (int m1 = 0; m1 < V; m1++) {
         const int m = 8 * m1 + t;
-        a[m1] = m < N / 4 ? Cx<T>{L[(4 * m) * st], L[(4 * m + 2) * st]}
-                          : Cx<T>{L[(2 * N - 1 - 4 * m) * st], L[(2 * N - 3 - 4 * m) * st]};
+        a[m1] = m < N / 4 ? Cx<T>{L[lp<SW>(4 * m) * st], L[lp<SW>(4 * m + 2) * st]}
+                          : Cx<T>{L[lp<SW>(2 * N - 1 - 4 * m) * st], L[lp<SW>(2 * N - 3 - 4 * m) * st]};
     }
     __syncthreads();
 #pragma unroll
-    for (int m1 = 0; m1 < V; m1++) scx(L, st, 8 * m1 + t, a[m1]);
+    for (int m1 = 0; m1 < V; m1++) scx<T, SW>(L, st, 8 * m1 + t, a[m1]);
     __syncthreads();
-    fft_line<T, false, M>(L, st, t, tw);
+    fft_line<T, false, M, SW>(L, st, t, tw);
     // V[k] = (Z[k] + conj Z[M-k])/2 - i/2 WN^k (Z[k] - conj Z[M-k]);
     // X[k] = 2 Re(W4N^k V[k]), X[N-k] = -2 Im(W4N^k V[k])
     T xr[V + 1], xi[V + 1];
@@ -1618,7 +1703,7 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
     for (int j = 0; j <= V; j++) {
         const int k = t + 8 * j;
         if (j < V || t == 0) {
-            const Cx<T> zk = lcx(L, st, k & (M - 1)), zc = cconj(lcx(L, st, (M - k) & (M - 1)));
+            const Cx<T> zk = lcx<T, SW>(L, st, k & (M - 1)), zc = cconj(lcx<T, SW>(L, st, (M - k) & (M - 1)));
             const Cx<T> ve = {T(0.5) * (zk.r + zc.r), T(0.5) * (zk.i + zc.i)};
             const Cx<T> df = {zk.r - zc.r, zk.i - zc.i};
             const Cx<T> vo = {T(0.5) * df.i, T(-0.5) * df.r};   // -i/2 (Zk - Zc)
@@ -1634,15 +1719,15 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
     for (int j = 0; j <= V; j++) {
         const int k = t + 8 * j;
         if (j < V || t == 0) {
-            L[k * st] = xr[j];
-            if (k >= 1 && k <= M - 1) L[(N - k) * st] = xi[j];
+            L[lp<SW>(k) * st] = xr[j];
+            if (k >= 1 && k <= M - 1) L[lp<SW>(N - k) * st] = xi[j];
         }
     }
     __syncthreads();
 }
 
 // REDFT01 of one line of N reals at stride st, in place
-template <typename T, int N>
+template <typename T, int N, bool SW = false>
 __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __restrict__ tw)
 {
     constexpr int M = N / 2, V = M / 8;
@@ -1650,7 +1735,7 @@ __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __res
     // E = Vk + conj V(M-k), O = (Vk - conj V(M-k)) conj(WN^k)
     auto vk_of = [&](int k) -> Cx<T> {
         if (k == 0) return Cx<T>{L[0], T(0)};
-        const Cx<T> x = {L[k * st], -L[(N - k) * st]};
+        const Cx<T> x = {L[lp<SW>(k) * st], -L[lp<SW>(N - k) * st]};
         return cmul(cconj(twl(tw, N + 1 + k)), x);
     };
     Cx<T> c[V];
@@ -1664,19 +1749,19 @@ __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __res
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < V; j++) scx(L, st, t + 8 * j, c[j]);
+    for (int j = 0; j < V; j++) scx<T, SW>(L, st, t + 8 * j, c[j]);
     __syncthreads();
-    fft_line<T, true, M>(L, st, t, tw);
+    fft_line<T, true, M, SW>(L, st, t, tw);
     // y[2n] = v[n], y[2n+1] = v[N-1-n], v[2m] = Re z[m], v[2m+1] = Im z[m]
     Cx<T> z[V];
 #pragma unroll
-    for (int k2 = 0; k2 < V; k2++) z[k2] = lcx(L, st, t + 8 * k2);
+    for (int k2 = 0; k2 < V; k2++) z[k2] = lcx<T, SW>(L, st, t + 8 * k2);
     __syncthreads();
 #pragma unroll
     for (int k2 = 0; k2 < V; k2++) {
         const int m = t + 8 * k2;
-        if (m < N / 4) { L[(4 * m) * st] = z[k2].r; L[(4 * m + 2) * st] = z[k2].i; }
-        else { L[(2 * N - 1 - 4 * m) * st] = z[k2].r; L[(2 * N - 3 - 4 * m) * st] = z[k2].i; }
+        if (m < N / 4) { L[lp<SW>(4 * m) * st] = z[k2].r; L[lp<SW>(4 * m + 2) * st] = z[k2].i; }
+        else { L[lp<SW>(2 * N - 1 - 4 * m) * st] = z[k2].r; L[lp<SW>(2 * N - 3 - 4 * m) * st] = z[k2].i; }
     }
     __syncthreads();
 }
@@ -1692,23 +1777,25 @@ __global__ __launch_bounds__(1024) void k_dct_plane(const T* in, T* out, const T
     T* pl = reinterpret_cast<T*>(pllc_smem);   // [128][FD_P]
     const int b = blockIdx.x, tid = threadIdx.x;
     const long base = (long)b * FD_N * FD_N;
-    for (int e = tid; e < FD_N * FD_N; e += 1024) pl[(e >> 7) * FD_P + (e & 127)] = in[base + e];
+    // point (y, z) of the plane sits at row lp(y), column lp(z): every row
+    // and every column is a line in lp order
+    for (int e = tid; e < FD_N * FD_N; e += 1024) pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)] = in[base + e];
     __syncthreads();
     const int line = tid >> 3, t = tid & 7;
     if (INV) {
-        redft01_line<T, FD_N>(pl + line * FD_P, 1, t, tw);
-        redft01_line<T, FD_N>(pl + line, FD_P, t, tw);
+        redft01_line<T, FD_N, true>(pl + line * FD_P, 1, t, tw);
+        redft01_line<T, FD_N, true>(pl + line, FD_P, t, tw);
     } else {
-        redft10_line<T, FD_N>(pl + line * FD_P, 1, t, tw);
-        redft10_line<T, FD_N>(pl + line, FD_P, t, tw);
+        redft10_line<T, FD_N, true>(pl + line * FD_P, 1, t, tw);
+        redft10_line<T, FD_N, true>(pl + line, FD_P, t, tw);
     }
     if (INV && po.phi) {
         const T norm = T(4 * po.ny * po.nz);
         for (int e = tid; e < FD_N * FD_N; e += 1024)
-            phi_store(po, b, e >> 7, e & 127, fdiv<T>(pl[(e >> 7) * FD_P + (e & 127)], norm));
+            phi_store(po, b, e >> 7, e & 127, fdiv<T>(pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)], norm));
         return;
     }
-    for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[(e >> 7) * FD_P + (e & 127)];
+    for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)];
 }
 
 // N = 128, FP32 data: the 2D transform of plane b as two dense contractions
@@ -2122,6 +2209,8 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
     template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
+    template __global__ void k_dct_small<T, false>(int, int, const T*, T*, const T*, const T*, PhiOut<T>);     \
+    template __global__ void k_dct_small<T, true>(int, int, const T*, T*, const T*, const T*, PhiOut<T>);      \
     template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long,          \
                                              PhiOut<T>);                                                        \
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long,         \
@@ -3182,6 +3271,14 @@ bool Ops<T>::dct_mfma_ready(int n)
 }
 
 template <typename T>
+bool Ops<T>::dct_small_ready(int ny, int nz)
+{
+    if (ny % 16 || nz % 16 || ny > DS_MAX || nz > DS_MAX) return false;
+    const int sm = (ny * (nz + 1) + ny * 17) * (int)sizeof(T);
+    return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm);
+}
+
+template <typename T>
 bool Ops<T>::dct_fast_ready(int n)
 {
     if (n == FD_N) {
@@ -3242,6 +3339,21 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
         }
         return;
     }
+    if (c.dsmall && (in != out || (inverse && phi))) {
+        // small planes: both contractions in one launch, one block per 16
+        // columns of a plane; each block reads its whole plane, so the
+        // result may not overwrite the input (in place: the k_gemm pair)
+        const size_t sm = ((size_t)ny * (nz + 1) + (size_t)ny * 17) * sizeof(T);
+        const unsigned nb = (unsigned)nx * (unsigned)(nz / 16);
+        if (inverse) {
+            const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+            PLLC_L(c, "dct", (k_dct_small<T, true><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Ciz, c.Ciy, po)));
+        } else {
+            PLLC_L(c, "dct", (k_dct_small<T, false><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Cfz, c.Cfy,
+                                                                            PhiOut<T>{})));
+        }
+        return;
+    }
     const long R = (long)nx * ny, plane = (long)ny * nz;
     const dim3 gz((nz + 127) / 128, (unsigned)((R + 63) / 64), 1), gy((nz + 127) / 128, (ny + 63) / 64, nx);
     if (!inverse) {
@@ -3268,18 +3380,21 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts)
     const long nm = (long)c.d.ny * c.d.nz;
     if (from_div)
         PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
-    dct2(c, c.P, c.P, false);
+    // the spectrum: in place in P, or in Q for small planes (whose forward
+    // transform cannot run in place)
+    T* sp = c.dsmall ? c.Q : c.P;
+    dct2(c, c.P, sp, false);
     const int PC = c.line_solver == 0 ? part_chunks(c.d.nx, c.d.nx) : 0;
     if (PC > 0) {
         const int LPB = std::min(64, std::max(1, 256 / PC));
         const size_t sm = sizeof(double) * 3 * 2 * PC + sizeof(T) * 6 * PC * LPB;
         PLLC_L(c, "pois_x", (k_pois_part<T><<<(unsigned)((nm + LPB - 1) / LPB), dim3(LPB, PC, 1), sm, c.st>>>(
-                                c.d, c.lam_y, c.lam_z, c.P, c.lam00)));
+                                c.d, c.lam_y, c.lam_z, sp, c.lam00)));
     } else {
-        PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, c.P, c.QC, c.lam00)));
+        PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, sp, c.QC, c.lam00)));
     }
     // the inverse transform's last pass writes phi, normalised
-    dct2(c, c.P, c.P, true, c.f[PHI], ghosts);
+    dct2(c, sp, sp, true, c.f[PHI], ghosts);
 }
 
 template <typename T>

@@ -347,6 +347,9 @@ bool pllc_ctx_tables(Ctx<T>& c, std::vector<void*>& owned, const char* where)
         if (!get((void**)&c.cbf, hb.size() * 2)) return false;
         if (!pllc_hip_ok(where, hipMemcpy(c.cbf, hb.data(), hb.size() * 2, hipMemcpyHostToDevice))) return false;
     }
+    // planes of at most 64 x 64 (multiples of 16): both dense contractions
+    // in one launch (PLLC_DCT=gemm keeps the two k_gemm launches)
+    if (!(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_small_ready(d.ny, d.nz)) c.dsmall = 1;
     if (d.nz == n && (n == 128 || n == 256) && !(dm && strcmp(dm, "gemm") == 0) && Ops<T>::dct_fast_ready(n)) {
         // (cos, sin) pairs: e^{-2 pi i k/M} (k < M = n/2), e^{-2 pi i k/n}
         // (k <= n/2), e^{-i pi k/(2n)} (k < n)
@@ -393,13 +396,16 @@ struct pllc_sim {
     // after an edit of c_old alone (shift_conc), so that the never-written
     // points of c_new (inlet ghosts, ...) reach c_old as in the reference
     bool copy_next = false;
-    hipGraphExec_t gx = nullptr;
+    // one graph per pattern of the pair's closure corrections (bit q: step q
+    // of the pair makes a new one; 0 without a closure)
+    hipGraphExec_t gx[4] = {};
     std::vector<void*> gtab;
     // learned closure: the network whose correction of N each substage adds
     // (null: none); snapshot buffers of the guarded run (NSTATE fields, the
     // device scalars and tval) and the guard's reduction scratch
     pllc_cnn* cnn = nullptr;
-    bool cnn_once = false;        // the correction of substage 0 held for the step's three
+    int cnn_every = 0;            // 0: a new correction every substage; k: at the first substage
+    long cnn_steps = 0;           // of every k-th step, held in between (steps counted here)
     std::vector<void*> snap;
     double* gscratch = nullptr;
 };
@@ -409,9 +415,17 @@ static void apply_tuned(pllc_sim* s);
 static void graph_drop(pllc_sim* s)
 {
     std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
-    if (s->gx) (void)hipGraphExecDestroy(s->gx);
-    s->gx = nullptr;
+    for (auto& g : s->gx) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     s->gtab.clear();
+}
+
+// whether the step with closure step count n makes a new correction
+static bool cnn_refresh(const pllc_sim* s, long n)
+{
+    return s->cnn && (s->cnn_every == 0 || n % s->cnn_every == 0);
 }
 
 template <typename T> static Ctx<T>& ctx(pllc_sim* s);
@@ -757,7 +771,7 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
         Ops<T>::bulk_stale(c, rk, g + r, true, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         // the learned closure's correction of N from the substage's start
         // state (c, u, v, w and mu_old.cell)
-        if (s->cnn && c.ncorr && (rk == 0 || !s->cnn_once))
+        if (s->cnn && c.ncorr && (s->cnn_every == 0 || (rk == 0 && s->cnn_steps % s->cnn_every == 0)))
             pllc_cnn_correct<T>(s->cnn, c.d, c.f[C_OLD], c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.f[MUO_CELL], c.ncorr,
                                 c.st);
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
@@ -777,6 +791,7 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
             s->new_stale = true;
         }
     }
+    if (s->cnn) s->cnn_steps++;
 }
 
 static void step_any(pllc_sim* s)
@@ -791,19 +806,24 @@ static std::vector<void*> ptr_table(Ctx<T>& c)
     return std::vector<void*>(c.f, c.f + NFIELDS);
 }
 
-// two steps through the captured graph; false (nothing launched, pointer
-// table unchanged) when capture is not possible
+// two steps through the captured graph of the pair's closure pattern;
+// false (nothing launched, pointer table unchanged) when capture is not
+// possible.  A replay does not run fused_step's host code, so the closure's
+// step count is advanced here.
 template <typename T>
 static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
 {
     if (!s->use_graph || c.hook || s->copy_next) return false;
     std::vector<void*> tab = ptr_table(c);
-    if (s->gx && tab != s->gtab) graph_drop(s);
-    if (!s->gx) {
+    if (!s->gtab.empty() && tab != s->gtab) graph_drop(s);
+    const int pat = (cnn_refresh(s, s->cnn_steps) ? 1 : 0) | (cnn_refresh(s, s->cnn_steps + 1) ? 2 : 0);
+    if (!s->gx[pat]) {
         std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
+        const long n0 = s->cnn_steps;
         if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
         fused_step<T>(s, c);
         fused_step<T>(s, c);
+        s->cnn_steps = n0;
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s->st, &g);
         (void)hipGetLastError();
@@ -819,15 +839,16 @@ static bool step_pair_graph(pllc_sim* s, Ctx<T>& c)
             s->use_graph = false;
             return false;
         }
-        s->gx = gx;
+        s->gx[pat] = gx;
         s->gtab = tab;
     }
-    if (hipGraphLaunch(s->gx, s->st) != hipSuccess) {
+    if (hipGraphLaunch(s->gx[pat], s->st) != hipSuccess) {
         (void)hipGetLastError();
         graph_drop(s);
         s->use_graph = false;
         return false;
     }
+    if (s->cnn) s->cnn_steps += 2;
     s->new_stale = true;
     return true;
 }
@@ -881,8 +902,9 @@ extern "C" int pllc_sim_time_steps(pllc_sim* s, int warmup, int nsteps, double* 
 //
 // A simulation with a closure adds the network's correction to the explicit
 // advective term N of every concentration update (no counterpart in the
-// reference); with once_per_step the correction of a step's first substage
-// is kept for its other two.  The guarded run checks, after every
+// reference); with a refresh period of k >= 1 steps the correction is made
+// at the first substage of every k-th step and kept until the next one
+// (0: made at every substage).  The guarded run checks, after every
 // check_every steps, that c stays within [-tol_bound, 1 + tol_bound] and that
 // the mass the correction injects in a step, |dt sum(ncorr)|, stays within
 // tol_mass of the mass sum(c) over the interior cells; when either is
@@ -899,7 +921,7 @@ extern "C" int pllc_closure_guard_ok(const double* stats, double dt, double tol_
 }
 
 template <typename T>
-static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m, int once_per_step)
+static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m, int every)
 {
     const char* W = "pllc_sim_set_closure";
     graph_drop(s);
@@ -919,15 +941,19 @@ static int set_closure_t(pllc_sim* s, Ctx<T>& c, pllc_cnn* m, int once_per_step)
         if (!pllc_hip_ok(W, hipMemset(p, 0, npts(s) * sizeof(T)))) return -1;
         c.ncorr = (T*)p;
     }
+    if (every < 0) {
+        pllc_set_error(W, "negative refresh period");
+        return -1;
+    }
     s->cnn = m;
-    s->cnn_once = once_per_step != 0;
+    s->cnn_every = every;
+    s->cnn_steps = 0;
     return 0;
 }
 
-extern "C" int pllc_sim_set_closure(pllc_sim* s, pllc_cnn* m, int once_per_step)
+extern "C" int pllc_sim_set_closure(pllc_sim* s, pllc_cnn* m, int every)
 {
-    return s->prec ? set_closure_t<float>(s, s->cf, m, once_per_step)
-                   : set_closure_t<double>(s, s->cd, m, once_per_step);
+    return s->prec ? set_closure_t<float>(s, s->cf, m, every) : set_closure_t<double>(s, s->cd, m, every);
 }
 
 template <typename T>
@@ -987,6 +1013,7 @@ static int run_guarded_t(pllc_sim* s, Ctx<T>& c, int nsteps, int check_every, do
             for (int q = 0; q < k; q++) step_any(s);
             s->cnn = m;
             c.ncorr = nc;
+            s->cnn_steps = 0;   // the next step makes a new correction
             fb += k;
         }
         nsteps -= k;

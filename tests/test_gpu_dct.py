@@ -54,3 +54,30 @@ def test_fdct_idct_match_scipy(n1, n2, nt):
     ref3 = scipy.fft.dctn(bv, type=3, axes=(0, 1)) / (4.0 * n1 * n2)
     assert np.linalg.norm(cv - ref3) / np.linalg.norm(ref3) < 1e-14
     assert np.linalg.norm(cv - x) / np.linalg.norm(x) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(8, 64, 64), (12, 32, 48), (8, 16, 64)])
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_small_plane_transform_equals_two_launch_path(monkeypatch, shape, precision):
+    """Planes of at most 64 x 64 (multiples of 16) run both dense
+    contractions of the Poisson solve in one launch (k_dct_small); it makes
+    the MFMA sequence of the two k_gemm launches (PLLC_DCT=gemm), so three
+    steps from a state that varies in z leave every field equal in every
+    bit."""
+    nx, ny, nz = shape
+    prm = P.make_params(nx, ny, nz, Vratio=3.0, Pe=1000.0, Inf_thick=0.05)
+    c0 = P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.02)
+    out = {}
+    for mode in ("gemm", "default"):
+        if mode == "gemm":
+            monkeypatch.setenv("PLLC_DCT", "gemm")
+        else:
+            monkeypatch.delenv("PLLC_DCT", raising=False)
+        print("[dct_small] %s %s %s" % (shape, precision, mode), flush=True)
+        s = P.Sim(prm, precision)
+        s.init(c0)
+        s.run(3)
+        out[mode] = {f: s.download(f) for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "V_PG", "W_PG")}
+        s.close()
+    for f, a in out["gemm"].items():
+        assert np.array_equal(a, out["default"][f]), (shape, precision, f)
