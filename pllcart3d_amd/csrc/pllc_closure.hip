@@ -68,19 +68,24 @@ enum { EPI_TANH = 0, EPI_OUT0 = 1, EPI_DTANH = 2 };
 //             pass through the previous layer's tanh)
 // One block: the cells (i, j0 .. j0+7, k0 .. k0+31).  Its input window,
 // (i-1 .. i+1, j0-1 .. j0+8, k0-1 .. k0+32) with zeros outside the grid, is
-// staged in LDS once (32 KB), so each input cell is fetched about 4 times
-// (3 x-planes, 1.33 for the j and k halo) instead of once per offset.  One wave: 16 consecutive cells
-// along k of one (i, j) row (the rows of the MFMA's A operand) times the 16
-// output channels, four such segments per wave.  Per offset o the A operand
-// is the 16 cells' input channels, lane l holding cell l % 16 and channels
-// 4 (l / 16) .. 4 (l / 16) + 3 (8 contiguous bytes of the window); the B
-// operand is W[o], packed on the device so that lane l reads its four
-// values W[o][4 (l / 16) + q][l % 16] as 8 contiguous bytes:
+// staged in LDS once, so each input cell is fetched about 4 times (3
+// x-planes, 1.33 for the j and k halo) instead of once per offset.  The
+// window is stored by channel group: entry (g, cell) holds channels
+// 4 g .. 4 g + 3 of the cell (8 bytes) at g * CW_PITCH + cell, so the 16
+// lanes that read one channel group of 16 consecutive cells read 128
+// contiguous bytes, and the pitch (24 mod 32 entries) puts the four groups,
+// and the two groups one thread stores at a time, in different banks.  One wave: 16 consecutive cells along k of one (i, j)
+// row (the rows of the MFMA's A operand) times the 16 output channels, four
+// such segments per wave.  Per offset o the A operand is the 16 cells'
+// input channels, lane l holding cell l % 16 and channels 4 (l / 16) ..
+// 4 (l / 16) + 3; the B operand is W[o], packed on the device so that lane l
+// reads its four values W[o][4 (l / 16) + q][l % 16] as 8 contiguous bytes:
 // Wp[(o * 64 + l) * 4 + q].  The accumulator gives lane l the cells
 // 4 (l / 16) + q, channel l % 16.  Each wave keeps the 27 B operands in
 // registers; per segment all 27 A reads are issued before the 27 MFMAs.
 constexpr int CT_J = 8, CT_K = 32, CW_J = CT_J + 2, CW_K = CT_K + 2;
-constexpr int CW_CELLS = 3 * CW_J * CW_K;
+constexpr int CW_CELLS = 3 * CW_J * CW_K;   // 1020
+constexpr int CW_PITCH = 1048;              // >= CW_CELLS, 24 mod 32
 
 __device__ __forceinline__ float fast_tanh(float z)
 {
@@ -94,23 +99,28 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
                                               int epi, const unsigned short* __restrict__ aprev,
                                               unsigned short* __restrict__ out_bf, float* __restrict__ out_f)
 {
-    __shared__ uint4 win[CW_CELLS * 2];   // 16 BF16 channels = two uint4 per cell
+    __shared__ bf16x4 win[4 * CW_PITCH];   // [channel group][cell]
     const int tk = (g.nz + CT_K - 1) / CT_K, tj = (g.ny + CT_J - 1) / CT_J;
     int bid = blockIdx.x;
     const int k0 = (bid % tk) * CT_K;
     bid /= tk;
     const int j0 = (bid % tj) * CT_J;
     const int i = bid / tj;
-    // 32-bit cell indices: the grid holds < 2^27 cells
+    // 32-bit cell indices: the grid holds < 2^27 cells.  Thread e loads
+    // channel groups 2 h and 2 h + 1 (16 bytes) of cell e / 2.
     for (int e = threadIdx.x; e < CW_CELLS * 2; e += 256) {
-        const int c = e >> 1;
+        const int c = e >> 1, h = e & 1;
         const int pk = c % CW_K, r = c / CW_K;
         const int pj = r % CW_J, pi = r / CW_J;
         const int ii = i + pi - 1, jj = j0 + pj - 1, kk = k0 + pk - 1;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
         if (ii >= 0 && ii < g.nx && jj >= 0 && jj < g.ny && kk >= 0 && kk < g.nz)
-            v = reinterpret_cast<const uint4*>(in + ((ii * g.ny + jj) * g.nz + kk) * CH)[e & 1];
-        win[e] = v;
+            v = reinterpret_cast<const uint4*>(in + ((ii * g.ny + jj) * g.nz + kk) * CH)[h];
+        uint2 lo, hi;
+        lo.x = v.x; lo.y = v.y;
+        hi.x = v.z; hi.y = v.w;
+        *reinterpret_cast<uint2*>(&win[(2 * h) * CW_PITCH + c]) = lo;
+        *reinterpret_cast<uint2*>(&win[(2 * h + 1) * CW_PITCH + c]) = hi;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
@@ -119,7 +129,7 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
     for (int o = 0; o < NOFF; o++) b[o] = *(const bf16x4*)(Wp + (o * 64 + lane) * 4);
     const float bc = bias ? bias[m] : 0.f;
     __syncthreads();
-    const unsigned short* ws = reinterpret_cast<const unsigned short*>(win);
+    const bf16x4* wq = win + kq * CW_PITCH;
     for (int sgm = 0; sgm < 4; sgm++) {
         // rows 2 wave and 2 wave + 1 of the tile, two segments each
         const int rj = 2 * wave + (sgm >> 1), sk = (sgm & 1) * 16;
@@ -131,7 +141,7 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
             // window row of offset (oi, oj) = (r / 3 - 1, r % 3 - 1)
             const int wr = ((r / 3) * CW_J + rj + r % 3) * CW_K + sk + m;
 #pragma unroll
-            for (int dk = 0; dk < 3; dk++) a[3 * r + dk] = *(const bf16x4*)(ws + (wr + dk) * CH + 4 * kq);
+            for (int dk = 0; dk < 3; dk++) a[3 * r + dk] = wq[wr + dk];
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

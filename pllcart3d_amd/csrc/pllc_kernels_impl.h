@@ -1478,6 +1478,11 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 // the result equals the two k_gemm launches bit for bit, without Q's round
 // trip through HBM.
 constexpr int DS_MAX = 64;
+// LDS pitches: the plane's rows at nz + 4 values (the 16 x 4 lanes of an A
+// operand read rows li at column lk: li (nz + 4) + lk falls in distinct
+// banks, nz + 4 being 4 times an odd number), Q's rows at 16
+__host__ __device__ constexpr int ds_pitch(int nz) { return nz + 4; }
+constexpr int DS_QP = 16;
 
 template <typename T, bool INV>
 __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __restrict__ in, T* __restrict__ out,
@@ -1486,9 +1491,9 @@ __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __re
     extern __shared__ unsigned char pllc_smem[];
     typedef typename Mma<T>::acc acc_t;
     constexpr int KM = DS_MAX / 4;             // k steps of 4 at most
-    const int pz = nz + 1;
-    T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][nz + 1]
-    T* Q = P + ny * pz;                        // [ny][17]: the block's 16 columns
+    const int pz = ds_pitch(nz);
+    T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][pz]
+    T* Q = P + ny * pz;                        // [ny][DS_QP]: the block's 16 columns
     const int ng = nz >> 4;
     const int x = blockIdx.x / ng, z0 = (blockIdx.x % ng) * 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1517,7 +1522,7 @@ __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __re
         for (int q = 0; q < KM; q++)
             if (4 * q < nz) acc = Mma<T>::run(P[(y0 + li) * pz + 4 * q + lk], bz[q], acc);
 #pragma unroll
-        for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * 17 + li] = acc[r];
+        for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * DS_QP + li] = acc[r];
     }
     __syncthreads();
     if (wave >= ty) return;
@@ -1525,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __re
     acc_t acc = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int q = 0; q < KM; q++)
-        if (4 * q < ny) acc = Mma<T>::run(ay[q], Q[(4 * q + lk) * 17 + li], acc);
+        if (4 * q < ny) acc = Mma<T>::run(ay[q], Q[(4 * q + lk) * DS_QP + li], acc);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
@@ -1770,32 +1775,50 @@ __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __res
 // pitch 129): rows (z) then columns (y), 8 threads per line; the inverse
 // writes phi (or out).  in may equal out: a block reads and writes only its
 // own plane.
+// Threads per block and layout of the plane transform, measured per
+// precision: FP64 at 512 threads (two passes of 64 lines; at 1024 threads
+// the 128-register limit per thread spills) with both transforms in lp
+// order; FP32 at 1024 threads (one pass of 128 lines), the forward in
+// natural order.
+template <typename T> struct FdCfg {
+    static constexpr int threads = sizeof(T) == 8 ? 512 : 1024;
+    static constexpr bool sw_fwd = sizeof(T) == 8;
+};
+
 template <typename T, bool INV>
-__global__ __launch_bounds__(1024) void k_dct_plane(const T* in, T* out, const T* __restrict__ tw, PhiOut<T> po)
+__global__ __launch_bounds__(FdCfg<T>::threads) void k_dct_plane(const T* in, T* out, const T* __restrict__ tw,
+                                                                 PhiOut<T> po)
 {
+    constexpr int FD_THREADS = FdCfg<T>::threads;
+    constexpr bool SW = INV || FdCfg<T>::sw_fwd;
+    constexpr int LG = FD_THREADS / 8;   // lines transformed together
     extern __shared__ __align__(16) unsigned char pllc_smem[];
     T* pl = reinterpret_cast<T*>(pllc_smem);   // [128][FD_P]
     const int b = blockIdx.x, tid = threadIdx.x;
     const long base = (long)b * FD_N * FD_N;
-    // point (y, z) of the plane sits at row lp(y), column lp(z): every row
-    // and every column is a line in lp order
-    for (int e = tid; e < FD_N * FD_N; e += 1024) pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)] = in[base + e];
+    // point (y, z) of the plane sits at row lp(y), column lp(z) (lp the
+    // identity without SW): every row and every column is a line in lp order
+    for (int e = tid; e < FD_N * FD_N; e += FD_THREADS)
+        pl[lp<SW>(e >> 7) * FD_P + lp<SW>(e & 127)] = in[base + e];
     __syncthreads();
-    const int line = tid >> 3, t = tid & 7;
-    if (INV) {
-        redft01_line<T, FD_N, true>(pl + line * FD_P, 1, t, tw);
-        redft01_line<T, FD_N, true>(pl + line, FD_P, t, tw);
-    } else {
-        redft10_line<T, FD_N, true>(pl + line * FD_P, 1, t, tw);
-        redft10_line<T, FD_N, true>(pl + line, FD_P, t, tw);
+    const int t = tid & 7;
+    for (int l0 = 0; l0 < FD_N; l0 += LG) {
+        const int line = l0 + (tid >> 3);
+        if (INV) redft01_line<T, FD_N, SW>(pl + line * FD_P, 1, t, tw);
+        else redft10_line<T, FD_N, SW>(pl + line * FD_P, 1, t, tw);
+    }
+    for (int l0 = 0; l0 < FD_N; l0 += LG) {
+        const int line = l0 + (tid >> 3);
+        if (INV) redft01_line<T, FD_N, SW>(pl + line, FD_P, t, tw);
+        else redft10_line<T, FD_N, SW>(pl + line, FD_P, t, tw);
     }
     if (INV && po.phi) {
         const T norm = T(4 * po.ny * po.nz);
-        for (int e = tid; e < FD_N * FD_N; e += 1024)
-            phi_store(po, b, e >> 7, e & 127, fdiv<T>(pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)], norm));
+        for (int e = tid; e < FD_N * FD_N; e += FD_THREADS)
+            phi_store(po, b, e >> 7, e & 127, fdiv<T>(pl[lp<SW>(e >> 7) * FD_P + lp<SW>(e & 127)], norm));
         return;
     }
-    for (int e = tid; e < FD_N * FD_N; e += 1024) out[base + e] = pl[lp<true>(e >> 7) * FD_P + lp<true>(e & 127)];
+    for (int e = tid; e < FD_N * FD_N; e += FD_THREADS) out[base + e] = pl[lp<SW>(e >> 7) * FD_P + lp<SW>(e & 127)];
 }
 
 // N = 128, FP32 data: the 2D transform of plane b as two dense contractions
@@ -3274,7 +3297,7 @@ template <typename T>
 bool Ops<T>::dct_small_ready(int ny, int nz)
 {
     if (ny % 16 || nz % 16 || ny > DS_MAX || nz > DS_MAX) return false;
-    const int sm = (ny * (nz + 1) + ny * 17) * (int)sizeof(T);
+    const int sm = (ny * ds_pitch(nz) + ny * DS_QP) * (int)sizeof(T);
     return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm);
 }
 
@@ -3318,9 +3341,10 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
         const size_t sm = (size_t)FD_N * FD_P * sizeof(T);
         const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
         if (inverse)
-            PLLC_L(c, "dct", (k_dct_plane<T, true><<<nx, 1024, sm, c.st>>>(in, out, c.tw, po)));
+            PLLC_L(c, "dct", (k_dct_plane<T, true><<<nx, FdCfg<T>::threads, sm, c.st>>>(in, out, c.tw, po)));
         else
-            PLLC_L(c, "dct", (k_dct_plane<T, false><<<nx, 1024, sm, c.st>>>(in, out, c.tw, PhiOut<T>{})));
+            PLLC_L(c, "dct", (k_dct_plane<T, false><<<nx, FdCfg<T>::threads, sm, c.st>>>(in, out, c.tw,
+                                                                                           PhiOut<T>{})));
         return;
     }
     if (c.tw && ny == 256 && nz == 256) {
@@ -3343,7 +3367,7 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
         // small planes: both contractions in one launch, one block per 16
         // columns of a plane; each block reads its whole plane, so the
         // result may not overwrite the input (in place: the k_gemm pair)
-        const size_t sm = ((size_t)ny * (nz + 1) + (size_t)ny * 17) * sizeof(T);
+        const size_t sm = ((size_t)ny * ds_pitch(nz) + (size_t)ny * DS_QP) * sizeof(T);
         const unsigned nb = (unsigned)nx * (unsigned)(nz / 16);
         if (inverse) {
             const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};

@@ -5,6 +5,12 @@
         calibration of the HBM counters) and `steps` fused steps; run it
         under rocprofv3 --kernel-trace --stats or --pmc FETCH_SIZE / WRITE_SIZE.
 
+    python pllcart3d_amd/profiling.py run --closure [--steps 20]
+        the 2x-coarser mixed-precision grid of the configuration with the
+        learned closure's network (seeded initial weights, the correction
+        scaled to zero so the state is the plain one) making a correction
+        every step: the network's kernels beside the step's.
+
     python pllcart3d_amd/profiling.py summary --trace T.db [--fetch F.db --write W.db]
         a markdown summary: per-kernel durations from the kernel trace and the
         HBM bytes per dispatch from the two counter passes, scaled by what the
@@ -32,6 +38,22 @@ def run(args):
     prm = config_params(args.config)
     cfg = CONFIGS[args.config]
     c0 = P.ic_fill(prm, cfg["ic"], cfg["seed"], cfg["amp"])
+    if args.closure:
+        import numpy as np
+        from pllcart3d_amd import closure as CL
+        cp = CL.coarse_params(prm)
+        net = CL.Network(cp.nx, cp.ny, cp.nz)
+        net.set_params(CL.flat_from_dict(CL.init_params(1)))
+        net.set_norm(np.zeros(CL.NIN), np.ones(CL.NIN), 0.0)
+        sim = P.Sim(cp, "mixed")
+        sim.init(CL.coarse_c0(prm, c0))
+        CL.set_closure(sim, net, 1)
+        sim.run(args.steps)
+        sim.synchronize()
+        print("profile run: closure on the %dx%dx%d grid, %d steps" % (cp.nx, cp.ny, cp.nz, args.steps), flush=True)
+        sim.close()
+        net.close()
+        return
     if args.tune:
         # the sweeps' block shapes tuned on a throwaway handle first, as
         # bench.py does; later handles of this grid and precision use them
@@ -123,6 +145,7 @@ def main():
     r.add_argument("--steps", type=int, default=2)
     r.add_argument("--precision", default="plain", choices=["plain", "mixed"])
     r.add_argument("--tune", action="store_true", help="autotune the sweeps' block shapes first (as bench.py)")
+    r.add_argument("--closure", action="store_true", help="the coarse grid with the learned closure instead")
     s = sub.add_parser("summary")
     s.add_argument("--trace", required=True)
     s.add_argument("--fetch")

@@ -251,3 +251,43 @@ def test_trained_closure_gains_over_no_correction():
     print("[closure pipeline] 40 guarded steps: fallbacks %d, c in [%.4f, %.4f]" % (fb, c.min(), c.max()), flush=True)
     assert np.all(np.isfinite(c)) and c.min() > -0.02 and c.max() < 1.02
     net.close()
+
+
+@pytest.mark.parametrize("every", [0, 1, 3])
+def test_refresh_period_graph_equals_eager(monkeypatch, every):
+    """A correction made every `every` steps (0: every substage) and kept in
+    between: pairs of steps replayed from the captured graphs (one per
+    pattern of corrections within the pair) equal eager steps
+    (PLLC_GRAPH=0) in every bit, over 9 steps, so the patterns repeat and
+    the odd last step runs eagerly; and the refresh period changes the
+    result (the correction is really held)."""
+    fp = P.make_params(32, 16, 16, Vratio=3.0, Pe=2000.0, Inf_thick=0.02)
+    cp = CL.coarse_params(fp)
+    c0 = P.ic_fill(fp, P.IC_MULTIMODE, 2, 0.01)
+    Pd = R.init_params(5)
+    Pd["W3"] *= 10.0
+    net = CL.Network(cp.nx, cp.ny, cp.nz)
+    net.set_params(CL.flat_from_dict(Pd))
+    net.set_norm(np.zeros(5), np.ones(5), 0.05)
+    out = {}
+    for mode in ("graph", "eager"):
+        if mode == "eager":
+            monkeypatch.setenv("PLLC_GRAPH", "0")
+        else:
+            monkeypatch.delenv("PLLC_GRAPH", raising=False)
+        s = P.Sim(cp, "mixed")
+        s.init(CL.coarse_c0(fp, c0))
+        CL.set_closure(s, net, every)
+        print("[closure refresh] every %d %s" % (every, mode), flush=True)
+        s.run(9)
+        out[mode] = s.download("C_OLD")
+        s.close()
+    assert np.array_equal(out["graph"], out["eager"]), every
+    monkeypatch.delenv("PLLC_GRAPH", raising=False)
+    s = P.Sim(cp, "mixed")
+    s.init(CL.coarse_c0(fp, c0))
+    CL.set_closure(s, net, 2 if every != 2 else 1)
+    s.run(9)
+    assert not np.array_equal(s.download("C_OLD"), out["graph"]), every
+    s.close()
+    net.close()
