@@ -172,7 +172,7 @@ def measured_traffic_mb(bytes_per_value):
                     continue
                 if table and line.startswith("| `"):
                     cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                    rows[cells[0].strip("`")] = float(cells[2]) + float(cells[3])
+                    rows[cells[0].strip("`")] = round(float(cells[2]) + float(cells[3]), 1)
                 elif table and not line.startswith("|"):
                     table = False
     except OSError:

@@ -27,7 +27,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _short(name):
-    s = re.sub(r"\(.*", "", name).replace("void ", "").replace("pllc::", "")
+    s = name.replace("(anonymous namespace)::", "")
+    s = re.sub(r"\(.*", "", s).replace("void ", "").replace("pllc::", "")
     return s.strip()
 
 
