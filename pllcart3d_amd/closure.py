@@ -2,14 +2,16 @@
 the reference).
 
 The coarse solver misses what the fine grid resolves: a fine run's state,
-box-filtered onto cells twice as large and advanced one step by the coarse
-solver, lands away from the filtered fine state one step later.  That
-difference per unit time is the target; a 3-layer 3x3x3 convolutional
-network (libpllcart3d_amd's BF16 kernels, trained on the GPU) predicts it
-from the coarse cells' c, u, v, w and mu, and the coarse run adds the
-prediction to N in every concentration update (Hc_op's explicit term,
-pllcart3d.c:1539-1644), so that a step adds dt times it to c.  A guarded run falls back to plain WENO where
-c leaves [0, 1] or the correction injects mass (pllc_sim_run_guarded).  The
+box-filtered onto cells twice as large and advanced k steps by the coarse
+solver, lands away from the filtered fine state k steps later.  That
+difference per unit time is the target (k = 1, or the horizon given to
+collect); a 3-layer 3x3x3 convolutional network (libpllcart3d_amd's BF16
+kernels, trained on the GPU) predicts it from the coarse cells' c, u, v, w
+and mu, and the coarse run adds the prediction to N in every concentration
+update (Hc_op's explicit term, pllcart3d.c:1539-1644), so that a step adds
+dt times it to c; the prediction is made anew every `every` steps
+(set_closure).  A guarded run falls back to plain WENO where c leaves
+[0, 1] or the correction injects mass (pllc_sim_run_guarded).  The
 mixing-zone length, from the y-z average of c along x (the profile
 conc_average writes to cavgyz.bin, pllcart3d.c:3691-3724), judges the
 coarse run against the fine one.

@@ -46,6 +46,9 @@ FLOORS = {
     # (the gaps peak at step 1 for fft and step 2 for ulp)
     "front_256x128x128": {"c": 3.1e-16, "velocity": 1.2e-13, "phi": 1.9e-14, "pg": 8.0e-11,
                           "U_OLD": 4.7e-14, "V_OLD": 8.0e-13, "W_OLD": 9.9e-11},
+    # the 512x256x256 workload (16x16-mode front, seed 2), first 2 steps
+    "front_512x256x256": {"c": 9.5e-16, "velocity": 3.2e-13, "phi": 3.0e-13, "pg": 4.2e-10,
+                          "U_OLD": 1.2e-13, "V_OLD": 2.2e-12, "W_OLD": 2.6e-10},
     # the 64x32x32 workload (+-0.01 noise, seed 1), all 200 steps
     "noise_64x32x32": {"c": 9.7e-14, "velocity": 8.7e-15, "phi": 1.3e-12, "pg": 9.4e-13,
                        "U_OLD": 4.3e-15, "V_OLD": 8.1e-14, "W_OLD": 1.9e-10},
@@ -156,7 +159,7 @@ def case_setup(case):
     c = CONFIGS[case]
     prm = config_params(case)
     c0 = P.ic_fill(prm, c["ic"], c["seed"], c["amp"])
-    if case == "front_256x128x128":
+    if case in ("front_256x128x128", "front_512x256x256"):
         return prm, c0, 2, ("fft", "ulp")
     return prm, c0, c["steps"], ("fft", "ulp")
 

@@ -245,7 +245,7 @@ def test_trained_closure_gains_over_no_correction():
     net.set_norm(shift, scale, 0.5 * osc)
     s = P.Sim(cp, "mixed")
     s.init(CL.coarse_c0(fp, c0))
-    CL.set_closure(s, net, True)
+    CL.set_closure(s, net, 4)
     fb, worst = CL.run_guarded(s, 40, 10, 2e-2, 1e-4)
     c = s.download("C_OLD")[3:-3, 1:-1, 1:-1]
     print("[closure pipeline] 40 guarded steps: fallbacks %d, c in [%.4f, %.4f]" % (fb, c.min(), c.max()), flush=True)

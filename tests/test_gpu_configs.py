@@ -15,6 +15,8 @@ step), never below 1e-11.
     full run).
   * noise_64x32x32: 64x32x32, R=2, Pe=500, width 0.05, +-0.01 noise (seed
     1), all of its 200 steps.
+  * front_512x256x256: 512x256x256, R=3, Pe=4000, the same front recipe,
+    its first 3 steps.
   * planes that vary in z: ny = nz = 128 and 256 with 24 x-cells and a
     16x16-mode front, so the (y, z) transforms and the z-sweeps see 3D data.
 """
@@ -101,6 +103,16 @@ def test_front_256x128x128_tracks_oracle(oracle_mod, threads):
     print("[front_256x128x128] |w| max %.3e |u| max %.3e" % (np.abs(w).max(), np.abs(orc.field("U_OLD")).max()))
     _track(orc, sim, STEPS_256, "front_256x128x128", "front_256x128x128")
     assert np.abs(orc.field("W_OLD")).max() > 1e-3
+
+
+def test_front_512x256x256_first_steps_track_oracle(oracle_mod, threads):
+    """The 512x256x256 workload (33.5 M cells; its 256x256 planes take the
+    line-pass transforms), plain FP64, its first 3 steps against the oracle
+    (about 8 s per oracle step on 16 threads), compared after every step."""
+    print("[front_512x256x256] 3 steps, oracle on %d threads" % threads, flush=True)
+    prm, orc, sim = _config_pair(oracle_mod, "front_512x256x256")
+    _track(orc, sim, 3, "front_512x256x256", "front_512x256x256")
+    sim.close()
 
 
 def test_noise_64x32x32_all_200_steps(oracle_mod, threads):
