@@ -233,7 +233,8 @@ def hbm_table(kernels, cells, bytes_per_value, prm):
 
 def ensemble_line(a):
     """The batched ensemble (64 members of three grid sizes, random R and Pe)
-    in the mixed precision: aggregate interior cell-steps per second."""
+    in the mixed precision, the members of each grid size stepped as one
+    batch: aggregate interior cell-steps per second."""
     import pllcart3d_amd as P
     from pllcart3d_amd.configs import ensemble_members
     mem = ensemble_members(a.ensemble_members)
@@ -243,9 +244,10 @@ def ensemble_line(a):
     cells = sum(m[0].nx * m[0].ny * m[0].nz for m in mem)
     steps = max(2, min(a.steps, 10))
     ms = ens.time_steps(2, steps)
+    batches = ens.batches()
     ens.close()
     return {"value": cells * steps / (ms * 1e-3), "unit": "cell-steps/s", "members": len(mem),
-            "cells": cells, "steps": steps, "ms_per_step": ms / steps, "dtype": "f32/f64",
+            "batches": batches, "cells": cells, "steps": steps, "ms_per_step": ms / steps, "dtype": "f32/f64",
             "workload": "ensemble: %d members from {64x32x32, 128x48x48, 192x64x64}, R~U[1,4], "
                         "Pe log-uniform [200,4000], mixed precision" % len(mem)}
 

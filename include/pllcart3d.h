@@ -284,11 +284,16 @@ int pllc_sim_autotune(pllc_sim *s, int nsteps, int *lpb_out);
 
 /* Ensembles (no counterpart in the reference): n independent members with
  * their own parameters (p[0..n-1]) and precision, advanced together; each
- * member is a pllc_sim (upload, download, op, ...) owned by the ensemble. */
+ * member is a pllc_sim (upload, download, op, ...) owned by the ensemble.
+ * In mixed precision the members of equal grids form batches, each advanced
+ * by one launch per kernel of the step (PLLC_ENSEMBLE_BATCH=0: none); a
+ * member's parameters are read at the ensemble's first run. */
 typedef struct pllc_ensemble pllc_ensemble;
 pllc_ensemble *pllc_ensemble_create(const pllc_params *p, int n, int precision);
 void pllc_ensemble_destroy(pllc_ensemble *e);
 int pllc_ensemble_size(const pllc_ensemble *e);
+/* the number of batches (groups of members of equal grids stepped together) */
+int pllc_ensemble_batches(const pllc_ensemble *e);
 pllc_sim *pllc_ensemble_member(pllc_ensemble *e, int i);
 /* nsteps fused steps of every member */
 int pllc_ensemble_run(pllc_ensemble *e, int nsteps);

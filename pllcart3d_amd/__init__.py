@@ -149,8 +149,10 @@ class _Member(Sim):
 
 class Ensemble:
     """Independent members (own grid, Vratio, Pe, initial state) advanced
-    together: each member replays its own captured two-step graph on its own
-    stream, launched round-robin so the members' kernels overlap."""
+    together.  In mixed precision the members of equal grids form batches,
+    each advanced by one launch per kernel of the step; the other members
+    replay their own captured two-step graphs on their own streams as the
+    branches of one graph, so their kernels overlap."""
 
     def __init__(self, params_list, precision="plain"):
         self.L = load()
@@ -160,6 +162,10 @@ class Ensemble:
             raise RuntimeError("pllc_ensemble_create: " + last_error())
         self.members = [_Member(self, self.L.pllc_ensemble_member(self.h, i), p, precision)
                         for i, p in enumerate(params_list)]
+
+    def batches(self):
+        """The number of batches of members stepped together."""
+        return self.L.pllc_ensemble_batches(self.h)
 
     def run(self, nsteps):
         if self.L.pllc_ensemble_run(self.h, nsteps) != 0:

@@ -73,6 +73,7 @@ def load():
     L.pllc_ensemble_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int]
     L.pllc_ensemble_destroy.argtypes = [vp]
     L.pllc_ensemble_size.argtypes = [vp]
+    L.pllc_ensemble_batches.argtypes = [vp]
     L.pllc_ensemble_member.restype = vp
     L.pllc_ensemble_member.argtypes = [vp, C.c_int]
     L.pllc_ensemble_run.argtypes = [vp, C.c_int]

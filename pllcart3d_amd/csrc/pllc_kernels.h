@@ -23,6 +23,22 @@ enum FieldId {
     NFIELDS
 };
 
+// A launch over a batch of simulations of equal dims (the members of an
+// ensemble with one grid shape): member m's fields lie m * fs elements after
+// member 0's, its packed Poisson blocks m * pk after, its reduction partials
+// m * np doubles after, its S_SIZE device scalars m * S_SIZE after and its
+// 16 stale values m * 16 after; ph (device) holds every member's physics and
+// lc (device, of the field type) the line sets' constant coefficients
+// [m][field][direction][del, ca, cb, cc] as the host computes them for a
+// single simulation.  nm = 1 with null ph and lc: one simulation, whose
+// kernels use their own arguments.
+struct Bt {
+    int nm = 1;
+    long fs = 0, pk = 0, np = 0;
+    const Phys* ph = nullptr;
+    const void* lc = nullptr;
+};
+
 // RK3 low-storage coefficients (global_variables.h:17-18)
 static const double kRK1[3] = {(8.0 / 15.0), (5.0 / 12.0), (3.0 / 4.0)};
 static const double kRK2[3] = {(0.0), (-17.0 / 60.0), (-5.0 / 12.0)};
@@ -68,6 +84,8 @@ struct Ctx {
     // lines per block of the partitioned sweeps along x, y, z (0: the
     // built-in choice); set by pllc_sim_autotune
     int lpb[3] = {0, 0, 0};
+    // the batch this context launches over (one simulation by default)
+    Bt bt;
     // optional per-launch hook (phase 0 before, 1 after a launch of the
     // named kernel class), used for HIP-event timing of one kernel class
     void (*hook)(void* user, const char* name, int phase) = nullptr;
@@ -123,6 +141,12 @@ struct Ops {
     // whether planes of ny x nz (multiples of 16, at most 64) run both dense
     // contractions in one launch (k_dct_small) with the kernels set up for it
     static bool dct_small_ready(int ny, int nz);
+    // {del, ca, cb, cc} of field F's lines in direction DIR (host, T)
+    static void line_consts(const Phys& ph, const Dims& d, int F, int DIR, T* k);
+    // whether the fused step of this context has a batched form (FP32,
+    // partitioned line solves in every direction, the small-plane
+    // transform, one-block bulk reduction)
+    static bool batchable(const Ctx<T>& c);
 };
 
 // Tds-style batched lines (entry points pll_tridiag / tridiag_1rhs)

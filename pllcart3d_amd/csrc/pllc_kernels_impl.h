@@ -15,9 +15,47 @@
 #include <utility>
 
 #include "pllc_dev.h"
+#include "pllc_host.h"
 #include "pllc_kernels.h"
 
 namespace pllc {
+
+// The member of a batched launch (Bt) whose block has index idx along the
+// grid dimension of size dim that carries the members (nm equal ranges),
+// and the block's index within its member's range.
+__device__ __forceinline__ int bt_member(const Bt& bt, unsigned idx, unsigned dim, int& local)
+{
+    const int per = (int)(dim / (unsigned)bt.nm);
+    const int m = (int)idx / per;
+    local = (int)idx - m * per;
+    return m;
+}
+// member m's copy of a field pointer (null stays null)
+template <typename T>
+__device__ __forceinline__ T* bt_f(const Bt& bt, T* p, int m)
+{
+    return p ? p + (long)m * bt.fs : p;
+}
+// member m's physics: the launch's own argument, or with B (a batched
+// launch; a compile-time choice, so that the single form keeps its
+// registers) the batch's table
+template <bool B>
+__device__ __forceinline__ const Phys& bt_phys(const Bt& bt, const Phys& own, int m);
+template <>
+__device__ __forceinline__ const Phys& bt_phys<false>(const Bt&, const Phys& own, int)
+{
+    return own;
+}
+template <>
+__device__ __forceinline__ const Phys& bt_phys<true>(const Bt& bt, const Phys&, int m)
+{
+    return bt.ph[m];
+}
+template <typename T>
+__device__ __forceinline__ Visc4<T> bt_mu(const Bt& bt, const Visc4<T>& mu, int m)
+{
+    return {bt_f(bt, mu.cell, m), bt_f(bt, mu.xy, m), bt_f(bt, mu.xz, m), bt_f(bt, mu.yz, m)};
+}
 
 // ------------------------------------------------------------ stale D2
 
@@ -84,20 +122,26 @@ __device__ __forceinline__ int2 xcd_tile()
 
 // concentration's explicit part: Hc_op (1539-1644) and the source loop
 // (573-590).  One thread per interior cell.
-template <typename T, bool FAST>
-__global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph, const T* __restrict__ cold, const T* __restrict__ un,
+template <typename T, bool FAST, bool B = false>
+__global__ __launch_bounds__(256) void k_rhs_c(Dims d, Phys ph0, const T* __restrict__ cold, const T* __restrict__ un,
                                                const T* __restrict__ vn, const T* __restrict__ wn,
                                                const T* __restrict__ Nm1, T* __restrict__ N, T* __restrict__ S,
-                                               const T* __restrict__ stale, const T* __restrict__ ncorr,
-                                               const double* __restrict__ scal, T g, T r)
+                                               const T* __restrict__ stale0, const T* __restrict__ ncorr,
+                                               const double* __restrict__ scal0, T g, T r, Bt bt)
 {
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + 1;
-    const int j = bt.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 3;
+    const int2 tl = xcd_tile();
+    const int k = tl.x * blockDim.x + threadIdx.x + 1;
+    const int j = tl.y * blockDim.y + threadIdx.y + 1;
+    int il;
+    const int m = bt_member(bt, blockIdx.z, gridDim.z, il);
+    const int i = il + 3;
     if (k > d.nz || j > d.ny) return;
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    const double* __restrict__ scal = scal0 + m * S_SIZE;
+    const T* __restrict__ stale = stale0 + m * 16;
     const Rng R = field_rng(d, FC);
-    const long o = IX(d, i, j, k), sx = d.sx, sy = d.sy;
+    // every field of member m at the same offset
+    const long o = IX(d, i, j, k) + m * bt.fs, sx = d.sx, sy = d.sy;
     const T dt = T(scal[S_DT]);
 
     // every operand first (one memory round trip), then the arithmetic
@@ -302,20 +346,25 @@ __device__ __forceinline__ void rhs_mom_point(const Dims& d, const Phys& ph, con
 }
 
 // one component over its range (one thread per point)
-template <typename T, int F, bool FAST>
-__global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
+template <typename T, int F, bool FAST, bool B = false>
+__global__ __launch_bounds__(256) void k_rhs_mom(Dims d, Phys ph0, const T* __restrict__ u, const T* __restrict__ v,
                                                  const T* __restrict__ w, Visc4<T> mu, const T* __restrict__ cn,
                                                  const T* __restrict__ pg, const T* __restrict__ Nm1, T* __restrict__ N,
                                                  T* __restrict__ S, const T* __restrict__ stale,
-                                                 const double* __restrict__ scal, T g, T r)
+                                                 const double* __restrict__ scal, T g, T r, Bt bt)
 {
     const Rng R = field_rng(d, F);
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + R.ks;
-    const int j = bt.y * blockDim.y + threadIdx.y + R.js;
-    const int i = blockIdx.z + R.is;
+    const int2 tl = xcd_tile();
+    const int k = tl.x * blockDim.x + threadIdx.x + R.ks;
+    const int j = tl.y * blockDim.y + threadIdx.y + R.js;
+    int il;
+    const int m = bt_member(bt, blockIdx.z, gridDim.z, il);
+    const int i = il + R.is;
     if (k > R.ke || j > R.je) return;
-    rhs_mom_point<T, F, FAST>(d, ph, R, i, j, k, u, v, w, mu, cn, pg, Nm1, N, S, stale, T(scal[S_DT]), g, r);
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    rhs_mom_point<T, F, FAST>(d, ph, R, i, j, k, bt_f(bt, u, m), bt_f(bt, v, m), bt_f(bt, w, m), bt_mu(bt, mu, m),
+                              bt_f(bt, cn, m), bt_f(bt, pg, m), bt_f(bt, Nm1, m), bt_f(bt, N, m), bt_f(bt, S, m),
+                              stale + m * 16, T(scal[m * S_SIZE + S_DT]), g, r);
 }
 
 // --------------------------------------------------------- line solvers
@@ -355,7 +404,31 @@ struct LineSet {
     const T* Nm1;
     int ghost_c;          // also fill c_new's y/z ghosts (700-712)
     int ghost_v;          // 0, or FU/FV/FW: also fill that component's ghosts (velo_ghost_point)
+    int F;                // the field (FC, FU, FV, FW): its constants in a batch's table
 };
+
+// member m's copy of a line set in a batched launch: the fields moved to
+// the member's, the constant coefficients from the batch's table
+// [m][F][DIR][del, ca, cb, cc] (the host's values for that member)
+template <typename T, int DIR>
+__device__ __forceinline__ LineSet<T> bt_lines(const Bt& bt, const LineSet<T>& L0, int m)
+{
+    LineSet<T> L = L0;
+    if (bt.nm == 1) return L;
+    L.S = bt_f(bt, L.S, m);
+    L.CC = bt_f(bt, L.CC, m);
+    L.mu = bt_f(bt, L.mu, m);
+    L.qold = bt_f(bt, L.qold, m);
+    L.qnew = bt_f(bt, L.qnew, m);
+    L.N = bt_f(bt, L.N, m);
+    L.Nm1 = bt_f(bt, L.Nm1, m);
+    const T* k = reinterpret_cast<const T*>(bt.lc) + ((m * 4 + L.F) * 3 + DIR) * 4;
+    L.del = k[0];
+    L.ca = k[1];
+    L.cb = k[2];
+    L.cc = k[3];
+    return L;
+}
 
 template <typename T>
 struct LineSets {
@@ -678,10 +751,18 @@ __global__ __launch_bounds__(ZL) void k_sweep_z_lds(Dims d, LineSets<T> sets, co
 // XCD a block runs on (its linear id mod 8) depends only on blockIdx.x, and
 // XCD x takes chunks x nb/8 .. (x+1) nb/8 - 1 of every plane: the previous
 // plane's c (o - sx) was read into the same XCD's L2.
-template <typename T>
-__global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph, const T* __restrict__ cn, T* __restrict__ cell,
-                                              T* __restrict__ xy, T* __restrict__ xz, T* __restrict__ yz)
+template <typename T, bool B = false>
+__global__ __launch_bounds__(256) void k_visc(Dims d, Phys ph0, const T* __restrict__ cn, T* __restrict__ cell,
+                                              T* __restrict__ xy, T* __restrict__ xz, T* __restrict__ yz, Bt bt)
 {
+    // member: blockIdx.z
+    const int m = blockIdx.z;
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    cn = bt_f(bt, cn, m);
+    cell = bt_f(bt, cell, m);
+    xy = bt_f(bt, xy, m);
+    xz = bt_f(bt, xz, m);
+    yz = bt_f(bt, yz, m);
     const int nb8 = gridDim.x >> 3;
     const int chunk = (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
     const int l = chunk * 256 + threadIdx.x;
@@ -766,17 +847,22 @@ __global__ void k_phi_ghost(Dims d, T* __restrict__ phi)
 
 // divergence (1983-1999).  Writes div (if non-null) and/or the scaled
 // Poisson source Re/(dt*alpha)*div packed as [x][y][z] (2018-2024).
-template <typename T>
-__global__ __launch_bounds__(256) void k_div(Dims d, Phys ph, const T* __restrict__ u, const T* __restrict__ v,
+template <typename T, bool B = false>
+__global__ __launch_bounds__(256) void k_div(Dims d, Phys ph0, const T* __restrict__ u, const T* __restrict__ v,
                                              const T* __restrict__ w, T* __restrict__ div, T* __restrict__ P,
-                                             const double* __restrict__ scal, T g, T r)
+                                             const double* __restrict__ scal0, T g, T r, Bt bt)
 {
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + 1;
-    const int j = bt.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 3;
+    const int2 tl = xcd_tile();
+    const int k = tl.x * blockDim.x + threadIdx.x + 1;
+    const int j = tl.y * blockDim.y + threadIdx.y + 1;
+    int il;
+    const int m = bt_member(bt, blockIdx.z, gridDim.z, il);
+    const int i = il + 3;
     if (k > d.nz || j > d.ny) return;
-    const long o = IX(d, i, j, k);
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    const double* __restrict__ scal = scal0 + m * S_SIZE;
+    if (P) P += m * bt.pk;
+    const long o = IX(d, i, j, k) + m * bt.fs;
     const T dvx = divh<T>(u[o + d.sx] - u[o], T(d.dx), d.rdx);
     const T dvy = divh<T>(v[o + d.sy] - v[o], T(d.dy), d.rdy);
     const T dvz = divh<T>(w[o + 1] - w[o], T(d.dz), d.rdz);
@@ -1000,9 +1086,12 @@ __global__ __launch_bounds__(256) void k_velo_corr(Dims d, Phys ph, const T* __r
 // sequential); pass 2: one block sums the row integrals over j, forms the
 // deficit (2216) and adds it to the outlet face (2219-2223).
 template <typename T>
-__global__ __launch_bounds__(64) void k_flux_rows(Dims d, const T* __restrict__ u, double* __restrict__ part)
+__global__ __launch_bounds__(64) void k_flux_rows(Dims d, const T* __restrict__ u, double* __restrict__ part, Bt bt)
 {
     __shared__ double ri[1056], ro[1056];
+    // member: blockIdx.y
+    u = bt_f(bt, u, blockIdx.y);
+    part += blockIdx.y * bt.np;
     const int j = 1 + blockIdx.x;
     const double hz = d.dz;
     const long bi = IX(d, 3, j, 0), bo = IX(d, d.nx + 3, j, 0);
@@ -1024,12 +1113,18 @@ __global__ __launch_bounds__(64) void k_flux_rows(Dims d, const T* __restrict__ 
 // reference's order, forms the deficit and adds it to row j of the outlet
 // face; every block forms the same value, and block 0 also stores qin,
 // qout and the deficit.
-template <typename T>
-__global__ __launch_bounds__(128) void k_flux_apply(Dims d, Phys ph, T* __restrict__ u, double* __restrict__ scal,
-                                                    const double* __restrict__ part, int rk)
+template <typename T, bool B = false>
+__global__ __launch_bounds__(128) void k_flux_apply(Dims d, Phys ph0, T* __restrict__ u, double* __restrict__ scal,
+                                                    const double* __restrict__ part, int rk, Bt bt)
 {
     __shared__ double qk_in[1056], qk_out[1056];
     __shared__ double def;
+    // member: blockIdx.y
+    const int m = blockIdx.y;
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    u = bt_f(bt, u, m);
+    scal += m * S_SIZE;
+    part += m * bt.np;
     const double hy = d.dy;
     for (int j = 1 + threadIdx.x; j <= d.ny; j += blockDim.x) { qk_in[j] = part[j]; qk_out[j] = part[1056 + j]; }
     __syncthreads();
@@ -1060,18 +1155,23 @@ __global__ __launch_bounds__(128) void k_flux_apply(Dims d, Phys ph, T* __restri
 // substage in one pass: both use the same phi gradient g0 at a velocity
 // point, and pg reads neither u, v nor w, so updating it here, before the
 // outflow deficit and the ghost copies, gives the reference's values.
-template <typename T>
-__global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph, Visc4<T> mu, const T* __restrict__ phi,
+template <typename T, bool B = false>
+__global__ __launch_bounds__(256) void k_corr_pg(Dims d, Phys ph0, Visc4<T> mu, const T* __restrict__ phi,
                                                  T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
                                                  T* __restrict__ pgu, T* __restrict__ pgv, T* __restrict__ pgw,
-                                                 const double* __restrict__ scal, T g, T r, int vghost)
+                                                 const double* __restrict__ scal0, T g, T r, int vghost, Bt bt)
 {
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + 1;
-    const int j = bt.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 3;
+    const int2 tl = xcd_tile();
+    const int k = tl.x * blockDim.x + threadIdx.x + 1;
+    const int j = tl.y * blockDim.y + threadIdx.y + 1;
+    int il;
+    const int m = bt_member(bt, blockIdx.z, gridDim.z, il);
+    const int i = il + 3;
     if (k > d.nz || j > d.ny) return;
-    const long o = IX(d, i, j, k);
+    const Phys& ph = bt_phys<B>(bt, ph0, m);
+    const double* __restrict__ scal = scal0 + m * S_SIZE;
+    // every field of member m at the same offset
+    const long o = IX(d, i, j, k) + m * bt.fs;
     const T fac = T(0.5) * T(scal[S_DT]) * (g + r);
     const T facv = T(scal[S_DT]) * (g + r) / T(ph.Re);
     // the three components' operands are loaded before any update, for
@@ -1127,17 +1227,19 @@ __global__ __launch_bounds__(256) void k_pg(Dims d, Phys ph, Visc4<T> mu, const 
 // |u|/dx + |v|/dy + |w|/dz at cell centres.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict__ u, const T* __restrict__ v,
-                                                     const T* __restrict__ w, double* __restrict__ part)
+                                                     const T* __restrict__ w, double* __restrict__ part, Bt bt)
 {
     __shared__ double red[256];
-    const int2 bt = xcd_tile();
-    const int k = bt.x * blockDim.x + threadIdx.x + 1;
-    const int j = bt.y * blockDim.y + threadIdx.y + 1;
-    const int i = blockIdx.z + 3;
+    const int2 tl = xcd_tile();
+    const int k = tl.x * blockDim.x + threadIdx.x + 1;
+    const int j = tl.y * blockDim.y + threadIdx.y + 1;
+    int il;
+    const int mb = bt_member(bt, blockIdx.z, gridDim.z, il);
+    const int i = il + 3;
     const int t = threadIdx.y * blockDim.x + threadIdx.x;
     double m = 0.0;
     if (k <= d.nz && j <= d.ny) {
-        const long o = IX(d, i, j, k);
+        const long o = IX(d, i, j, k) + mb * bt.fs;
         const double uv = 0.5 * ((double)u[o] + (double)u[o + d.sx]);
         const double vv = 0.5 * ((double)v[o] + (double)v[o + d.sy]);
         const double wv = 0.5 * ((double)w[o] + (double)w[o + 1]);
@@ -1149,7 +1251,7 @@ __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict
         if (t < s) red[t] = red[t] > red[t + s] ? red[t] : red[t + s];
         __syncthreads();
     }
-    if (t == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+    if (t == 0) part[mb * bt.np + ((long)il * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
 }
 
 // pass 2: max over the partials; dt = min(CFLmax/max, Timestep)
@@ -1158,10 +1260,17 @@ __global__ __launch_bounds__(256) void k_cfl_partial(Dims d, const T* __restrict
 // a max in any order is the same value)
 template <typename T>
 __global__ __launch_bounds__(1024) void k_cfl_final(const double* __restrict__ part, long np, double CFLmax,
-                                                    double Timestep, double* __restrict__ scal)
+                                                    double Timestep, double* __restrict__ scal, Bt bt)
 {
     constexpr int NT = 1024, U = 8;
     __shared__ double red[NT];
+    // member: blockIdx.x
+    if (bt.ph) {
+        CFLmax = bt.ph[blockIdx.x].CFLmax;
+        Timestep = bt.ph[blockIdx.x].Timestep;
+    }
+    part += blockIdx.x * bt.np;
+    scal += blockIdx.x * S_SIZE;
     const int t = threadIdx.x;
     double m = 0.0;
     long l = t;
@@ -1258,10 +1367,22 @@ __global__ __launch_bounds__(1024) void k_bulk_stale(Dims d, const T* __restrict
                                                      int rk, double alpha, double* __restrict__ tval,
                                                      const T* __restrict__ q0, const T* __restrict__ q1,
                                                      const T* __restrict__ q2, const T* __restrict__ q3,
-                                                     T* __restrict__ stale)
+                                                     T* __restrict__ stale, Bt bt)
 {
     constexpr int NT = 1024, U = 8;
     const int t = threadIdx.x;
+    // member: blockIdx.y
+    {
+        const int mb = blockIdx.y;
+        u = bt_f(bt, u, mb);
+        q0 = bt_f(bt, q0, mb);
+        q1 = bt_f(bt, q1, mb);
+        q2 = bt_f(bt, q2, mb);
+        q3 = bt_f(bt, q3, mb);
+        scal += mb * S_SIZE;
+        if (tval) tval += mb;
+        if (stale) stale += mb * 16;
+    }
     if (blockIdx.x == 1) {
         if (t < 16) stale[t] = stale_value<T>(d, q0, q1, q2, q3, t);
         return;
@@ -1486,8 +1607,13 @@ constexpr int DS_QP = 16;
 
 template <typename T, bool INV>
 __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __restrict__ in, T* __restrict__ out,
-                                                   const T* __restrict__ Cz, const T* __restrict__ Cy, PhiOut<T> po)
+                                                   const T* __restrict__ Cz, const T* __restrict__ Cy, PhiOut<T> po,
+                                                   Bt bt)
 {
+    // member: blockIdx.y (packed blocks pk apart, phi fs apart)
+    in += blockIdx.y * bt.pk;
+    out += blockIdx.y * bt.pk;
+    po.phi = bt_f(bt, po.phi, blockIdx.y);
     extern __shared__ unsigned char pllc_smem[];
     typedef typename Mma<T>::acc acc_t;
     constexpr int KM = DS_MAX / 4;             // k steps of 4 at most
@@ -1536,6 +1662,100 @@ __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __re
         const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
         if (INV && po.phi) phi_store(po, x, gm, gn, fdiv<T>(acc[r], norm));
         else out[x * plane + (long)gm * nz + gn] = acc[r];
+    }
+}
+
+// The same transforms for the many planes of a batch (Bt): persistent
+// blocks stage Cz and Cy in LDS once and take planes in turn (plane p of the
+// batch's packed blocks is member p / nx's plane p % nx), each plane staged
+// in LDS and contracted along z into Q, then along y; a wave makes the
+// 16 x 16 tiles w, w + 4, ... of both contractions, its tiles' accumulators
+// interleaved over k.  Each tile's MFMA sequence is k_gemm's, so the result
+// equals k_dct_small's (and the two-launch path's) bit for bit.
+template <typename T, bool INV>
+__global__ __launch_bounds__(256) void k_dct_planes(int ny, int nz, int nx, int nplanes, const T* __restrict__ in,
+                                                    T* __restrict__ out, const T* __restrict__ Cz,
+                                                    const T* __restrict__ Cy, PhiOut<T> po, Bt bt)
+{
+    extern __shared__ unsigned char pllc_smem[];
+    typedef typename Mma<T>::acc acc_t;
+    constexpr int TW = (DS_MAX / 16) * (DS_MAX / 16) / 4;   // tiles per wave at most
+    const int pz = ds_pitch(nz), py = ds_pitch(ny);
+    T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][pz]
+    T* Q = P + ny * pz;                        // [ny][pz]
+    T* Z = Q + ny * pz;                        // Cz [nz][pz]
+    T* Y = Z + nz * pz;                        // Cy [ny][py]
+    for (int e = threadIdx.x; e < nz * nz; e += 256) {
+        const int r = e / nz;
+        Z[r * pz + e - r * nz] = Cz[e];
+    }
+    for (int e = threadIdx.x; e < ny * ny; e += 256) {
+        const int r = e / ny;
+        Y[r * py + e - r * ny] = Cy[e];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int tz = nz >> 4, nt = (ny >> 4) * tz;
+    const long plane = (long)ny * nz;
+    const T norm = T(4 * po.ny * po.nz);
+    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {
+        __syncthreads();   // the tables are staged, the previous plane's P and Q read
+        const T* __restrict__ src = in + p * plane;
+        for (int e = threadIdx.x; e < ny * nz; e += 256) {
+            const int y = e / nz;
+            P[y * pz + e - y * nz] = src[e];
+        }
+        __syncthreads();
+        acc_t acc[TW];
+#pragma unroll
+        for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
+        for (int k = 0; k < nz; k += 4) {
+#pragma unroll
+            for (int u = 0; u < TW; u++) {
+                const int t = wave + 4 * u;
+                if (t < nt) {
+                    const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+                    acc[u] = Mma<T>::run(P[(y0 + li) * pz + k + lk], Z[(z0 + li) * pz + k + lk], acc[u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TW; u++) {
+            const int t = wave + 4 * u;
+            if (t < nt) {
+                const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+#pragma unroll
+                for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * pz + z0 + li] = acc[u][r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
+        for (int k = 0; k < ny; k += 4) {
+#pragma unroll
+            for (int u = 0; u < TW; u++) {
+                const int t = wave + 4 * u;
+                if (t < nt) {
+                    const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+                    acc[u] = Mma<T>::run(Y[(y0 + li) * py + k + lk], Q[(k + lk) * pz + z0 + li], acc[u]);
+                }
+            }
+        }
+        const int m = p / nx, x = p - m * nx;
+        PhiOut<T> pm = po;
+        pm.phi = bt_f(bt, po.phi, m);
+#pragma unroll
+        for (int u = 0; u < TW; u++) {
+            const int t = wave + 4 * u;
+            if (t >= nt) continue;
+            const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
+                if (INV && po.phi) phi_store(pm, x, gm, gn, fdiv<T>(acc[u][r], norm));
+                else out[p * plane + (long)gm * nz + gn] = acc[u][r];
+            }
+        }
     }
 }
 
@@ -2196,16 +2416,16 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
 
 #define PLLC_INST_RHS(T, FAST)                                                                                 \
     template __global__ void k_rhs_c<T, FAST>(Dims, Phys, const T*, const T*, const T*, const T*, const T*, T*,  \
-                                              T*, const T*, const T*, const double*, T, T);                     \
+                                              T*, const T*, const T*, const double*, T, T, Bt);                 \
     template __global__ void k_rhs_mom<T, FU, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
                                                     const T*, const T*, const T*, T*, T*, const T*,             \
-                                                    const double*, T, T);                                       \
+                                                    const double*, T, T, Bt);                                       \
     template __global__ void k_rhs_mom<T, FV, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
                                                     const T*, const T*, const T*, T*, T*, const T*,             \
-                                                    const double*, T, T);                                       \
+                                                    const double*, T, T, Bt);                                       \
     template __global__ void k_rhs_mom<T, FW, FAST>(Dims, Phys, const T*, const T*, const T*, Visc4<T>,        \
                                                     const T*, const T*, const T*, T*, T*, const T*,             \
-                                                    const double*, T, T);
+                                                    const double*, T, T, Bt);
 
 #define PLLC_INST(T)                                                                                           \
     template __global__ void k_stale<T>(Dims, const T*, const T*, const T*, const T*, T*);                      \
@@ -2215,25 +2435,30 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_sweep<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep<T, 2>(Dims, LineSets<T>, const double*, int, T, T);                      \
     template __global__ void k_sweep_z_lds<T>(Dims, LineSets<T>, const double*, T, T);                         \
-    template __global__ void k_sweep_part<T, 0>(Dims, LineSets<T>, const double*, int, T, T);                 \
-    template __global__ void k_sweep_part<T, 1>(Dims, LineSets<T>, const double*, int, T, T);                 \
+    template __global__ void k_sweep_part<T, 0>(Dims, LineSets<T>, const double*, int, T, T, Bt);             \
+    template __global__ void k_sweep_part<T, 1>(Dims, LineSets<T>, const double*, int, T, T, Bt);             \
     template __global__ void k_sweep_part_z<T>(Dims, LineSets<T>, const double*, T, T, int, int, int, int,     \
                                                int);                                                            \
-    template __global__ void k_sweep_part_zt<T>(Dims, LineSets<T>, const double*, T, T, int);                  \
-    template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*);                                  \
+    template __global__ void k_sweep_part_zt<T>(Dims, LineSets<T>, const double*, T, T, int, Bt);              \
+    template __global__ void k_visc<T>(Dims, Phys, const T*, T*, T*, T*, T*, Bt);                              \
     template __global__ void k_velo_ghost<T>(Dims, T*, T*, T*);                                                \
     template __global__ void k_phi_ghost<T>(Dims, T*);                                                         \
-    template __global__ void k_div<T>(Dims, Phys, const T*, const T*, const T*, T*, T*, const double*, T, T);  \
+    template __global__ void k_div<T>(Dims, Phys, const T*, const T*, const T*, T*, T*, const double*, T, T,   \
+                                     Bt);                                                                   \
     template __global__ void k_pack_div<T>(Dims, Phys, const T*, T*, const double*, T, T);                     \
     template __global__ void k_pois_x<T>(Dims, const T*, const T*, T*, T*, const double*);                     \
     template __global__ void k_velo_corr<T>(Dims, Phys, const T*, T*, T*, T*, const double*, T, T);            \
-    template __global__ void k_flux_rows<T>(Dims, const T*, double*);                                          \
-    template __global__ void k_flux_apply<T>(Dims, Phys, T*, double*, const double*, int);                     \
+    template __global__ void k_flux_rows<T>(Dims, const T*, double*, Bt);                                      \
+    template __global__ void k_flux_apply<T>(Dims, Phys, T*, double*, const double*, int, Bt);                 \
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
-    template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*);                    \
+    template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*, Bt);                \
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
-    template __global__ void k_dct_small<T, false>(int, int, const T*, T*, const T*, const T*, PhiOut<T>);     \
-    template __global__ void k_dct_small<T, true>(int, int, const T*, T*, const T*, const T*, PhiOut<T>);      \
+    template __global__ void k_dct_small<T, false>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt); \
+    template __global__ void k_dct_small<T, true>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt);  \
+    template __global__ void k_dct_planes<T, false>(int, int, int, int, const T*, T*, const T*, const T*,      \
+                                                    PhiOut<T>, Bt);                                            \
+    template __global__ void k_dct_planes<T, true>(int, int, int, int, const T*, T*, const T*, const T*,       \
+                                                   PhiOut<T>, Bt);                                             \
     template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long,          \
                                              PhiOut<T>);                                                        \
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long,         \
@@ -2449,13 +2674,15 @@ __device__ __forceinline__ void prow(const LineSet<T>& L, int x, int n, T fac, T
 // P chunks); a wavefront reads one row of consecutive k, coalesced
 template <typename T, int DIR>
 __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, const double* __restrict__ scal,
-                                                    int slot, T g, T r)
+                                                    int slot, T g, T r, Bt bt)
 {
     extern __shared__ unsigned char pllc_smem[];
     T* red = reinterpret_cast<T*>(pllc_smem);
     const int LPB = blockDim.x, P = blockDim.y, E = 2 * P;
     const int q = threadIdx.x, s = threadIdx.y;
-    const LineSet<T>& L = sets.s[blockIdx.y];
+    // member: blockIdx.z
+    const LineSet<T> L = bt_lines<T, DIR>(bt, sets.s[blockIdx.y], blockIdx.z);
+    scal += blockIdx.z * S_SIZE;
     const Rng R = L.R;
     const int n0 = R.ke - R.ks + 1;
     const int n1 = DIR == 0 ? R.je - R.js + 1 : R.ie - R.is + 1;
@@ -2575,17 +2802,21 @@ __global__ __launch_bounds__(512) void k_sweep_part(Dims d, LineSets<T> sets, co
 // w is the LDS row pitch of a line (at least n + 1).
 template <typename T>
 __global__ __launch_bounds__(256) void k_sweep_part_zt(Dims d, LineSets<T> sets, const double* __restrict__ scal, T g,
-                                                      T r, int w)
+                                                      T r, int w, Bt bt)
 {
     extern __shared__ unsigned char pllc_smem[];
     const int LPB = blockDim.x, P = blockDim.y, E = 2 * P, NT = LPB * P;
     const int q = threadIdx.x, s = threadIdx.y, tid = s * LPB + q;
-    const LineSet<T>& L = sets.s[blockIdx.z];
+    // blockIdx.z: member * sets + set
+    int set;
+    const int mb = bt_member(bt, blockIdx.z, gridDim.z, set);
+    const LineSet<T> L = bt_lines<T, 2>(bt, sets.s[set], mb);
+    scal += mb * S_SIZE;
     const Rng R = L.R;
     const int nj = R.je - R.js + 1, n = R.ke - R.ks + 1;
-    const int2 bt = xcd_block_xy();   // (j block, plane)
-    const int i = R.is + bt.y;
-    const int nl = (i <= R.ie) ? min(LPB, nj - (int)(bt.x * LPB)) : 0;
+    const int2 tb = xcd_block_xy();   // (j block, plane)
+    const int i = R.is + tb.y;
+    const int nl = (i <= R.ie) ? min(LPB, nj - (int)(tb.x * LPB)) : 0;
     if (nl <= 0) return;   // whole block
     // tiles tS[LPB][w], tM[LPB][w] (w >= n + 1: rows and the mu row above
     // the last), then the reduced rows
@@ -2595,7 +2826,7 @@ __global__ __launch_bounds__(256) void k_sweep_part_zt(Dims d, LineSets<T> sets,
     T* tS = reinterpret_cast<T*>(pllc_smem);
     T* tM = tS + LPB * w;
     T* red = cst ? tM : tM + LPB * w;
-    const long base = IX(d, i, R.js + bt.x * LPB, R.ks);
+    const long base = IX(d, i, R.js + tb.x * LPB, R.ks);
     const T* __restrict__ mu = cst ? nullptr : L.mu + L.mu_off;
     const T del = L.del, h = L.h;
     const double rh = L.rh;
@@ -2895,9 +3126,11 @@ __device__ __forceinline__ PChunk<double>& pick_chunk(PChunk<T>& a, PChunk<doubl
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_pois_part(Dims d, const T* __restrict__ lam_y, const T* __restrict__ lam_z,
-                                                   T* __restrict__ P, const double* __restrict__ lam00)
+                                                   T* __restrict__ P, const double* __restrict__ lam00, Bt bt)
 {
     extern __shared__ __align__(8) unsigned char pllc_smem[];
+    // member: blockIdx.y
+    P += blockIdx.y * bt.pk;
     const int LPB = blockDim.x, PC = blockDim.y, E = 2 * PC;
     double* red0 = reinterpret_cast<double*>(pllc_smem);          // [3][E] of the (0,0) line
     T* red = reinterpret_cast<T*>(pllc_smem + sizeof(double) * 3 * E);   // [3][E][LPB]
@@ -3007,6 +3240,16 @@ static inline bool fast_stencils(const Ctx<T>& c) { return c.line_solver == 0; }
 // 256-thread tiles of one i-plane: 128 x 2 (k, j) when a k row has 96 or
 // more points (measured faster than 64 x 4, 32 x 8 and 256 x 1 for the
 // cell kernels at 128-point rows, mostly in FP32), 64 x 4 for shorter rows
+// launchers without a batched form: false (and the error set) when the
+// context launches over a batch
+template <typename T>
+static bool single(const Ctx<T>& c, const char* what)
+{
+    if (c.bt.nm == 1) return true;
+    pllc_set_error(what, "no batched form of this operator");
+    return false;
+}
+
 static inline dim3 cell_grid(const Rng& R, dim3& blk)
 {
     const int nk = R.ke - R.ks + 1, bx = nk >= 96 ? 128 : 64, by = 256 / bx;
@@ -3014,6 +3257,13 @@ static inline dim3 cell_grid(const Rng& R, dim3& blk)
     return dim3((nk + bx - 1) / bx, (R.je - R.js + 1 + by - 1) / by, R.ie - R.is + 1);
 }
 static inline dim3 line_grid(long n) { return dim3((unsigned)((n + 255) / 256)); }
+// a cell grid (planes along z) over the context's batch: nm times the planes
+template <typename T>
+static inline dim3 bgrid(dim3 g, const Ctx<T>& c)
+{
+    g.z *= (unsigned)c.bt.nm;
+    return g;
+}
 static inline int bulk_grid(long n) { long b = (n + 255) / 256; return (int)(b > 4096 ? 4096 : (b < 1 ? 1 : b)); }
 
 template <typename T>
@@ -3029,6 +3279,26 @@ void Ops<T>::stale(Ctx<T>& c, const T* qc, const T* qu, const T* qv, const T* qw
 
 // the LineSet of field F's sweep in direction DIR (coefficient source as in
 // A_conc / A_umom / A_vmom / A_wmom, boundary rows as in bc_kind)
+// the constant coefficients of field F's lines in direction DIR, as
+// A_conc (1411-1440: del = 1/(Pe h^2), ca = cc = del, cb = -2 del) and
+// A_umom/A_vmom/A_wmom (1453-1532: del = 2/(Re h) along the component's own
+// direction, 1/(Re h) across) define them: {del, ca, cb, cc}
+template <typename T>
+void Ops<T>::line_consts(const Phys& ph, const Dims& d, int F, int DIR, T* k)
+{
+    const T h = DIR == 0 ? T(d.dx) : DIR == 1 ? T(d.dy) : T(d.dz);
+    k[0] = k[1] = k[2] = k[3] = T(0);
+    if (F == FC) {
+        const T del = T(1.0) / (T(ph.Pe) * h * h);
+        k[1] = del;
+        k[3] = del;
+        k[2] = (DIR == 2) ? (-del - del) : -(del + del);
+    } else {
+        const bool normal = (F == FU && DIR == 0) || (F == FV && DIR == 1) || (F == FW && DIR == 2);
+        k[0] = (normal ? T(2.0) : T(1.0)) / (T(ph.Re) * h);
+    }
+}
+
 template <typename T>
 static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T>& mu, const T* qold, T* qnew,
                            T* N, const T* Nm1)
@@ -3038,13 +3308,13 @@ static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T
     const T h = DIR == 0 ? T(c.d.dx) : DIR == 1 ? T(c.d.dy) : T(c.d.dz);
     L.h = h;
     L.rh = DIR == 0 ? c.d.rdx : DIR == 1 ? c.d.rdy : c.d.rdz;
+    T k[4];
+    Ops<T>::line_consts(c.ph, c.d, F, DIR, k);
+    L.del = k[0]; L.ca = k[1]; L.cb = k[2]; L.cc = k[3];
     if (F == FC) {
-        const T del = T(1.0) / (T(c.ph.Pe) * h * h);
-        L.mu = nullptr; L.ca = del; L.cc = del;
-        L.cb = (DIR == 2) ? (-del - del) : -(del + del);
+        L.mu = nullptr;
     } else {
         const bool normal = (F == FU && DIR == 0) || (F == FV && DIR == 1) || (F == FW && DIR == 2);
-        L.del = (normal ? T(2.0) : T(1.0)) / (T(c.ph.Re) * h);
         const long off[3] = {-c.d.sx, -c.d.sy, -1};
         if (normal) { L.mu = mu.cell; L.mu_off = off[DIR]; }
         else if ((F == FU && DIR == 1) || (F == FV && DIR == 0)) L.mu = mu.xy;
@@ -3057,6 +3327,7 @@ static LineSet<T> line_set(Ctx<T>& c, int F, int DIR, T* S, T* CC, const Visc4<T
     L.qold = qold; L.qnew = qnew; L.N = N; L.Nm1 = Nm1;
     L.ghost_c = (F == FC && DIR == 0) ? 1 : 0;
     L.ghost_v = 0;
+    L.F = F;
     return L;
 }
 
@@ -3068,6 +3339,28 @@ static inline int part_chunks(int nmin, int nmax)
     const int P = (nmax + PM - 1) / PM;
     if (P > 64 || nmin / P < 3) return 0;
     return P;
+}
+
+template <typename T>
+bool Ops<T>::batchable(const Ctx<T>& c)
+{
+    if (sizeof(T) != 4 || c.line_solver != 0 || !c.dsmall || c.cbf || c.ncorr) return false;
+    if ((long)c.d.ny * c.d.nz > BULK1_MAX || part_chunks(c.d.nx, c.d.nx) == 0) return false;
+    for (int DIR = 0; DIR < 3; DIR++) {
+        int nmin = 1 << 30, nmax = 0;
+        for (int F = FC; F <= FW; F++) {
+            const Rng R = field_rng(c.d, F);
+            const int n = DIR == 0 ? R.ie - R.is + 1 : DIR == 1 ? R.je - R.js + 1 : R.ke - R.ks + 1;
+            if (F == FC) {
+                if (part_chunks(n, n) == 0) return false;
+                continue;
+            }
+            nmin = std::min(nmin, n);
+            nmax = std::max(nmax, n);
+        }
+        if (part_chunks(nmin, nmax) == 0) return false;
+    }
+    return true;
 }
 
 // y, z then x sweeps of up to three fields at once (one launch per direction)
@@ -3106,12 +3399,12 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
             if (c.lpb[DIR] > 0) LPB = std::min(c.lpb[DIR], std::max(1, 512 / P));   // tuned
             // block count rounded up to a multiple of 8 (XCD order); the
             // blocks past the last line do nothing
-            const dim3 blk(LPB, P, 1), grd((unsigned)(((lines + LPB - 1) / LPB + 7) / 8 * 8), nset, 1);
+            const dim3 blk(LPB, P, 1), grd((unsigned)(((lines + LPB - 1) / LPB + 7) / 8 * 8), nset, c.bt.nm);
             const size_t sm = (size_t)6 * P * LPB * sizeof(T);
             if (DIR == 0)
-                PLLC_L(c, nm[0], (k_sweep_part<T, 0><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+                PLLC_L(c, nm[0], (k_sweep_part<T, 0><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r, c.bt)));
             else
-                PLLC_L(c, nm[1], (k_sweep_part<T, 1><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r)));
+                PLLC_L(c, nm[1], (k_sweep_part<T, 1><<<grd, blk, sm, c.st>>>(c.d, ls, c.scal, slot, g, r, c.bt)));
             continue;
         }
         int nj = 0, ni = 0;
@@ -3142,15 +3435,17 @@ static void adi_sets(Ctx<T>& c, T g, T r, int slot, const int* F, int nset, T* c
             const int nbx = (nj + LPB - 1) / LPB, ntiles = nbx * ni * nset;
             const int mt = any_mu ? 1 : 0;
             if (sizeof(T) == 8) {
+                if (!single(c, "sweep_z")) return;
                 const int G = std::max(1, std::min(ntiles, resident_blocks(k_sweep_part_z<T>, LPB * P, sm)));
                 PLLC_L(c, nm[2], (k_sweep_part_z<T><<<G, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w, mt, nbx, ni,
                                                                           ntiles)));
             } else {
-                const dim3 gz(nbx, ni, nset);
-                PLLC_L(c, nm[2], (k_sweep_part_zt<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w)));
+                const dim3 gz(nbx, ni, nset * c.bt.nm);
+                PLLC_L(c, nm[2], (k_sweep_part_zt<T><<<gz, blk, sm, c.st>>>(c.d, ls, c.scal, g, r, w, c.bt)));
             }
             continue;
         }
+        if (!single(c, "sweep")) return;
         const dim3 grd((unsigned)((lines + 255) / 256), nset, 1);
         if (DIR == 0) PLLC_L(c, nm[0], (k_sweep<T, 0><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
         else if (DIR == 1) PLLC_L(c, nm[1], (k_sweep<T, 1><<<grd, 256, 0, c.st>>>(c.d, ls, c.scal, slot, g, r)));
@@ -3166,15 +3461,22 @@ template <typename T>
 void Ops<T>::concentration(Ctx<T>& c, double g, double r, int slot, const T* un, const T* vn, const T* wn)
 {
     dim3 blk;
-    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
-    if (fast_stencils(c))
-        PLLC_L(c, "rhs_c", (k_rhs_c<T, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
-                                                                     c.f[C_N], c.f[F_SRC], c.stale + 0, c.ncorr, c.scal, T(g),
-                                                                     T(r))));
-    else
-        PLLC_L(c, "rhs_c", (k_rhs_c<T, false><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn, c.f[C_NM1],
-                                                                      c.f[C_N], c.f[F_SRC], c.stale + 0, c.ncorr, c.scal, T(g),
-                                                                      T(r))));
+    const dim3 grd = bgrid(cell_grid(field_rng(c.d, FC), blk), c);
+    auto go = [&](auto bv) {
+        constexpr bool B = decltype(bv)::value;
+        if (fast_stencils(c))
+            PLLC_L(c, "rhs_c", (k_rhs_c<T, true, B><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn,
+                                                                            c.f[C_NM1], c.f[C_N], c.f[F_SRC],
+                                                                            c.stale + 0, c.ncorr, c.scal, T(g), T(r),
+                                                                            c.bt)));
+        else
+            PLLC_L(c, "rhs_c", (k_rhs_c<T, false, B><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[C_OLD], un, vn, wn,
+                                                                             c.f[C_NM1], c.f[C_N], c.f[F_SRC],
+                                                                             c.stale + 0, c.ncorr, c.scal, T(g), T(r),
+                                                                             c.bt)));
+    };
+    if (c.bt.nm > 1) go(std::true_type{});
+    else go(std::false_type{});
     const int F[1] = {FC};
     T* S[1] = {c.f[F_SRC]};
     T* CC[1] = {c.f[F_CC]};
@@ -3190,8 +3492,11 @@ void Ops<T>::viscosity(Ctx<T>& c, const T* cn, T* const mu[4])
 {
     // chunks of 256 points per (j, k) plane, rounded up to a multiple of 8
     const int nb = (int)(((long)(c.d.ny + 2) * (c.d.nz + 2) + 256 * 8 - 1) / (256 * 8)) * 8;
-    PLLC_L(c, "visc", (k_visc<T><<<dim3(nb, c.d.nx + 2, 1), 256, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2],
-                                                                           mu[3])));
+    const dim3 grd(nb, c.d.nx + 2, c.bt.nm);
+    if (c.bt.nm > 1)
+        PLLC_L(c, "visc", (k_visc<T, true><<<grd, 256, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2], mu[3], c.bt)));
+    else
+        PLLC_L(c, "visc", (k_visc<T><<<grd, 256, 0, c.st>>>(c.d, c.ph, cn, mu[0], mu[1], mu[2], mu[3], c.bt)));
 }
 
 template <typename T, int F>
@@ -3200,15 +3505,22 @@ static void rhs_mom(Ctx<T>& c, T g, T r, T* S)
     const int b = F == FU ? U_OLD : F == FV ? V_OLD : W_OLD;
     const char* nm = F == FU ? "rhs_u" : F == FV ? "rhs_v" : "rhs_w";
     dim3 blk;
-    const dim3 grd = cell_grid(field_rng(c.d, F), blk);
-    if (fast_stencils(c))
-        PLLC_L(c, nm, (k_rhs_mom<T, F, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD],
-                                                                     muo(c), c.f[C_NEW], c.f[b + 2], c.f[b + 4],
-                                                                     c.f[b + 3], S, c.stale + 4 * F, c.scal, g, r)));
-    else
-        PLLC_L(c, nm, (k_rhs_mom<T, F, false><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD],
-                                                                      muo(c), c.f[C_NEW], c.f[b + 2], c.f[b + 4],
-                                                                      c.f[b + 3], S, c.stale + 4 * F, c.scal, g, r)));
+    const dim3 grd = bgrid(cell_grid(field_rng(c.d, F), blk), c);
+    auto go = [&](auto bv) {
+        constexpr bool B = decltype(bv)::value;
+        if (fast_stencils(c))
+            PLLC_L(c, nm, (k_rhs_mom<T, F, true, B><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD],
+                                                                            c.f[W_OLD], muo(c), c.f[C_NEW],
+                                                                            c.f[b + 2], c.f[b + 4], c.f[b + 3], S,
+                                                                            c.stale + 4 * F, c.scal, g, r, c.bt)));
+        else
+            PLLC_L(c, nm, (k_rhs_mom<T, F, false, B><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_OLD], c.f[V_OLD],
+                                                                             c.f[W_OLD], muo(c), c.f[C_NEW],
+                                                                             c.f[b + 2], c.f[b + 4], c.f[b + 3], S,
+                                                                             c.stale + 4 * F, c.scal, g, r, c.bt)));
+    };
+    if (c.bt.nm > 1) go(std::true_type{});
+    else go(std::false_type{});
 }
 
 // comps: bit 0 u, bit 1 v, bit 2 w.  With several components their sources
@@ -3260,10 +3572,15 @@ template <typename T>
 void Ops<T>::divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack)
 {
     dim3 blk;
-    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
-    PLLC_L(c, "div", (k_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW],
-                                                       write_div ? c.f[DIV] : nullptr, pack ? c.P : nullptr, c.scal,
-                                                       T(g), T(r))));
+    const dim3 grd = bgrid(cell_grid(field_rng(c.d, FC), blk), c);
+    if (c.bt.nm > 1)
+        PLLC_L(c, "div", (k_div<T, true><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW],
+                                                                 write_div ? c.f[DIV] : nullptr, pack ? c.P : nullptr,
+                                                                 c.scal, T(g), T(r), c.bt)));
+    else
+        PLLC_L(c, "div", (k_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.f[V_NEW], c.f[W_NEW],
+                                                           write_div ? c.f[DIV] : nullptr, pack ? c.P : nullptr, c.scal,
+                                                           T(g), T(r), c.bt)));
 }
 
 // opts kernel f in to sm bytes of dynamic LDS per block; false when the
@@ -3293,12 +3610,21 @@ bool Ops<T>::dct_mfma_ready(int n)
     return false;
 }
 
+// LDS of k_dct_planes: the plane, Q, Cz and Cy
+template <typename T>
+static int dct_planes_lds(int ny, int nz)
+{
+    return (2 * ny * ds_pitch(nz) + nz * ds_pitch(nz) + ny * ds_pitch(ny)) * (int)sizeof(T);
+}
+
 template <typename T>
 bool Ops<T>::dct_small_ready(int ny, int nz)
 {
     if (ny % 16 || nz % 16 || ny > DS_MAX || nz > DS_MAX) return false;
     const int sm = (ny * ds_pitch(nz) + ny * DS_QP) * (int)sizeof(T);
-    return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm);
+    const int smp = dct_planes_lds<T>(ny, nz);
+    return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm) &&
+           lds_optin((const void*)k_dct_planes<T, false>, smp) && lds_optin((const void*)k_dct_planes<T, true>, smp);
 }
 
 template <typename T>
@@ -3367,17 +3693,34 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
         // small planes: both contractions in one launch, one block per 16
         // columns of a plane; each block reads its whole plane, so the
         // result may not overwrite the input (in place: the k_gemm pair)
+        if (c.bt.nm > 1) {
+            // a batch's planes: persistent blocks with the tables in LDS
+            const size_t sm = (size_t)dct_planes_lds<T>(ny, nz);
+            const int np = nx * c.bt.nm;
+            const PhiOut<T> po{inverse ? phi : nullptr, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+            if (inverse) {
+                const int G = std::max(1, std::min(np, resident_blocks(k_dct_planes<T, true>, 256, sm)));
+                PLLC_L(c, "dct", (k_dct_planes<T, true><<<G, 256, sm, c.st>>>(ny, nz, nx, np, in, out, c.Ciz, c.Ciy,
+                                                                               po, c.bt)));
+            } else {
+                const int G = std::max(1, std::min(np, resident_blocks(k_dct_planes<T, false>, 256, sm)));
+                PLLC_L(c, "dct", (k_dct_planes<T, false><<<G, 256, sm, c.st>>>(ny, nz, nx, np, in, out, c.Cfz, c.Cfy,
+                                                                                po, c.bt)));
+            }
+            return;
+        }
         const size_t sm = ((size_t)ny * ds_pitch(nz) + (size_t)ny * DS_QP) * sizeof(T);
-        const unsigned nb = (unsigned)nx * (unsigned)(nz / 16);
+        const dim3 nb((unsigned)nx * (unsigned)(nz / 16), c.bt.nm, 1);
         if (inverse) {
             const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
-            PLLC_L(c, "dct", (k_dct_small<T, true><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Ciz, c.Ciy, po)));
+            PLLC_L(c, "dct", (k_dct_small<T, true><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Ciz, c.Ciy, po, c.bt)));
         } else {
             PLLC_L(c, "dct", (k_dct_small<T, false><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Cfz, c.Cfy,
-                                                                            PhiOut<T>{})));
+                                                                            PhiOut<T>{}, c.bt)));
         }
         return;
     }
+    if (!single(c, "dct2")) return;
     const long R = (long)nx * ny, plane = (long)ny * nz;
     const dim3 gz((nz + 127) / 128, (unsigned)((R + 63) / 64), 1), gy((nz + 127) / 128, (ny + 63) / 64, nx);
     if (!inverse) {
@@ -3402,8 +3745,10 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts)
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
     const long nm = (long)c.d.ny * c.d.nz;
-    if (from_div)
+    if (from_div) {
+        if (!single(c, "poisson")) return;
         PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
+    }
     // the spectrum: in place in P, or in Q for small planes (whose forward
     // transform cannot run in place)
     T* sp = c.dsmall ? c.Q : c.P;
@@ -3412,9 +3757,11 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts)
     if (PC > 0) {
         const int LPB = std::min(64, std::max(1, 256 / PC));
         const size_t sm = sizeof(double) * 3 * 2 * PC + sizeof(T) * 6 * PC * LPB;
-        PLLC_L(c, "pois_x", (k_pois_part<T><<<(unsigned)((nm + LPB - 1) / LPB), dim3(LPB, PC, 1), sm, c.st>>>(
-                                c.d, c.lam_y, c.lam_z, sp, c.lam00)));
+        const dim3 gp((unsigned)((nm + LPB - 1) / LPB), c.bt.nm, 1);
+        PLLC_L(c, "pois_x", (k_pois_part<T><<<gp, dim3(LPB, PC, 1), sm, c.st>>>(c.d, c.lam_y, c.lam_z, sp, c.lam00,
+                                                                               c.bt)));
     } else {
+        if (!single(c, "poisson")) return;
         PLLC_L(c, "pois_x", (k_pois_x<T><<<line_grid(nm), 256, 0, c.st>>>(c.d, c.lam_y, c.lam_z, sp, c.QC, c.lam00)));
     }
     // the inverse transform's last pass writes phi, normalised
@@ -3433,22 +3780,31 @@ void Ops<T>::velo_correction(Ctx<T>& c, double g, double r, int slot)
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    if (!single(c, "velo_correction")) return;
     PLLC_L(c, "velo_corr", (k_velo_corr<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[PHI], c.f[U_NEW], c.f[V_NEW],
                                                                    c.f[W_NEW], c.scal, T(g), T(r))));
-    PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
-    PLLC_L(c, "flux", (k_flux_apply<T><<<c.d.ny, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
+    PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part, c.bt)));
+    PLLC_L(c, "flux", (k_flux_apply<T><<<c.d.ny, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot, c.bt)));
 }
 
 template <typename T>
 void Ops<T>::correction_and_gradient(Ctx<T>& c, double g, double r, int slot, bool ghosts)
 {
     dim3 blk;
-    const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
-    PLLC_L(c, "corr_pg", (k_corr_pg<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_NEW], c.f[V_NEW],
-                                                               c.f[W_NEW], c.f[U_PG], c.f[V_PG], c.f[W_PG], c.scal,
-                                                               T(g), T(r), ghosts ? 1 : 0)));
-    PLLC_L(c, "flux", (k_flux_rows<T><<<c.d.ny, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part)));
-    PLLC_L(c, "flux", (k_flux_apply<T><<<c.d.ny, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot)));
+    const dim3 grd = bgrid(cell_grid(field_rng(c.d, FC), blk), c);
+    const dim3 gf(c.d.ny, c.bt.nm, 1);
+    auto go = [&](auto bv) {
+        constexpr bool B = decltype(bv)::value;
+        PLLC_L(c, "corr_pg", (k_corr_pg<T, B><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_NEW],
+                                                                      c.f[V_NEW], c.f[W_NEW], c.f[U_PG], c.f[V_PG],
+                                                                      c.f[W_PG], c.scal, T(g), T(r), ghosts ? 1 : 0,
+                                                                      c.bt)));
+        PLLC_L(c, "flux", (k_flux_rows<T><<<gf, 64, 0, c.st>>>(c.d, c.f[U_NEW], c.part, c.bt)));
+        PLLC_L(c, "flux", (k_flux_apply<T, B><<<gf, 128, 0, c.st>>>(c.d, c.ph, c.f[U_NEW], c.scal, c.part, slot,
+                                                                     c.bt)));
+    };
+    if (c.bt.nm > 1) go(std::true_type{});
+    else go(std::false_type{});
 }
 
 template <typename T>
@@ -3456,6 +3812,7 @@ void Ops<T>::pressure_gradient(Ctx<T>& c, double g, double r, int mask)
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
+    if (!single(c, "pressure_gradient")) return;
     PLLC_L(c, "pg", (k_pg<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, mun(c), c.f[PHI], c.f[U_PG], c.f[V_PG], c.f[W_PG],
                                                      c.scal, T(g), T(r), mask)));
 }
@@ -3465,9 +3822,11 @@ void Ops<T>::cfl(Ctx<T>& c)
 {
     dim3 blk;
     const dim3 grd = cell_grid(field_rng(c.d, FC), blk);
-    PLLC_L(c, "cfl", (k_cfl_partial<T><<<grd, blk, 0, c.st>>>(c.d, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD], c.part)));
-    PLLC_L(c, "cfl", (k_cfl_final<T><<<1, 1024, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
-                                                          c.ph.Timestep, c.scal)));
+    // partials of member m at m * bt.np (one per block of its planes)
+    PLLC_L(c, "cfl", (k_cfl_partial<T><<<bgrid(grd, c), blk, 0, c.st>>>(c.d, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD],
+                                                                        c.part, c.bt)));
+    PLLC_L(c, "cfl", (k_cfl_final<T><<<c.bt.nm, 1024, 0, c.st>>>(c.part, (long)grd.x * grd.y * grd.z, c.ph.CFLmax,
+                                                                c.ph.Timestep, c.scal, c.bt)));
 }
 
 template <typename T>
@@ -3475,11 +3834,13 @@ void Ops<T>::bulk(Ctx<T>& c, int slot, double alpha, bool advance_t)
 {
     const long np = (long)c.d.ny * c.d.nz;
     if (np <= BULK1_MAX) {
-        PLLC_L(c, "bulk", (k_bulk_stale<T><<<1, 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
-                                                               advance_t ? c.tval : nullptr, nullptr, nullptr,
-                                                               nullptr, nullptr, nullptr)));
+        PLLC_L(c, "bulk", (k_bulk_stale<T><<<dim3(1, c.bt.nm), 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
+                                                                              advance_t ? c.tval : nullptr, nullptr,
+                                                                              nullptr, nullptr, nullptr, nullptr,
+                                                                              c.bt)));
         return;
     }
+    if (!single(c, "bulk")) return;
     const int nb = (int)((np + 255) / 256);
     PLLC_L(c, "bulk", (k_bulk_partial<T><<<nb, 256, 0, c.st>>>(c.d, c.f[U_OLD], c.part)));
     PLLC_L(c, "bulk", (k_bulk_final<T><<<1, 256, 0, c.st>>>(c.part, nb, c.scal, slot, alpha,
@@ -3491,12 +3852,14 @@ void Ops<T>::bulk_stale(Ctx<T>& c, int slot, double alpha, bool advance_t, const
                         const T* qw)
 {
     if ((long)c.d.ny * c.d.nz > BULK1_MAX) {
+        if (!single(c, "bulk_stale")) return;
         bulk(c, slot, alpha, advance_t);
         stale(c, qc, qu, qv, qw);
         return;
     }
-    PLLC_L(c, "bulk", (k_bulk_stale<T><<<2, 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
-                                                           advance_t ? c.tval : nullptr, qc, qu, qv, qw, c.stale)));
+    PLLC_L(c, "bulk", (k_bulk_stale<T><<<dim3(2, c.bt.nm), 1024, 0, c.st>>>(c.d, c.f[U_OLD], c.scal, slot, alpha,
+                                                                          advance_t ? c.tval : nullptr, qc, qu, qv,
+                                                                          qw, c.stale, c.bt)));
 }
 
 template <typename T>

@@ -1114,11 +1114,195 @@ extern "C" int pllc_sim_autotune(pllc_sim* s, int nsteps, int* lpb_out)
 
 // ------------------------------------------------------------ ensembles
 //
+// A batch of ensemble members with equal dims (mixed precision): their
+// fields live in slabs, member i's copy of every array i strides after
+// member 0's, and one launch per kernel of the fused step advances them all
+// (the kernels take the member from a grid dimension, Bt).  The members'
+// own contexts point into the slabs, so their set-up, uploads and downloads
+// are unchanged; `c` is member 0's context with the batch's Bt, whose old/new
+// pointer swaps the members' contexts follow.
+struct ens_batch {
+    std::vector<pllc_sim*> m;
+    Ctx<float> c;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    Phys* phd = nullptr;          // the members' physics
+    float* lcd = nullptr;         // the members' line constants [m][F][DIR][4]
+    hipGraphExec_t gx = nullptr;  // two fused steps of the batch
+    std::vector<void*> gtab;
+    std::vector<void*> owned;
+    bool broken = false;          // the members' tables no longer agree: each steps alone
+};
+
+static void batch_destroy(ens_batch* b)
+{
+    if (!b) return;
+    if (b->gx) (void)hipGraphExecDestroy(b->gx);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    if (b->st) (void)hipStreamDestroy(b->st);
+    for (void* q : b->owned) (void)hipFree(q);
+    delete b;
+}
+
+// the members' contexts onto the slabs (member i at i strides), with the
+// batch context's current pointer table
+static void batch_point_members(ens_batch* b)
+{
+    const Bt& bt = b->c.bt;
+    for (size_t i = 0; i < b->m.size(); i++) {
+        Ctx<float>& mc = b->m[i]->cf;
+        for (int f = 0; f < NFIELDS; f++) mc.f[f] = b->c.f[f] + (long)i * bt.fs;
+        mc.P = b->c.P + (long)i * bt.pk;
+        mc.Q = b->c.Q + (long)i * bt.pk;
+        mc.scal = b->c.scal + (long)i * S_SIZE;
+        mc.part = b->c.part + (long)i * bt.np;
+        mc.tval = b->c.tval + i;
+        mc.stale = b->c.stale + (long)i * 16;
+    }
+}
+
+// a batch of the members ms (equal dims, FP32), or null (error set)
+static ens_batch* batch_create(const std::vector<pllc_sim*>& ms)
+{
+    const char* W = "pllc_ensemble_create";
+    ens_batch* b = new ens_batch();
+    b->m = ms;
+    const int nm = (int)ms.size();
+    const Ctx<float>& c0 = ms[0]->cf;
+    const long fs = (long)npts(ms[0]);
+    const long pk = (long)c0.d.nx * c0.d.ny * c0.d.nz;
+    const long np = c0.npart;
+    auto get = [&](size_t bytes) -> void* {
+        void* q = nullptr;
+        if (!pllc_hip_ok(W, hipMalloc(&q, bytes))) return nullptr;
+        b->owned.push_back(q);
+        if (!pllc_hip_ok(W, hipMemset(q, 0, bytes))) return nullptr;
+        return q;
+    };
+    b->c = c0;
+    bool ok = pllc_hip_ok(W, hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking)) &&
+              pllc_hip_ok(W, hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
+    for (int f = 0; ok && f < NFIELDS; f++) ok = (b->c.f[f] = (float*)get((size_t)nm * fs * 4)) != nullptr;
+    ok = ok && (b->c.P = (float*)get((size_t)nm * pk * 4)) && (b->c.Q = (float*)get((size_t)nm * pk * 4)) &&
+         (b->c.scal = (double*)get((size_t)nm * S_SIZE * 8)) && (b->c.part = (double*)get((size_t)nm * np * 8)) &&
+         (b->c.tval = (double*)get((size_t)nm * 8)) && (b->c.stale = (float*)get((size_t)nm * 16 * 4)) &&
+         (b->phd = (Phys*)get((size_t)nm * sizeof(Phys))) && (b->lcd = (float*)get((size_t)nm * 48 * 4));
+    if (!ok) {
+        batch_destroy(b);
+        return nullptr;
+    }
+    b->c.st = b->st;
+    b->c.QC = nullptr;
+    b->c.bt.nm = nm;
+    b->c.bt.fs = fs;
+    b->c.bt.pk = pk;
+    b->c.bt.np = np;
+    b->c.bt.ph = b->phd;
+    b->c.bt.lc = b->lcd;
+    batch_point_members(b);
+    for (pllc_sim* s : ms) graph_drop(s);
+    return b;
+}
+
+// the members' physics and line constants into the batch's tables (after
+// their set-up, which fixes Timestep and the viscosity law)
+static bool batch_tables(ens_batch* b, const char* W)
+{
+    const int nm = (int)b->m.size();
+    std::vector<Phys> ph(nm);
+    std::vector<float> lc((size_t)nm * 48);
+    for (int i = 0; i < nm; i++) {
+        ph[i] = b->m[i]->cf.ph;
+        for (int F = 0; F < 4; F++)
+            for (int D = 0; D < 3; D++) Ops<float>::line_consts(ph[i], b->m[i]->cf.d, F, D, &lc[((size_t)i * 12 + F * 3 + D) * 4]);
+    }
+    b->c.ph = ph[0];
+    return pllc_hip_ok(W, hipMemcpy(b->phd, ph.data(), nm * sizeof(Phys), hipMemcpyHostToDevice)) &&
+           pllc_hip_ok(W, hipMemcpy(b->lcd, lc.data(), lc.size() * 4, hipMemcpyHostToDevice));
+}
+
+// the batch's pointer table from the members' after they stepped alone;
+// broken when they no longer agree (member i at i strides from member 0)
+static void batch_sync_from_members(ens_batch* b)
+{
+    const Ctx<float>& m0 = b->m[0]->cf;
+    for (int f = 0; f < NFIELDS; f++) b->c.f[f] = m0.f[f];
+    for (size_t i = 1; i < b->m.size(); i++)
+        for (int f = 0; f < NFIELDS; f++)
+            if (b->m[i]->cf.f[f] != m0.f[f] + (long)i * b->c.bt.fs) b->broken = true;
+}
+
+// k (1 or 2) fused steps of the batch on its stream, after the members'
+// queued work and before their later work; false when the members must
+// step alone: an old/new copy pending on one of them (then all of them copy
+// instead of swapping, which leaves the same values, so that their tables
+// stay in step), or the batch's step failed
+static bool batch_steps(ens_batch* b, int k)
+{
+    if (b->broken) return false;
+    bool copy = false;
+    for (pllc_sim* s : b->m) {
+        if (s->cnn || s->cf.hook) {
+            b->broken = true;
+            return false;
+        }
+        copy = copy || s->copy_next;
+    }
+    if (copy) {
+        for (pllc_sim* s : b->m) s->copy_next = true;
+        return false;
+    }
+    for (pllc_sim* s : b->m) {
+        if (hipEventRecord(b->ev, s->st) != hipSuccess || hipStreamWaitEvent(b->st, b->ev, 0) != hipSuccess)
+            return false;
+    }
+    pllc_sim* s0 = b->m[0];
+    std::vector<void*> tab = ptr_table(b->c);
+    if (b->gx && tab != b->gtab) {
+        (void)hipGraphExecDestroy(b->gx);
+        b->gx = nullptr;
+    }
+    bool done = false;
+    if (k == 2 && s0->use_graph) {
+        if (!b->gx) {
+            std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
+            if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                fused_step<float>(s0, b->c);
+                fused_step<float>(s0, b->c);   // six swaps: the table is back to tab
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(b->st, &g);
+                (void)hipGetLastError();
+                hipGraphExec_t gx = nullptr;
+                if (ec == hipSuccess && g && hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) == hipSuccess) {
+                    b->gx = gx;
+                    b->gtab = tab;
+                } else {
+                    (void)hipGetLastError();
+                }
+                if (g) (void)hipGraphDestroy(g);
+            }
+        }
+        if (b->gx && hipGraphLaunch(b->gx, b->st) == hipSuccess) done = true;
+        else (void)hipGetLastError();
+    }
+    if (!done)
+        for (int q = 0; q < k; q++) fused_step<float>(s0, b->c);
+    batch_point_members(b);
+    if (hipEventRecord(b->ev, b->st) != hipSuccess) return false;
+    for (pllc_sim* s : b->m) {
+        if (hipStreamWaitEvent(s->st, b->ev, 0) != hipSuccess) return false;
+        s->new_stale = true;
+    }
+    return hipGetLastError() == hipSuccess;
+}
+
 // Independent realisations (own grid, viscosity ratio, Peclet number, seed)
-// advanced together: every member keeps its own stream and captured
-// two-step graph, and the members' graphs are launched round-robin, pair of
-// steps by pair of steps, so the device runs several members' kernels at
-// once (small members do not fill the chip on their own).
+// advanced together.  In mixed precision the members of equal dims form
+// batches (ens_batch: one launch per kernel for all of them, replayed from
+// a captured two-step graph; PLLC_ENSEMBLE_BATCH=0 keeps every member on
+// its own); the other members keep their own streams and captured two-step
+// graphs, launched as the branches of one graph pair of steps by pair of
+// steps, so the device runs several members' kernels at once.
 struct pllc_ensemble {
     std::vector<pllc_sim*> m;
     hipStream_t st = nullptr;
@@ -1132,11 +1316,15 @@ struct pllc_ensemble {
     hipGraphExec_t gx = nullptr;
     std::vector<std::vector<void*>> gtab;
     hipEvent_t fork = nullptr, after = nullptr;
+    std::vector<ens_batch*> batches;
+    std::vector<int> single;      // members in no batch
+    bool tables = false;          // the batches' tables are filled
 };
 
 extern "C" void pllc_ensemble_destroy(pllc_ensemble* e)
 {
     if (!e) return;
+    for (ens_batch* b : e->batches) batch_destroy(b);
     for (pllc_sim* s : e->m) pllc_sim_destroy(s);
     for (hipEvent_t v : e->done) (void)hipEventDestroy(v);
     if (e->gx) (void)hipGraphExecDestroy(e->gx);
@@ -1167,8 +1355,36 @@ extern "C" pllc_ensemble* pllc_ensemble_create(const pllc_params* p, int n, int 
         if (v) e->done.push_back(v);
     }
     if (!ok) { pllc_ensemble_destroy(e); return nullptr; }
+    // batches of the members with equal dims (mixed precision)
+    const char* eb = getenv("PLLC_ENSEMBLE_BATCH");
+    std::vector<bool> taken(n, false);
+    if (precision == PLLC_PREC_MIXED && !(eb && strcmp(eb, "0") == 0)) {
+        for (int i = 0; i < n; i++) {
+            if (taken[i] || !Ops<float>::batchable(e->m[i]->cf)) continue;
+            std::vector<pllc_sim*> ms{e->m[i]};
+            std::vector<int> ids{i};
+            for (int j = i + 1; j < n; j++) {
+                if (taken[j]) continue;
+                const Dims &a = e->m[i]->cf.d, &d = e->m[j]->cf.d;
+                if (a.nx == d.nx && a.ny == d.ny && a.nz == d.nz && a.dx == d.dx && a.dy == d.dy && a.dz == d.dz &&
+                    e->m[i]->cf.lpb[0] == e->m[j]->cf.lpb[0] && Ops<float>::batchable(e->m[j]->cf)) {
+                    ms.push_back(e->m[j]);
+                    ids.push_back(j);
+                }
+            }
+            if (ms.size() < 2) continue;
+            ens_batch* b = batch_create(ms);
+            if (!b) { pllc_ensemble_destroy(e); return nullptr; }
+            e->batches.push_back(b);
+            for (int j : ids) taken[j] = true;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        if (!taken[i]) e->single.push_back(i);
     return e;
 }
+
+extern "C" int pllc_ensemble_batches(const pllc_ensemble* e) { return (int)e->batches.size(); }
 
 extern "C" int pllc_ensemble_size(const pllc_ensemble* e) { return (int)e->m.size(); }
 
@@ -1182,9 +1398,11 @@ extern "C" pllc_sim* pllc_ensemble_member(pllc_ensemble* e, int i)
 // (nothing launched) when that is not possible
 static bool ensemble_pair_graph(pllc_ensemble* e)
 {
-    if (!e->one_graph) return false;
+    if (!e->one_graph || e->single.empty()) return false;
+    std::vector<pllc_sim*> ms;
+    for (int i : e->single) ms.push_back(e->m[i]);
     std::vector<std::vector<void*>> tab;
-    for (pllc_sim* s : e->m) {
+    for (pllc_sim* s : ms) {
         if (!s->use_graph || s->copy_next || s->cd.hook || s->cf.hook || s->cnn) return false;
         tab.push_back(s->prec ? ptr_table(s->cf) : ptr_table(s->cd));
     }
@@ -1199,8 +1417,8 @@ static bool ensemble_pair_graph(pllc_ensemble* e)
             return false;
         }
         bool ok = hipEventRecord(e->fork, e->st) == hipSuccess;
-        for (size_t i = 0; ok && i < e->m.size(); i++) {
-            pllc_sim* s = e->m[i];
+        for (size_t i = 0; ok && i < ms.size(); i++) {
+            pllc_sim* s = ms[i];
             ok = hipStreamWaitEvent(s->st, e->fork, 0) == hipSuccess;
             if (!ok) break;
             step_any(s);
@@ -1223,8 +1441,8 @@ static bool ensemble_pair_graph(pllc_ensemble* e)
         if (!e->gx) return false;
     }
     // the members' streams start after whatever they have queued
-    for (size_t i = 0; i < e->m.size(); i++) {
-        if (hipEventRecord(e->done[i], e->m[i]->st) != hipSuccess ||
+    for (size_t i = 0; i < ms.size(); i++) {
+        if (hipEventRecord(e->done[i], ms[i]->st) != hipSuccess ||
             hipStreamWaitEvent(e->st, e->done[i], 0) != hipSuccess)
             return false;
     }
@@ -1235,26 +1453,41 @@ static bool ensemble_pair_graph(pllc_ensemble* e)
     }
     // later work on a member's own stream waits for the graph
     if (hipEventRecord(e->after, e->st) != hipSuccess) return false;
-    for (pllc_sim* s : e->m) {
+    for (pllc_sim* s : ms) {
         if (hipStreamWaitEvent(s->st, e->after, 0) != hipSuccess) return false;
         s->new_stale = true;
     }
     return true;
 }
 
-static void ensemble_steps(pllc_ensemble* e, int nsteps)
+static int ensemble_steps(pllc_ensemble* e, int nsteps)
 {
+    const char* W = "pllc_ensemble_run";
+    if (!e->tables && !e->batches.empty()) {
+        // the members' set-up is done by now (their physics is final)
+        for (pllc_sim* s : e->m)
+            if (finish(s, W)) return -1;
+        for (ens_batch* b : e->batches)
+            if (!batch_tables(b, W)) return -1;
+        e->tables = true;
+    }
     for (int done = 0; done < nsteps;) {
         const int k = nsteps - done >= 2 ? 2 : 1;
+        for (ens_batch* b : e->batches) {
+            if (batch_steps(b, k)) continue;
+            for (pllc_sim* s : b->m) steps(s, k);
+            batch_sync_from_members(b);
+        }
         if (!(k == 2 && ensemble_pair_graph(e)))
-            for (pllc_sim* s : e->m) steps(s, k);
+            for (int i : e->single) steps(e->m[i], k);
         done += k;
     }
+    return 0;
 }
 
 extern "C" int pllc_ensemble_run(pllc_ensemble* e, int nsteps)
 {
-    ensemble_steps(e, nsteps);
+    if (ensemble_steps(e, nsteps)) return -1;
     for (pllc_sim* s : e->m)
         if (finish(s, "pllc_ensemble_run")) return -1;
     return 0;
@@ -1265,13 +1498,13 @@ extern "C" int pllc_ensemble_run(pllc_ensemble* e, int nsteps)
 extern "C" int pllc_ensemble_time_steps(pllc_ensemble* e, int warmup, int nsteps, double* ms)
 {
     const char* W = "pllc_ensemble_time_steps";
-    ensemble_steps(e, warmup);
+    if (ensemble_steps(e, warmup)) return -1;
     for (pllc_sim* s : e->m)
         if (finish(s, W)) return -1;
     if (!pllc_hip_ok(W, hipEventRecord(e->e0, e->st))) return -1;
     for (pllc_sim* s : e->m)
         if (!pllc_hip_ok(W, hipStreamWaitEvent(s->st, e->e0, 0))) return -1;
-    ensemble_steps(e, nsteps);
+    if (ensemble_steps(e, nsteps)) return -1;
     for (size_t i = 0; i < e->m.size(); i++) {
         if (!pllc_hip_ok(W, hipEventRecord(e->done[i], e->m[i]->st))) return -1;
         if (!pllc_hip_ok(W, hipStreamWaitEvent(e->st, e->done[i], 0))) return -1;

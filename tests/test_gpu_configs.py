@@ -173,3 +173,52 @@ def test_ensemble_members_at_workload_shapes_equal_standalone(precision):
             assert np.array_equal(m.download(f), s.download(f)), (precision, i, f)
     print("[ensemble] %s members %s x 6 steps equal" % (precision, sorted(seen)), flush=True)
     ens.close()
+
+
+def test_ensemble_batches_equal_standalone():
+    """Mixed precision: members of equal grids are stepped as batches (one
+    launch per kernel for all of them, members taken from a grid dimension).
+    The first 9 members of the 64-member workload (three to four per grid
+    shape, each with its own drawn R, Pe and seed) advanced together equal
+    the same simulations run alone in every bit: 6 steps (graph-replayed
+    pairs), then 1 (eager), then an upload into one member (whose next step
+    copies instead of swapping, so every member of its batch does) and 4
+    more steps."""
+    specs = ensemble_members(9)
+    ens = P.Ensemble([m[0] for m in specs], "mixed")
+    shapes = {}
+    for p, *_ in specs:
+        shapes[(p.nx, p.ny, p.nz)] = shapes.get((p.nx, p.ny, p.nz), 0) + 1
+    nb = ens.batches()
+    print("[ensemble batches] %d batches over %s" % (nb, shapes), flush=True)
+    assert nb == sum(1 for v in shapes.values() if v >= 2)
+    alone = []
+    for (p, ic, seed, amp), m in zip(specs, ens.members):
+        c0 = P.ic_fill(p, ic, seed, amp)
+        m.init(c0)
+        s = P.Sim(p, "mixed")
+        s.init(c0)
+        alone.append(s)
+
+    def check(tag):
+        for i, (m, s) in enumerate(zip(ens.members, alone)):
+            for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "W_PG"):
+                assert np.array_equal(m.download(f), s.download(f)), (tag, i, f)
+        print("[ensemble batches] %s: every member equal" % tag, flush=True)
+
+    ens.run(6)
+    for s in alone:
+        s.run(6)
+    check("6 steps")
+    ens.run(1)
+    for s in alone:
+        s.run(1)
+    check("7 steps")
+    v = alone[0].download("V_OLD") * 0.5
+    ens.members[0].upload("V_OLD", v)
+    alone[0].upload("V_OLD", v)
+    ens.run(4)
+    for s in alone:
+        s.run(4)
+    check("upload + 4 steps")
+    ens.close()
