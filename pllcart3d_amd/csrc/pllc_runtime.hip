@@ -1319,11 +1319,17 @@ struct pllc_ensemble {
     std::vector<ens_batch*> batches;
     std::vector<int> single;      // members in no batch
     bool tables = false;          // the batches' tables are filled
+    // two fused steps of every batch as the branches of one graph (their
+    // pointer tables as captured in btab)
+    hipGraphExec_t bgx = nullptr;
+    std::vector<std::vector<void*>> btab;
+    bool batch_graph = true;
 };
 
 extern "C" void pllc_ensemble_destroy(pllc_ensemble* e)
 {
     if (!e) return;
+    if (e->bgx) (void)hipGraphExecDestroy(e->bgx);
     for (ens_batch* b : e->batches) batch_destroy(b);
     for (pllc_sim* s : e->m) pllc_sim_destroy(s);
     for (hipEvent_t v : e->done) (void)hipEventDestroy(v);
@@ -1381,6 +1387,10 @@ extern "C" pllc_ensemble* pllc_ensemble_create(const pllc_params* p, int n, int 
     }
     for (int i = 0; i < n; i++)
         if (!taken[i]) e->single.push_back(i);
+    // the batch with the most cells first (the first branch of their graph)
+    std::stable_sort(e->batches.begin(), e->batches.end(), [](const ens_batch* a, const ens_batch* b) {
+        return (long)a->m.size() * a->c.bt.pk > (long)b->m.size() * b->c.bt.pk;
+    });
     return e;
 }
 
@@ -1460,6 +1470,73 @@ static bool ensemble_pair_graph(pllc_ensemble* e)
     return true;
 }
 
+// two fused steps of every batch through one graph whose branches are the
+// batches (each on its own stream, forked from and joined to the ensemble
+// stream), so that the small batches' kernels run beside the large ones';
+// false (nothing launched) when that is not possible
+static bool batches_pair_graph(pllc_ensemble* e)
+{
+    if (!e->batch_graph || e->batches.empty()) return false;
+    std::vector<std::vector<void*>> tab;
+    for (ens_batch* b : e->batches) {
+        if (b->broken || !b->m[0]->use_graph) return false;
+        for (pllc_sim* s : b->m)
+            if (s->copy_next || s->cnn || s->cf.hook) return false;
+        tab.push_back(ptr_table(b->c));
+    }
+    if (e->bgx && tab != e->btab) {
+        (void)hipGraphExecDestroy(e->bgx);
+        e->bgx = nullptr;
+    }
+    // every batch's stream, and with it the graph, after its members' queued work
+    for (ens_batch* b : e->batches)
+        for (pllc_sim* s : b->m)
+            if (hipEventRecord(b->ev, s->st) != hipSuccess || hipStreamWaitEvent(e->st, b->ev, 0) != hipSuccess)
+                return false;
+    if (!e->bgx) {
+        std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
+        if (hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            e->batch_graph = false;
+            return false;
+        }
+        bool ok = hipEventRecord(e->fork, e->st) == hipSuccess;
+        for (ens_batch* b : e->batches) {
+            ok = ok && hipStreamWaitEvent(b->st, e->fork, 0) == hipSuccess;
+            if (!ok) break;
+            fused_step<float>(b->m[0], b->c);
+            fused_step<float>(b->m[0], b->c);   // six swaps: the table is back to tab
+            ok = hipEventRecord(b->ev, b->st) == hipSuccess && hipStreamWaitEvent(e->st, b->ev, 0) == hipSuccess;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->st, &g);
+        (void)hipGetLastError();
+        hipGraphExec_t gx = nullptr;
+        if (ok && ec == hipSuccess && g && hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) == hipSuccess) {
+            e->bgx = gx;
+            e->btab = tab;
+        } else {
+            (void)hipGetLastError();
+            e->batch_graph = false;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (!e->bgx) return false;
+    }
+    if (hipGraphLaunch(e->bgx, e->st) != hipSuccess) {
+        (void)hipGetLastError();
+        e->batch_graph = false;
+        return false;
+    }
+    if (hipEventRecord(e->after, e->st) != hipSuccess) return false;
+    for (ens_batch* b : e->batches) {
+        batch_point_members(b);
+        for (pllc_sim* s : b->m) {
+            if (hipStreamWaitEvent(s->st, e->after, 0) != hipSuccess) return false;
+            s->new_stale = true;
+        }
+    }
+    return true;
+}
+
 static int ensemble_steps(pllc_ensemble* e, int nsteps)
 {
     const char* W = "pllc_ensemble_run";
@@ -1473,10 +1550,12 @@ static int ensemble_steps(pllc_ensemble* e, int nsteps)
     }
     for (int done = 0; done < nsteps;) {
         const int k = nsteps - done >= 2 ? 2 : 1;
-        for (ens_batch* b : e->batches) {
-            if (batch_steps(b, k)) continue;
-            for (pllc_sim* s : b->m) steps(s, k);
-            batch_sync_from_members(b);
+        if (!(k == 2 && batches_pair_graph(e))) {
+            for (ens_batch* b : e->batches) {
+                if (batch_steps(b, k)) continue;
+                for (pllc_sim* s : b->m) steps(s, k);
+                batch_sync_from_members(b);
+            }
         }
         if (!(k == 2 && ensemble_pair_graph(e)))
             for (int i : e->single) steps(e->m[i], k);
