@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_conv(G3 g, const unsigned short* __rest
 // shifted by o), B[p][co] (lane l: cells 4 (l / 16) + q, channel l % 16),
 // accumulator D[ci][co] (lane l: ci = 4 (l / 16) + q, co = l % 16).  The
 // waves of the grid take 16-cell segments in turn; a block's four waves
-// add their sums in LDS.
+// add their sums in LDS, one wave after the other.
 __global__ __launch_bounds__(256) void k_conv_wgrad(G3 g, const unsigned short* __restrict__ a,
                                                     const unsigned short* __restrict__ dz, float* __restrict__ part)
 {
@@ -213,12 +213,24 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(G3 g, const unsigned short* 
             acc[o] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b, acc[o], 0, 0, 0);
         }
     }
+    // the four waves add their sums in turn (a fixed order: the result does
+    // not depend on the waves' timing); a wave's lanes hold distinct weight
+    // entries, and its bias sum of channel m is the four lanes m + 16 kq
+    // added in kq order
+    float dbw = 0.f;
 #pragma unroll
-    for (int o = 0; o < NOFF; o++)
+    for (int k = 0; k < 4; k++) dbw += __shfl(db, m + 16 * k, 64);
+    const int wv = threadIdx.x >> 6;
+    for (int w = 0; w < 4; w++) {
+        if (wv == w) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) atomicAdd(&red[o * 256 + (4 * kq + q) * CH + m], acc[o][q]);
-    atomicAdd(&red[NOFF * 256 + m], db);
-    __syncthreads();
+            for (int o = 0; o < NOFF; o++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) red[o * 256 + (4 * kq + q) * CH + m] += acc[o][q];
+            if (kq == 0) red[NOFF * 256 + m] += dbw;
+        }
+        __syncthreads();
+    }
     float* dst = part + (long)blockIdx.x * (NOFF * 256 + CH);
     for (int x = threadIdx.x; x < NOFF * 256 + CH; x += 256) dst[x] = red[x];
 }

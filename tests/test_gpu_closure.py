@@ -291,3 +291,25 @@ def test_refresh_period_graph_equals_eager(monkeypatch, every):
     assert not np.array_equal(s.download("C_OLD"), out["graph"]), every
     s.close()
     net.close()
+
+
+def test_training_repeats_bit_for_bit():
+    """Two trainings from the same parameters, samples and sample order end
+    with equal parameters in every bit: the weight-gradient sums of a
+    block's waves go in a fixed order."""
+    shape = (8, 6, 20)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=shape + (5,)).astype(np.float32)
+    t = rng.normal(size=shape).astype(np.float32)
+    mask = np.ones(shape, np.uint8)
+    out = []
+    for rep in range(2):
+        net = CL.Network(*shape)
+        net.set_params(CL.flat_from_dict(R.init_params(2)))
+        net.set_norm(np.zeros(5), np.ones(5), 1.0)
+        i = net.add_sample(x, t, mask)
+        losses = net.train(60, [i], 1e-3)
+        out.append(net.get_params())
+        net.close()
+        print("[cnn determinism] run %d: loss %.4e -> %.4e" % (rep, losses[0], losses[-1]), flush=True)
+    assert np.array_equal(out[0], out[1])
