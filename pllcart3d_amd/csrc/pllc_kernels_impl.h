@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 
@@ -1668,94 +1669,134 @@ __global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __re
 // The same transforms for the many planes of a batch (Bt): persistent
 // blocks stage Cz and Cy in LDS once and take planes in turn (plane p of the
 // batch's packed blocks is member p / nx's plane p % nx), each plane staged
-// in LDS and contracted along z into Q, then along y; a wave makes the
-// 16 x 16 tiles w, w + 4, ... of both contractions, its tiles' accumulators
-// interleaved over k.  Each tile's MFMA sequence is k_gemm's, so the result
-// equals k_dct_small's (and the two-launch path's) bit for bit.
-template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_dct_planes(int ny, int nz, int nx, int nplanes, const T* __restrict__ in,
+// in LDS and contracted along z into Q, then along y.  A block's next plane
+// is loaded into registers while it contracts the current one, and written
+// to LDS once the z contraction has read the current one, so HBM latency
+// overlaps the matrix work.  N > 0: square planes ny = nz = N (so Cy = Cz,
+// one table staged), N / 16 waves, wave w making the 16 x 16 tiles of row
+// tile w (their A operand shared), the k loops unrolled; N = 0: any ny, nz (multiples of 16, at most 64), 4
+// waves, wave w making tiles w, w + 4, ...  A wave's tiles' accumulators
+// are interleaved over k.  Each tile's MFMA sequence is k_gemm's (a zero
+// accumulator, k ascending in steps of 4), so the result equals
+// k_dct_small's (and the two-launch path's) bit for bit.
+__host__ __device__ constexpr int dct_planes_threads(int N) { return N ? 4 * N : 256; }
+
+template <typename T, bool INV, int N>
+__global__ __launch_bounds__(256) void k_dct_planes(int ny_, int nz_, int nx, int nplanes, const T* __restrict__ in,
                                                     T* __restrict__ out, const T* __restrict__ Cz,
                                                     const T* __restrict__ Cy, PhiOut<T> po, Bt bt)
 {
+    static_assert(N % 16 == 0 && N <= DS_MAX, "square planes of 16 .. 64");
     extern __shared__ unsigned char pllc_smem[];
     typedef typename Mma<T>::acc acc_t;
-    constexpr int TW = (DS_MAX / 16) * (DS_MAX / 16) / 4;   // tiles per wave at most
+    constexpr int NT = dct_planes_threads(N);
+    constexpr int TW = N ? N / 16 : (DS_MAX / 16) * (DS_MAX / 16) / 4;   // tiles per wave at most
+    constexpr int PF = (N ? N * N : DS_MAX * DS_MAX) / NT;              // plane values per thread at most
+    constexpr int KN = N ? N : DS_MAX;
+    const int ny = N ? N : ny_, nz = N ? N : nz_;
     const int pz = ds_pitch(nz), py = ds_pitch(ny);
     T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][pz]
     T* Q = P + ny * pz;                        // [ny][pz]
     T* Z = Q + ny * pz;                        // Cz [nz][pz]
-    T* Y = Z + nz * pz;                        // Cy [ny][py]
-    for (int e = threadIdx.x; e < nz * nz; e += 256) {
-        const int r = e / nz;
-        Z[r * pz + e - r * nz] = Cz[e];
-    }
-    for (int e = threadIdx.x; e < ny * ny; e += 256) {
-        const int r = e / ny;
-        Y[r * py + e - r * ny] = Cy[e];
-    }
+    T* Y = N ? Z : Z + nz * pz;                // Cy [ny][py] (square planes: Cy = Cz)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int tz = nz >> 4, nt = (ny >> 4) * tz;
-    const long plane = (long)ny * nz;
+    const int plane = ny * nz;
     const T norm = T(4 * po.ny * po.nz);
-    for (int p = blockIdx.x; p < nplanes; p += gridDim.x) {
-        __syncthreads();   // the tables are staged, the previous plane's P and Q read
-        const T* __restrict__ src = in + p * plane;
-        for (int e = threadIdx.x; e < ny * nz; e += 256) {
-            const int y = e / nz;
-            P[y * pz + e - y * nz] = src[e];
+    const int G = gridDim.x;
+    // this wave's tiles: origin (y0, z0) and whether the tile exists
+    int ty0[TW], tz0[TW];
+    bool tok[TW];
+#pragma unroll
+    for (int u = 0; u < TW; u++) {
+        if (N) { ty0[u] = 16 * wave; tz0[u] = 16 * u; tok[u] = true; }
+        else {
+            const int t = wave + 4 * u;
+            tok[u] = t < nt;
+            ty0[u] = (t / tz) * 16;
+            tz0[u] = (t % tz) * 16;
         }
-        __syncthreads();
+    }
+    T pf[PF];
+    auto fetch = [&](int p) {
+        const T* __restrict__ src = in + (long)p * plane;
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + NT * u;
+            if (N || e < plane) pf[u] = src[e];
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int e = threadIdx.x + NT * u;
+            if (N || e < plane) {
+                const int y = e / nz;
+                P[y * pz + e - y * nz] = pf[u];
+            }
+        }
+    };
+    int p = blockIdx.x;
+    if (p < nplanes) fetch(p);
+    for (int e = threadIdx.x; e < nz * nz; e += NT) {
+        const int r = e / nz;
+        Z[r * pz + e - r * nz] = Cz[e];
+    }
+    if (!N)
+        for (int e = threadIdx.x; e < ny * ny; e += NT) {
+            const int r = e / ny;
+            Y[r * py + e - r * ny] = Cy[e];
+        }
+    if (p < nplanes) put();
+    if (p + G < nplanes) fetch(p + G);
+    __syncthreads();
+    for (; p < nplanes; p += G) {
+        // P holds plane p; pf holds plane p + G (in flight)
         acc_t acc[TW];
 #pragma unroll
         for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
-        for (int k = 0; k < nz; k += 4) {
 #pragma unroll
-            for (int u = 0; u < TW; u++) {
-                const int t = wave + 4 * u;
-                if (t < nt) {
-                    const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
-                    acc[u] = Mma<T>::run(P[(y0 + li) * pz + k + lk], Z[(z0 + li) * pz + k + lk], acc[u]);
-                }
-            }
+        for (int k = 0; k < KN; k += 4) {
+            if (!N && k >= nz) break;
+#pragma unroll
+            for (int u = 0; u < TW; u++)
+                if (tok[u])
+                    acc[u] = Mma<T>::run(P[(ty0[u] + li) * pz + k + lk], Z[(tz0[u] + li) * pz + k + lk], acc[u]);
         }
 #pragma unroll
-        for (int u = 0; u < TW; u++) {
-            const int t = wave + 4 * u;
-            if (t < nt) {
-                const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+        for (int u = 0; u < TW; u++)
+            if (tok[u]) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * pz + z0 + li] = acc[u][r];
+                for (int r = 0; r < 4; r++) Q[(ty0[u] + Mma<T>::row(lk, r)) * pz + tz0[u] + li] = acc[u][r];
             }
-        }
-        __syncthreads();
+        __syncthreads();   // P read by every wave, Q written
+        if (p + G < nplanes) put();
+        if (p + 2 * G < nplanes) fetch(p + 2 * G);
 #pragma unroll
         for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
-        for (int k = 0; k < ny; k += 4) {
 #pragma unroll
-            for (int u = 0; u < TW; u++) {
-                const int t = wave + 4 * u;
-                if (t < nt) {
-                    const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
-                    acc[u] = Mma<T>::run(Y[(y0 + li) * py + k + lk], Q[(k + lk) * pz + z0 + li], acc[u]);
-                }
-            }
+        for (int k = 0; k < KN; k += 4) {
+            if (!N && k >= ny) break;
+#pragma unroll
+            for (int u = 0; u < TW; u++)
+                if (tok[u])
+                    acc[u] = Mma<T>::run(Y[(ty0[u] + li) * py + k + lk], Q[(k + lk) * pz + tz0[u] + li], acc[u]);
         }
         const int m = p / nx, x = p - m * nx;
         PhiOut<T> pm = po;
         pm.phi = bt_f(bt, po.phi, m);
 #pragma unroll
         for (int u = 0; u < TW; u++) {
-            const int t = wave + 4 * u;
-            if (t >= nt) continue;
-            const int y0 = (t / tz) * 16, z0 = (t % tz) * 16;
+            if (!tok[u]) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
+                const int gm = ty0[u] + Mma<T>::row(lk, r), gn = tz0[u] + li;
                 if (INV && po.phi) phi_store(pm, x, gm, gn, fdiv<T>(acc[u][r], norm));
-                else out[p * plane + (long)gm * nz + gn] = acc[u][r];
+                else out[(long)p * plane + (long)gm * nz + gn] = acc[u][r];
             }
         }
+        __syncthreads();   // Q read by every wave, P holds plane p + G
     }
 }
 
@@ -2427,6 +2468,12 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
                                                     const T*, const T*, const T*, T*, T*, const T*,             \
                                                     const double*, T, T, Bt);
 
+#define PLLC_INST_DCT_PLANES(T, N)                                                                             \
+    template __global__ void k_dct_planes<T, false, N>(int, int, int, int, const T*, T*, const T*, const T*,   \
+                                                       PhiOut<T>, Bt);                                         \
+    template __global__ void k_dct_planes<T, true, N>(int, int, int, int, const T*, T*, const T*, const T*,    \
+                                                      PhiOut<T>, Bt);
+
 #define PLLC_INST(T)                                                                                           \
     template __global__ void k_stale<T>(Dims, const T*, const T*, const T*, const T*, T*);                      \
     PLLC_INST_RHS(T, true)                                                                                     \
@@ -2455,10 +2502,11 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
     template __global__ void k_dct_small<T, false>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt); \
     template __global__ void k_dct_small<T, true>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt);  \
-    template __global__ void k_dct_planes<T, false>(int, int, int, int, const T*, T*, const T*, const T*,      \
-                                                    PhiOut<T>, Bt);                                            \
-    template __global__ void k_dct_planes<T, true>(int, int, int, int, const T*, T*, const T*, const T*,       \
-                                                   PhiOut<T>, Bt);                                             \
+    PLLC_INST_DCT_PLANES(T, 0)                                                                                 \
+    PLLC_INST_DCT_PLANES(T, 16)                                                                                \
+    PLLC_INST_DCT_PLANES(T, 32)                                                                                \
+    PLLC_INST_DCT_PLANES(T, 48)                                                                                \
+    PLLC_INST_DCT_PLANES(T, 64)                                                                                \
     template __global__ void k_gemm<T, true>(int, int, int, const T*, long, const T*, long, T*, long,          \
                                              PhiOut<T>);                                                        \
     template __global__ void k_gemm<T, false>(int, int, int, const T*, long, const T*, long, T*, long,         \
@@ -3208,16 +3256,16 @@ const char* kernel_class_names =
     } while (0)
 
 // blocks of kernel f (nthreads threads, sm bytes of LDS) resident on the
-// device at once: the occupancy per CU times the CUs (cached per shape,
-// behind a lock: handles may launch from several host threads)
+// device at once: the occupancy per CU times the CUs (cached per kernel
+// and shape, behind a lock: handles may launch from several host threads)
 template <typename K>
 static int resident_blocks(K f, int nthreads, size_t sm)
 {
-    static std::map<std::pair<int, size_t>, int> cache;
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
     static int cus = 0;
     static std::mutex mu;
     std::lock_guard<std::mutex> lk(mu);
-    const auto key = std::make_pair(nthreads, sm);
+    const auto key = std::make_tuple((const void*)f, nthreads, sm);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     if (cus == 0) {
@@ -3610,11 +3658,27 @@ bool Ops<T>::dct_mfma_ready(int n)
     return false;
 }
 
-// LDS of k_dct_planes: the plane, Q, Cz and Cy
+// the k_dct_planes form for (ny, nz): unrolled for square planes
+template <typename T>
+using DctPlanesFn = void (*)(int, int, int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt);
+template <typename T>
+static DctPlanesFn<T> dct_planes_fn(bool inv, int ny, int nz)
+{
+    if (ny == nz) switch (ny) {
+        case 16: return inv ? k_dct_planes<T, true, 16> : k_dct_planes<T, false, 16>;
+        case 32: return inv ? k_dct_planes<T, true, 32> : k_dct_planes<T, false, 32>;
+        case 48: return inv ? k_dct_planes<T, true, 48> : k_dct_planes<T, false, 48>;
+        case 64: return inv ? k_dct_planes<T, true, 64> : k_dct_planes<T, false, 64>;
+        default: break;
+        }
+    return inv ? k_dct_planes<T, true, 0> : k_dct_planes<T, false, 0>;
+}
+
+// LDS of k_dct_planes: the plane, Q, Cz and Cy (one table for square planes)
 template <typename T>
 static int dct_planes_lds(int ny, int nz)
 {
-    return (2 * ny * ds_pitch(nz) + nz * ds_pitch(nz) + ny * ds_pitch(ny)) * (int)sizeof(T);
+    return (2 * ny * ds_pitch(nz) + nz * ds_pitch(nz) + (ny == nz ? 0 : ny * ds_pitch(ny))) * (int)sizeof(T);
 }
 
 template <typename T>
@@ -3624,7 +3688,8 @@ bool Ops<T>::dct_small_ready(int ny, int nz)
     const int sm = (ny * ds_pitch(nz) + ny * DS_QP) * (int)sizeof(T);
     const int smp = dct_planes_lds<T>(ny, nz);
     return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm) &&
-           lds_optin((const void*)k_dct_planes<T, false>, smp) && lds_optin((const void*)k_dct_planes<T, true>, smp);
+           lds_optin((const void*)dct_planes_fn<T>(false, ny, nz), smp) &&
+           lds_optin((const void*)dct_planes_fn<T>(true, ny, nz), smp);
 }
 
 template <typename T>
@@ -3698,15 +3763,12 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
             const size_t sm = (size_t)dct_planes_lds<T>(ny, nz);
             const int np = nx * c.bt.nm;
             const PhiOut<T> po{inverse ? phi : nullptr, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
-            if (inverse) {
-                const int G = std::max(1, std::min(np, resident_blocks(k_dct_planes<T, true>, 256, sm)));
-                PLLC_L(c, "dct", (k_dct_planes<T, true><<<G, 256, sm, c.st>>>(ny, nz, nx, np, in, out, c.Ciz, c.Ciy,
-                                                                               po, c.bt)));
-            } else {
-                const int G = std::max(1, std::min(np, resident_blocks(k_dct_planes<T, false>, 256, sm)));
-                PLLC_L(c, "dct", (k_dct_planes<T, false><<<G, 256, sm, c.st>>>(ny, nz, nx, np, in, out, c.Cfz, c.Cfy,
-                                                                                po, c.bt)));
-            }
+            const T* tz = inverse ? c.Ciz : c.Cfz;
+            const T* ty = inverse ? c.Ciy : c.Cfy;
+            const DctPlanesFn<T> kern = dct_planes_fn<T>(inverse, ny, nz);
+            const int nthr = dct_planes_threads(ny == nz ? ny : 0);
+            const int G = std::max(1, std::min(np, resident_blocks(kern, nthr, sm)));
+            PLLC_L(c, "dct", (kern<<<G, nthr, sm, c.st>>>(ny, nz, nx, np, in, out, tz, ty, po, c.bt)));
             return;
         }
         const size_t sm = ((size_t)ny * ds_pitch(nz) + (size_t)ny * DS_QP) * sizeof(T);

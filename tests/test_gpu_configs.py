@@ -222,3 +222,31 @@ def test_ensemble_batches_equal_standalone():
         s.run(4)
     check("upload + 4 steps")
     ens.close()
+
+
+def test_ensemble_batches_of_other_planes_equal_standalone():
+    """Batches whose (y, z) planes take the other forms of the batched
+    transform: 16 x 32 planes (any ny, nz) and 16 x 16 planes (square, one
+    wave per block), each member with its own R, Pe and seed, equal the
+    same simulations run alone in every bit after 5 steps."""
+    grids = [(24, 16, 32)] * 3 + [(20, 16, 16)] * 2
+    specs = []
+    for i, (nx, ny, nz) in enumerate(grids):
+        prm = P.make_params(nx, ny, nz, Vratio=1.5 + 0.5 * i, Pe=300.0 + 200.0 * i, Inf_thick=0.05)
+        specs.append((prm, P.IC_NOISE, 700 + i, 0.01))
+    ens = P.Ensemble([m[0] for m in specs], "mixed")
+    assert ens.batches() == 2
+    alone = []
+    for (p, ic, seed, amp), m in zip(specs, ens.members):
+        c0 = P.ic_fill(p, ic, seed, amp)
+        m.init(c0)
+        s = P.Sim(p, "mixed")
+        s.init(c0)
+        alone.append(s)
+    ens.run(5)
+    for s in alone:
+        s.run(5)
+    for i, (m, s) in enumerate(zip(ens.members, alone)):
+        for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "W_PG"):
+            assert np.array_equal(m.download(f), s.download(f)), (grids[i], i, f)
+    print("[ensemble batches] 16x32 and 16x16 planes: every member equal", flush=True)
