@@ -250,3 +250,30 @@ def test_ensemble_batches_of_other_planes_equal_standalone():
         for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI", "U_PG", "W_PG"):
             assert np.array_equal(m.download(f), s.download(f)), (grids[i], i, f)
     print("[ensemble batches] 16x32 and 16x16 planes: every member equal", flush=True)
+
+
+def test_ensemble_64_members_400_steps_equal_standalone():
+    """The 64-member workload in full: members on the grids 64x32x32,
+    128x48x48 and 192x64x64 with their drawn R, Pe and seeds, advanced 400
+    steps as an ensemble in mixed precision (equal grids in batches), equal
+    every member run alone for 400 steps in every bit."""
+    specs = ensemble_members(64)
+    t0 = time.time()
+    ens = P.Ensemble([m[0] for m in specs], "mixed")
+    assert ens.batches() == 3
+    c0s = []
+    for (p, ic, seed, amp), m in zip(specs, ens.members):
+        c0 = P.ic_fill(p, ic, seed, amp)
+        m.init(c0)
+        c0s.append(c0)
+    ens.run(400)
+    print("[ensemble 64] 400 steps in %.1f s" % (time.time() - t0), flush=True)
+    for i, ((p, *_), m, c0) in enumerate(zip(specs, ens.members, c0s)):
+        s = P.Sim(p, "mixed")
+        s.init(c0)
+        s.run(400)
+        for f in ("C_OLD", "U_OLD", "V_OLD", "W_OLD", "PHI"):
+            assert np.array_equal(m.download(f), s.download(f)), ((p.nx, p.ny, p.nz), i, f)
+        s.close()
+    print("[ensemble 64] every member equal after 400 steps (%.1f s)" % (time.time() - t0), flush=True)
+    ens.close()
