@@ -69,7 +69,7 @@ struct Ctx {
     // [out][in]; null when not used
     unsigned short* cbf = nullptr;
     // small planes (ny, nz multiples of 16, at most 64): the dense transform
-    // in one launch per direction (k_dct_small) instead of two k_gemm
+    // in one launch per direction (k_dct_planes) instead of two k_gemm
     int dsmall = 0;
     double* scal = nullptr;   // S_SIZE device scalars
     double* part = nullptr;   // reduction partials
@@ -139,7 +139,7 @@ struct Ops {
     // (n = 128) with the kernels set up for it
     static bool dct_mfma_ready(int n);
     // whether planes of ny x nz (multiples of 16, at most 64) run both dense
-    // contractions in one launch (k_dct_small) with the kernels set up for it
+    // contractions in one launch (k_dct_planes) with the kernels set up for it
     static bool dct_small_ready(int ny, int nz);
     // {del, ca, cb, cc} of field F's lines in direction DIR (host, T)
     static void line_consts(const Phys& ph, const Dims& d, int F, int DIR, T* k);

@@ -1589,96 +1589,29 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T* __re
 }
 
 // Dense (y, z) transforms of small planes (ny and nz multiples of 16, at
-// most 64: the coarse and ensemble grids) in one launch.  Block (x, g) makes
-// the columns kz = 16 g .. 16 g + 15 of x-plane x: it stages the plane in
-// LDS, contracts it along z into Q[y][kz] = sum_m P[y][m] Cz[kz][m] (its 16
-// columns, every y), then along y into out[ky][kz] = sum_m Cy[ky][m] Q[m][kz]
-// (with phi set: phi, normalised, with its face ghosts).  Wave w makes the
-// 16-row tile w of both (ny <= 64); the Cz and Cy operands of its tiles are
-// loaded into registers at the start.  Every output tile is the MFMA
-// sequence of k_gemm's (a zero accumulator, k ascending in steps of 4), so
-// the result equals the two k_gemm launches bit for bit, without Q's round
-// trip through HBM.
+// most 64: the coarse and ensemble grids), both contractions per launch.
 constexpr int DS_MAX = 64;
-// LDS pitches: the plane's rows at nz + 4 values (the 16 x 4 lanes of an A
+// LDS pitch of a plane's rows: nz + 4 values (the 16 x 4 lanes of an A
 // operand read rows li at column lk: li (nz + 4) + lk falls in distinct
-// banks, nz + 4 being 4 times an odd number), Q's rows at 16
+// banks, nz + 4 being 4 times an odd number)
 __host__ __device__ constexpr int ds_pitch(int nz) { return nz + 4; }
-constexpr int DS_QP = 16;
 
-template <typename T, bool INV>
-__global__ __launch_bounds__(256) void k_dct_small(int ny, int nz, const T* __restrict__ in, T* __restrict__ out,
-                                                   const T* __restrict__ Cz, const T* __restrict__ Cy, PhiOut<T> po,
-                                                   Bt bt)
-{
-    // member: blockIdx.y (packed blocks pk apart, phi fs apart)
-    in += blockIdx.y * bt.pk;
-    out += blockIdx.y * bt.pk;
-    po.phi = bt_f(bt, po.phi, blockIdx.y);
-    extern __shared__ unsigned char pllc_smem[];
-    typedef typename Mma<T>::acc acc_t;
-    constexpr int KM = DS_MAX / 4;             // k steps of 4 at most
-    const int pz = ds_pitch(nz);
-    T* P = reinterpret_cast<T*>(pllc_smem);   // [ny][pz]
-    T* Q = P + ny * pz;                        // [ny][DS_QP]: the block's 16 columns
-    const int ng = nz >> 4;
-    const int x = blockIdx.x / ng, z0 = (blockIdx.x % ng) * 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int ty = ny >> 4;
-    // this lane's B operands of the z contraction (column z0 + li of Cz^T)
-    // and A operands of the y contraction (its first tile's rows of Cy)
-    T bz[KM], ay[KM];
-#pragma unroll
-    for (int q = 0; q < KM; q++) {
-        bz[q] = 4 * q < nz ? Cz[(long)(z0 + li) * nz + 4 * q + lk] : T(0);
-        ay[q] = (4 * q < ny && wave < ty) ? Cy[(long)(16 * wave + li) * ny + 4 * q + lk] : T(0);
-    }
-    const long plane = (long)ny * nz;
-    const T* __restrict__ src = in + x * plane;
-    for (int e = threadIdx.x; e < ny * nz; e += 256) {
-        const int y = e / nz;
-        P[y * pz + e - y * nz] = src[e];
-    }
-    __syncthreads();
-    // ny <= 64: at most four row tiles, one per wave
-    const int y0 = 16 * wave;
-    if (wave < ty) {
-        acc_t acc = acc_t{T(0), T(0), T(0), T(0)};
-#pragma unroll
-        for (int q = 0; q < KM; q++)
-            if (4 * q < nz) acc = Mma<T>::run(P[(y0 + li) * pz + 4 * q + lk], bz[q], acc);
-#pragma unroll
-        for (int r = 0; r < 4; r++) Q[(y0 + Mma<T>::row(lk, r)) * DS_QP + li] = acc[r];
-    }
-    __syncthreads();
-    if (wave >= ty) return;
-    const T norm = T(4 * po.ny * po.nz);
-    acc_t acc = acc_t{T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int q = 0; q < KM; q++)
-        if (4 * q < ny) acc = Mma<T>::run(ay[q], Q[(4 * q + lk) * DS_QP + li], acc);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int gm = y0 + Mma<T>::row(lk, r), gn = z0 + li;
-        if (INV && po.phi) phi_store(po, x, gm, gn, fdiv<T>(acc[r], norm));
-        else out[x * plane + (long)gm * nz + gn] = acc[r];
-    }
-}
-
-// The same transforms for the many planes of a batch (Bt): persistent
-// blocks stage Cz and Cy in LDS once and take planes in turn (plane p of the
-// batch's packed blocks is member p / nx's plane p % nx), each plane staged
-// in LDS and contracted along z into Q, then along y.  A block's next plane
-// is loaded into registers while it contracts the current one, and written
-// to LDS once the z contraction has read the current one, so HBM latency
-// overlaps the matrix work.  N > 0: square planes ny = nz = N (so Cy = Cz,
-// one table staged), N / 16 waves, wave w making the 16 x 16 tiles of row
-// tile w (their A operand shared), the k loops unrolled; N = 0: any ny, nz (multiples of 16, at most 64), 4
-// waves, wave w making tiles w, w + 4, ...  A wave's tiles' accumulators
-// are interleaved over k.  Each tile's MFMA sequence is k_gemm's (a zero
-// accumulator, k ascending in steps of 4), so the result equals
-// k_dct_small's (and the two-launch path's) bit for bit.
+// Persistent blocks take the planes in turn (plane p of the packed blocks
+// is member p / nx's plane p % nx when batched, Bt), the DCT matrices
+// staged in LDS once; each plane is staged in LDS and contracted along z
+// into Q[y][kz] = sum_m P[y][m] Cz[kz][m], then along y into out[ky][kz] =
+// sum_m Cy[ky][m] Q[m][kz] (with phi set: phi, normalised, with its face
+// ghosts).  A block's next plane is loaded into registers while it
+// contracts the current one, and written to LDS once the z contraction has
+// read the current one, so HBM latency overlaps the matrix work.  N > 0:
+// square planes ny = nz = N (so Cy = Cz, one table staged), N / 16 waves,
+// wave w making the 16 x 16 tiles of row tile w (their A operand shared),
+// the k loops unrolled; N = 0: any ny, nz, 4 waves, wave w making tiles
+// w, w + 4, ...  A wave's tiles' accumulators are interleaved over k.  Each
+// tile's MFMA sequence is k_gemm's (a zero accumulator, k ascending in
+// steps of 4), so the result equals the two k_gemm launches bit for bit,
+// without Q's round trip through HBM.  A plane is read whole before any of
+// it is written, by its one block, so in may equal out.
 __host__ __device__ constexpr int dct_planes_threads(int N) { return N ? 4 * N : 256; }
 
 template <typename T, bool INV, int N>
@@ -2500,8 +2433,6 @@ __global__ void k_weno_batch(const double* __restrict__ d1, const int* __restric
     template __global__ void k_pg<T>(Dims, Phys, Visc4<T>, const T*, T*, T*, T*, const double*, T, T, int);    \
     template __global__ void k_cfl_partial<T>(Dims, const T*, const T*, const T*, double*, Bt);                \
     template __global__ void k_bulk_partial<T>(Dims, const T*, double*);                                       \
-    template __global__ void k_dct_small<T, false>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt); \
-    template __global__ void k_dct_small<T, true>(int, int, const T*, T*, const T*, const T*, PhiOut<T>, Bt);  \
     PLLC_INST_DCT_PLANES(T, 0)                                                                                 \
     PLLC_INST_DCT_PLANES(T, 16)                                                                                \
     PLLC_INST_DCT_PLANES(T, 32)                                                                                \
@@ -3685,10 +3616,8 @@ template <typename T>
 bool Ops<T>::dct_small_ready(int ny, int nz)
 {
     if (ny % 16 || nz % 16 || ny > DS_MAX || nz > DS_MAX) return false;
-    const int sm = (ny * ds_pitch(nz) + ny * DS_QP) * (int)sizeof(T);
     const int smp = dct_planes_lds<T>(ny, nz);
-    return lds_optin((const void*)k_dct_small<T, false>, sm) && lds_optin((const void*)k_dct_small<T, true>, sm) &&
-           lds_optin((const void*)dct_planes_fn<T>(false, ny, nz), smp) &&
+    return lds_optin((const void*)dct_planes_fn<T>(false, ny, nz), smp) &&
            lds_optin((const void*)dct_planes_fn<T>(true, ny, nz), smp);
 }
 
@@ -3754,32 +3683,18 @@ void Ops<T>::dct2(Ctx<T>& c, const T* in, T* out, bool inverse, T* phi, bool gho
         }
         return;
     }
-    if (c.dsmall && (in != out || (inverse && phi))) {
-        // small planes: both contractions in one launch, one block per 16
-        // columns of a plane; each block reads its whole plane, so the
-        // result may not overwrite the input (in place: the k_gemm pair)
-        if (c.bt.nm > 1) {
-            // a batch's planes: persistent blocks with the tables in LDS
-            const size_t sm = (size_t)dct_planes_lds<T>(ny, nz);
-            const int np = nx * c.bt.nm;
-            const PhiOut<T> po{inverse ? phi : nullptr, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
-            const T* tz = inverse ? c.Ciz : c.Cfz;
-            const T* ty = inverse ? c.Ciy : c.Cfy;
-            const DctPlanesFn<T> kern = dct_planes_fn<T>(inverse, ny, nz);
-            const int nthr = dct_planes_threads(ny == nz ? ny : 0);
-            const int G = std::max(1, std::min(np, resident_blocks(kern, nthr, sm)));
-            PLLC_L(c, "dct", (kern<<<G, nthr, sm, c.st>>>(ny, nz, nx, np, in, out, tz, ty, po, c.bt)));
-            return;
-        }
-        const size_t sm = ((size_t)ny * ds_pitch(nz) + (size_t)ny * DS_QP) * sizeof(T);
-        const dim3 nb((unsigned)nx * (unsigned)(nz / 16), c.bt.nm, 1);
-        if (inverse) {
-            const PhiOut<T> po{phi, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
-            PLLC_L(c, "dct", (k_dct_small<T, true><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Ciz, c.Ciy, po, c.bt)));
-        } else {
-            PLLC_L(c, "dct", (k_dct_small<T, false><<<nb, 256, sm, c.st>>>(ny, nz, in, out, c.Cfz, c.Cfy,
-                                                                            PhiOut<T>{}, c.bt)));
-        }
+    if (c.dsmall) {
+        // small planes: both contractions in one launch, persistent blocks
+        // over the planes (of every member of a batch)
+        const size_t sm = (size_t)dct_planes_lds<T>(ny, nz);
+        const int np = nx * c.bt.nm;
+        const PhiOut<T> po{inverse ? phi : nullptr, c.d.sx, c.d.sy, nx, ny, nz, ghosts ? 1 : 0};
+        const T* tz = inverse ? c.Ciz : c.Cfz;
+        const T* ty = inverse ? c.Ciy : c.Cfy;
+        const DctPlanesFn<T> kern = dct_planes_fn<T>(inverse, ny, nz);
+        const int nthr = dct_planes_threads(ny == nz ? ny : 0);
+        const int G = std::max(1, std::min(np, resident_blocks(kern, nthr, sm)));
+        PLLC_L(c, "dct", (kern<<<G, nthr, sm, c.st>>>(ny, nz, nx, np, in, out, tz, ty, po, c.bt)));
         return;
     }
     if (!single(c, "dct2")) return;
@@ -3811,9 +3726,8 @@ void Ops<T>::poisson(Ctx<T>& c, double g, double r, bool from_div, bool ghosts)
         if (!single(c, "poisson")) return;
         PLLC_L(c, "div", (k_pack_div<T><<<grd, blk, 0, c.st>>>(c.d, c.ph, c.f[DIV], c.P, c.scal, T(g), T(r))));
     }
-    // the spectrum: in place in P, or in Q for small planes (whose forward
-    // transform cannot run in place)
-    T* sp = c.dsmall ? c.Q : c.P;
+    // the spectrum, in place in P
+    T* sp = c.P;
     dct2(c, c.P, sp, false);
     const int PC = c.line_solver == 0 ? part_chunks(c.d.nx, c.d.nx) : 0;
     if (PC > 0) {

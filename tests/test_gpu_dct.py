@@ -60,10 +60,10 @@ def test_fdct_idct_match_scipy(n1, n2, nt):
 @pytest.mark.parametrize("precision", ["plain", "mixed"])
 def test_small_plane_transform_equals_two_launch_path(monkeypatch, shape, precision):
     """Planes of at most 64 x 64 (multiples of 16) run both dense
-    contractions of the Poisson solve in one launch (k_dct_small); it makes
-    the MFMA sequence of the two k_gemm launches (PLLC_DCT=gemm), so three
-    steps from a state that varies in z leave every field equal in every
-    bit."""
+    contractions of the Poisson solve in one launch (k_dct_planes: square
+    planes and the any-shape form); it makes the MFMA sequence of the two
+    k_gemm launches (PLLC_DCT=gemm), so three steps from a state that varies
+    in z leave every field equal in every bit."""
     nx, ny, nz = shape
     prm = P.make_params(nx, ny, nz, Vratio=3.0, Pe=1000.0, Inf_thick=0.05)
     c0 = P.ic_fill(prm, P.IC_MULTIMODE, 4, 0.02)
