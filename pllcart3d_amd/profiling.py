@@ -11,6 +11,12 @@
         scaled to zero so the state is the plain one) making a correction
         every step: the network's kernels beside the step's.
 
+    python pllcart3d_amd/profiling.py run --ensemble [--steps 4]
+        the 64-member mixed-precision ensemble (three batches of equal grids,
+        their step pairs the branches of one graph) after two warm-up steps;
+        a separate handle's UPDATE_OLD (12 copy kernels) marks where the
+        summarised dispatches start.
+
     python pllcart3d_amd/profiling.py summary --trace T.db [--fetch F.db --write W.db]
         a markdown summary: per-kernel durations from the kernel trace and the
         HBM bytes per dispatch from the two counter passes, scaled by what the
@@ -54,6 +60,23 @@ def run(args):
         print("profile run: closure on the %dx%dx%d grid, %d steps" % (cp.nx, cp.ny, cp.nz, args.steps), flush=True)
         sim.close()
         net.close()
+        return
+    if args.ensemble:
+        from pllcart3d_amd.configs import ensemble_members
+        mem = ensemble_members(64)
+        ens = P.Ensemble([m[0] for m in mem], "mixed")
+        for (p, rec, seed, amp), s in zip(mem, ens.members):
+            s.init(P.ic_fill(p, rec, seed, amp))
+        ens.run(2)
+        mk = P.Sim(mem[0][0], "mixed")
+        mk.init(P.ic_fill(*mem[0]))
+        mk.op("UPDATE_OLD")
+        mk.synchronize()
+        mk.close()
+        ens.run(args.steps)
+        print("profile run: ensemble of %d members in %d batches, %d steps" % (len(mem), ens.batches(), args.steps),
+              flush=True)
+        ens.close()
         return
     if args.tune:
         # the sweeps' block shapes tuned on a throwaway handle first, as
@@ -103,7 +126,30 @@ def _trace_rows(db):
     return [(n, c, t, t / c, 100.0 * t / tot) for n, (c, t) in agg.items()]
 
 
+def ensemble_summary(args):
+    """Dispatches after the last copy kernel (the marker), every stream."""
+    con = sqlite3.connect(args.trace)
+    rows = list(con.execute("select name, dispatch_id, start, end, grid_x, grid_y, grid_z from kernels "
+                            "order by dispatch_id"))
+    last = max(di for n, di, a, b, gx, gy, gz in rows if _short(n).startswith("k_copy"))
+    rows = [r for r in rows if r[1] > last]
+    span = (max(r[3] for r in rows) - min(r[2] for r in rows)) / 1e6
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for name, di, a, b, gx, gy, gz in rows:
+        k = (_short(name), gx, gy, gz)
+        agg[k][0] += 1
+        agg[k][1] += (b - a) / 1e3
+    out = ["%d dispatches over %.2f ms: %.2f ms per step (%d steps; the batches' kernels overlap, so "
+           "durations below add up to more than the span)" % (len(rows), span, span / args.steps, args.steps), "",
+           "| kernel | grid (work-items) | calls | total ms | avg us |", "|---|---|---:|---:|---:|"]
+    for (n, gx, gy, gz), (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append("| `%s` | %d x %d x %d | %d | %.3f | %.2f |" % (n[:60], gx, gy, gz, c, t / 1e3, t / c))
+    sys.stdout.write("\n".join(out) + "\n")
+
+
 def summary(args):
+    if getattr(args, "ensemble", False):
+        return ensemble_summary(args)
     out = []
     rows = _trace_rows(args.trace)
     out.append("| kernel | calls | total ms | avg us | % |")
@@ -147,12 +193,15 @@ def main():
     r.add_argument("--precision", default="plain", choices=["plain", "mixed"])
     r.add_argument("--tune", action="store_true", help="autotune the sweeps' block shapes first (as bench.py)")
     r.add_argument("--closure", action="store_true", help="the coarse grid with the learned closure instead")
+    r.add_argument("--ensemble", action="store_true", help="the 64-member mixed-precision ensemble instead")
     s = sub.add_parser("summary")
     s.add_argument("--trace", required=True)
     s.add_argument("--fetch")
     s.add_argument("--write")
     s.add_argument("--points", type=int, default=262 * 130 * 130, help="points of one field (with ghosts)")
     s.add_argument("--bytes", type=int, default=8, help="bytes per value")
+    s.add_argument("--ensemble", action="store_true", help="an ensemble run's trace (every stream, per grid)")
+    s.add_argument("--steps", type=int, default=4, help="the ensemble run's profiled steps")
     a = ap.parse_args()
     run(a) if a.cmd == "run" else summary(a)
 

@@ -69,3 +69,27 @@ def test_summary_counts_the_profiled_handle_only(tmp_path, capsys):
     assert abs(float(cells[3]) - 16384 * 1024 / 1e6) < 0.05
     assert abs(float(cells[4]) - 2048 * 1024 / 1e6) < 0.05
     assert abs(float(cells[5]) - 2.25) < 1e-6
+
+
+def test_ensemble_summary_takes_every_stream_after_the_last_copy(tmp_path, capsys):
+    cp = "void pllc::k_copy<float>(float const*, float*, long)"
+    cv = "void pllc::k_corr_pg<float, true>(pllc::Dims)"
+    con = sqlite3.connect(str(tmp_path / "e.db"))
+    con.execute("create table kernels (name text, dispatch_id int, start int, end int, grid_x int, grid_y int, "
+                "grid_z int)")
+    rows = [(cv, 1, 0, 5000, 64, 64, 4224),                        # warm-up step
+            (cp, 2, 6000, 7000, 512, 1, 1), (cp, 3, 7000, 8000, 512, 1, 1),   # the marker
+            (cv, 4, 10000, 14000, 64, 64, 4224), (cv, 5, 11000, 13000, 64, 48, 3328),   # overlapping batches
+            (cv, 6, 20000, 24000, 64, 64, 4224), (cv, 7, 21000, 22000, 64, 48, 3328)]
+    con.executemany("insert into kernels values (?, ?, ?, ?, ?, ?, ?)", rows)
+    con.commit()
+    con.close()
+    args = argparse.Namespace(trace=str(tmp_path / "e.db"), ensemble=True, steps=2)
+    profiling.summary(args)
+    out = capsys.readouterr().out
+    # span 10 us .. 24 us = 0.014 ms over 2 steps
+    assert "4 dispatches over 0.01 ms: 0.01 ms per step (2 steps" in out, out
+    line = [l for l in out.splitlines() if l.startswith("| `k_corr_pg<float, true>` | 64 x 64 x 4224 |")]
+    assert line and line[0].split("|")[3].strip() == "2" and abs(float(line[0].split("|")[5]) - 4.0) < 1e-9, out
+    line = [l for l in out.splitlines() if l.startswith("| `k_corr_pg<float, true>` | 64 x 48 x 3328 |")]
+    assert line and abs(float(line[0].split("|")[5]) - 1.5) < 1e-9, out
