@@ -245,11 +245,23 @@ def ensemble_line(a):
     steps = max(2, min(a.steps, 10))
     ms = ens.time_steps(2, steps)
     batches = ens.batches()
+    # the batched Poisson transform on the matrix cores (FP32 MFMA, after
+    # the timed steps: each batch's members advance alone here): per
+    # launch, every plane of every member contracted along z and y
+    mfma = []
+    for b in range(batches):
+        nm, (nx, ny, nz) = ens.batch_info(b)
+        avg, n = ens.time_kernel(b, "dct", 2)
+        flops = nm * nx * 2.0 * (ny * nz * nz + ny * ny * nz)
+        mfma.append({"members": nm, "grid": "%dx%dx%d" % (nx, ny, nz), "planes": nm * nx, "launches": n,
+                     "avg_ms": round(avg, 5), "tflops": round(flops / (avg * 1e-3) / 1e12, 1),
+                     "frac_fp32_mfma_peak": round(flops / (avg * 1e-3) / 157.3e12, 3)})
     ens.close()
     return {"value": cells * steps / (ms * 1e-3), "unit": "cell-steps/s", "members": len(mem),
             "batches": batches, "cells": cells, "steps": steps, "ms_per_step": ms / steps, "dtype": "f32/f64",
             "workload": "ensemble: %d members from {64x32x32, 128x48x48, 192x64x64}, R~U[1,4], "
-                        "Pe log-uniform [200,4000], mixed precision" % len(mem)}
+                        "Pe log-uniform [200,4000], mixed precision" % len(mem),
+            "dct_mfma": mfma}
 
 
 def large_line(a):

@@ -299,6 +299,14 @@ pllc_sim *pllc_ensemble_member(pllc_ensemble *e, int i);
 int pllc_ensemble_run(pllc_ensemble *e, int nsteps);
 /* warmup steps, then nsteps timed with HIP events across all members */
 int pllc_ensemble_time_steps(pllc_ensemble *e, int warmup, int nsteps, double *ms);
+/* batch b's member count and grid (dims[3] = nx, ny, nz, interior cells) */
+int pllc_ensemble_batch_info(const pllc_ensemble *e, int b, int *members, int *dims);
+/* nsteps fused steps of batch b's members alone (the other members do not
+ * advance), each launch of the kernel class `name` (pllc_kernel_names)
+ * timed with HIP events on the batch's stream: average ms (avg_ms) and
+ * launch count (launches), both nullable */
+int pllc_ensemble_time_kernel(pllc_ensemble *e, int b, const char *name, int nsteps, double *avg_ms,
+                              int *launches);
 /* the wave perturbation (shift_conc) on c_old with Ampli, Lamb, Subharm
  * of param.input */
 int pllc_sim_shift_conc(pllc_sim *s, double Ampli, double Lamb, int Subharm);

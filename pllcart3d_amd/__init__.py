@@ -167,6 +167,24 @@ class Ensemble:
         """The number of batches of members stepped together."""
         return self.L.pllc_ensemble_batches(self.h)
 
+    def batch_info(self, b):
+        """(members, (nx, ny, nz)) of batch b."""
+        n = C.c_int(0)
+        d = (C.c_int * 3)()
+        if self.L.pllc_ensemble_batch_info(self.h, b, C.byref(n), d) != 0:
+            raise RuntimeError("pllc_ensemble_batch_info: " + last_error())
+        return n.value, (d[0], d[1], d[2])
+
+    def time_kernel(self, b, name, nsteps):
+        """Average ms and count of the launches of kernel class `name` over
+        nsteps steps of batch b's members alone (they advance; the other
+        members do not)."""
+        avg = C.c_double(0.0)
+        n = C.c_int(0)
+        if self.L.pllc_ensemble_time_kernel(self.h, b, name.encode(), nsteps, C.byref(avg), C.byref(n)) != 0:
+            raise RuntimeError("pllc_ensemble_time_kernel: " + last_error())
+        return avg.value, n.value
+
     def run(self, nsteps):
         if self.L.pllc_ensemble_run(self.h, nsteps) != 0:
             raise RuntimeError("pllc_ensemble_run: " + last_error())

@@ -78,6 +78,9 @@ def load():
     L.pllc_ensemble_member.argtypes = [vp, C.c_int]
     L.pllc_ensemble_run.argtypes = [vp, C.c_int]
     L.pllc_ensemble_time_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.pllc_ensemble_batch_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pllc_ensemble_time_kernel.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int)]
     L.pllc_sim_slope_parameter.restype = C.c_double
     L.pllc_sim_slope_parameter.argtypes = [vp]
     L.pllc_sim_synchronize.argtypes = [vp]

@@ -1537,17 +1537,23 @@ static bool batches_pair_graph(pllc_ensemble* e)
     return true;
 }
 
+// the batches' physics and line-constant tables, once: the members' set-up
+// is done by the first run (their physics is final)
+static bool ensemble_tables(pllc_ensemble* e, const char* W)
+{
+    if (e->tables || e->batches.empty()) return true;
+    for (pllc_sim* s : e->m)
+        if (finish(s, W)) return false;
+    for (ens_batch* b : e->batches)
+        if (!batch_tables(b, W)) return false;
+    e->tables = true;
+    return true;
+}
+
 static int ensemble_steps(pllc_ensemble* e, int nsteps)
 {
     const char* W = "pllc_ensemble_run";
-    if (!e->tables && !e->batches.empty()) {
-        // the members' set-up is done by now (their physics is final)
-        for (pllc_sim* s : e->m)
-            if (finish(s, W)) return -1;
-        for (ens_batch* b : e->batches)
-            if (!batch_tables(b, W)) return -1;
-        e->tables = true;
-    }
+    if (!ensemble_tables(e, W)) return -1;
     for (int done = 0; done < nsteps;) {
         const int k = nsteps - done >= 2 ? 2 : 1;
         if (!(k == 2 && batches_pair_graph(e))) {
@@ -1631,6 +1637,76 @@ extern "C" int pllc_sim_time_kernel(pllc_sim* s, const char* name, int nsteps, d
         tot += f;
         cnt++;
     }
+    if (avg_ms) *avg_ms = cnt ? tot / cnt : 0.0;
+    if (launches) *launches = cnt;
+    return 0;
+}
+
+extern "C" int pllc_ensemble_batch_info(const pllc_ensemble* e, int b, int* members, int* dims)
+{
+    if (!e || b < 0 || b >= (int)e->batches.size()) {
+        pllc_set_error("pllc_ensemble_batch_info", "no such batch");
+        return -1;
+    }
+    const ens_batch* x = e->batches[b];
+    if (members) *members = (int)x->m.size();
+    if (dims) { dims[0] = x->c.d.nx; dims[1] = x->c.d.ny; dims[2] = x->c.d.nz; }
+    return 0;
+}
+
+// events around the launches of one kernel class on a batch's stream
+struct BatchTimer {
+    const char* name;
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    size_t n = 0;
+};
+
+static void batch_timer_hook(void* user, const char* name, int phase)
+{
+    BatchTimer* t = (BatchTimer*)user;
+    if (strcmp(name, t->name) != 0) return;
+    if (t->n >= t->ev.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        t->ev.push_back(e);
+    }
+    (void)hipEventRecord(t->ev[t->n++], t->st);
+    (void)phase;
+}
+
+extern "C" int pllc_ensemble_time_kernel(pllc_ensemble* e, int b, const char* name, int nsteps, double* avg_ms,
+                                         int* launches)
+{
+    const char* W = "pllc_ensemble_time_kernel";
+    if (!e || b < 0 || b >= (int)e->batches.size()) { pllc_set_error(W, "no such batch"); return -1; }
+    if (!name) { pllc_set_error(W, "no name"); return -1; }
+    if (nsteps < 0) { pllc_set_error(W, "negative step count"); return -1; }
+    if (!ensemble_tables(e, W)) return -1;
+    ens_batch* x = e->batches[b];
+    BatchTimer t;
+    t.name = name;
+    t.st = x->st;
+    x->c.hook = batch_timer_hook;
+    x->c.hook_user = &t;
+    bool ok = true;
+    for (int n = 0; ok && n < nsteps; n++) ok = batch_steps(x, 1);
+    x->c.hook = nullptr;
+    x->c.hook_user = nullptr;
+    if (!ok) pllc_set_error(W, "the batch does not step as one (its members step alone)");
+    for (pllc_sim* s : x->m)
+        if (finish(s, W)) ok = false;
+    if (!pllc_hip_ok(W, hipStreamSynchronize(x->st))) ok = false;
+    double tot = 0.0;
+    int cnt = 0;
+    for (size_t k = 0; ok && k + 1 < t.n; k += 2) {
+        float f = 0.f;
+        if (!pllc_hip_ok(W, hipEventElapsedTime(&f, t.ev[k], t.ev[k + 1]))) ok = false;
+        tot += f;
+        cnt++;
+    }
+    for (hipEvent_t ev : t.ev) (void)hipEventDestroy(ev);
+    if (!ok) return -1;
     if (avg_ms) *avg_ms = cnt ? tot / cnt : 0.0;
     if (launches) *launches = cnt;
     return 0;

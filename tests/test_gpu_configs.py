@@ -277,3 +277,36 @@ def test_ensemble_64_members_400_steps_equal_standalone():
         s.close()
     print("[ensemble 64] every member equal after 400 steps (%.1f s)" % (time.time() - t0), flush=True)
     ens.close()
+
+
+def test_ensemble_batch_kernel_timing_advances_that_batch_only():
+    """pllc_ensemble_batch_info reports each batch's members and grid;
+    pllc_ensemble_time_kernel times one kernel class over steps of one
+    batch's members alone: they advance exactly as standalone runs do, and
+    the other batch's members keep their state."""
+    grids = [(24, 16, 16)] * 3 + [(20, 16, 32)] * 2
+    specs = []
+    for i, (nx, ny, nz) in enumerate(grids):
+        prm = P.make_params(nx, ny, nz, Vratio=2.0 + 0.3 * i, Pe=500.0 + 100.0 * i, Inf_thick=0.05)
+        specs.append((prm, P.IC_NOISE, 900 + i, 0.01))
+    ens = P.Ensemble([m[0] for m in specs], "mixed")
+    assert ens.batches() == 2
+    info = sorted(ens.batch_info(b) for b in range(2))
+    assert info == [(2, (20, 16, 32)), (3, (24, 16, 16))], info
+    c0s = []
+    for (p, ic, seed, amp), m in zip(specs, ens.members):
+        c0 = P.ic_fill(p, ic, seed, amp)
+        m.init(c0)
+        c0s.append(c0)
+    b = [ens.batch_info(k)[1] for k in range(2)].index((24, 16, 16))
+    avg, n = ens.time_kernel(b, "dct", 2)
+    assert n == 12 and avg > 0.0, (avg, n)
+    for i, ((p, *_), m, c0) in enumerate(zip(specs, ens.members, c0s)):
+        s = P.Sim(p, "mixed")
+        s.init(c0)
+        if grids[i] == (24, 16, 16):
+            s.run(2)
+        for f in ("C_OLD", "U_OLD", "PHI"):
+            assert np.array_equal(m.download(f), s.download(f)), (grids[i], i, f)
+        s.close()
+    ens.close()
