@@ -105,6 +105,37 @@ def test_front_256x128x128_tracks_oracle(oracle_mod, threads):
     assert np.abs(orc.field("W_OLD")).max() > 1e-3
 
 
+def test_mixed_front_256x128x128_tracks_fp64_oracle(oracle_mod, threads):
+    """The benchmark's workload in mixed precision against the FP64 oracle
+    for 20 steps: the concentration within 1e-5 relative L2 (north_star's
+    bar) and the velocity vector within 1e-5 after every 5th step; each
+    step's dt (an FP64 reduction over the FP32 velocities) within 1e-5 of
+    the oracle's."""
+    c = CONFIGS["front_256x128x128"]
+    prm = config_params("front_256x128x128")
+    c0 = P.ic_fill(prm, c["ic"], c["seed"], c["amp"])
+    orc = oracle_mod.Oracle(prm)
+    orc.init(c0)
+    sim = P.Sim(prm, "mixed")
+    sim.init(c0)
+    t0 = time.time()
+    worst = {}
+    for n in range(20):
+        _, dto = orc.run(1)
+        _, dtg = sim.run(1)
+        assert abs(dtg - dto) <= 1e-5 * dto, (n, dtg, dto)
+        if (n + 1) % 5 == 0:
+            e = _errors(sim, orc)
+            for k, v in e.items():
+                worst[k] = max(worst.get(k, 0.0), v)
+            print("[mixed front_256x128x128] step %d (%.0f s): %s" % (n + 1, time.time() - t0,
+                  " ".join("%s %.2e" % kv for kv in e.items())), flush=True)
+            assert e["c"] <= 1e-5 and e["velocity"] <= 1e-5, (n + 1, e)
+    print("[mixed front_256x128x128] worst over 20 steps: %s" % " ".join("%s %.2e" % kv for kv in worst.items()),
+          flush=True)
+    sim.close()
+
+
 def test_front_512x256x256_first_steps_track_oracle(oracle_mod, threads):
     """The 512x256x256 workload (33.5 M cells; its 256x256 planes take the
     line-pass transforms), plain FP64, its first 3 steps against the oracle
