@@ -1843,6 +1843,17 @@ __device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v)
 // over n1, twiddles by e^{-2 pi i t k1/M} and stores at 8 k1 + t; then the
 // 8-point DFTs over the contiguous 8 k1 .. 8 k1 + 7 (P/8 of them per thread)
 // give X[k1 + P k2].  Every thread of the block calls it (block barriers).
+// The 8 threads of a line lie in one wave (consecutive thread indices), so
+// their exchanges through LDS need the wave's accesses ordered, not the
+// block's: a wave-scope fence on both sides of a wave barrier (the writes
+// before it complete and are visible to the wave's reads after it).
+__device__ __forceinline__ void line_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, bool INV, int M, bool SW = false>
 __device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restrict__ tw)
 {
@@ -1858,10 +1869,10 @@ __device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restric
         const Cx<T> w = twl(tw, t * k1);
         a[k1] = cmul(a[k1], INV ? cconj(w) : w);
     }
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int k1 = 0; k1 < P; k1++) scx<T, SW>(L, st, 8 * k1 + t, a[k1]);
-    __syncthreads();
+    line_sync();
     Cx<T> b[P / 8][8];
 #pragma unroll
     for (int g = 0; g < P / 8; g++) {
@@ -1869,12 +1880,12 @@ __device__ __forceinline__ void fft_line(T* L, int st, int t, const T* __restric
         for (int m2 = 0; m2 < 8; m2++) b[g][m2] = lcx<T, SW>(L, st, 8 * (t + 8 * g) + m2);
         dft8<T, INV>(b[g]);
     }
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int g = 0; g < P / 8; g++)
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) scx<T, SW>(L, st, t + 8 * g + P * k2, b[g][k2]);
-    __syncthreads();
+    line_sync();
 }
 
 // REDFT10 of one line of N reals at stride st, in place
@@ -1890,10 +1901,10 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
         a[m1] = m < N / 4 ? Cx<T>{L[lp<SW>(4 * m) * st], L[lp<SW>(4 * m + 2) * st]}
                           : Cx<T>{L[lp<SW>(2 * N - 1 - 4 * m) * st], L[lp<SW>(2 * N - 3 - 4 * m) * st]};
     }
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int m1 = 0; m1 < V; m1++) scx<T, SW>(L, st, 8 * m1 + t, a[m1]);
-    __syncthreads();
+    line_sync();
     fft_line<T, false, M, SW>(L, st, t, tw);
     // V[k] = (Z[k] + conj Z[M-k])/2 - i/2 WN^k (Z[k] - conj Z[M-k]);
     // X[k] = 2 Re(W4N^k V[k]), X[N-k] = -2 Im(W4N^k V[k])
@@ -1913,7 +1924,7 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
             xi[j] = T(-2) * q.i;
         }
     }
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int j = 0; j <= V; j++) {
         const int k = t + 8 * j;
@@ -1922,7 +1933,7 @@ __device__ __forceinline__ void redft10_line(T* L, int st, int t, const T* __res
             if (k >= 1 && k <= M - 1) L[lp<SW>(N - k) * st] = xi[j];
         }
     }
-    __syncthreads();
+    line_sync();
 }
 
 // REDFT01 of one line of N reals at stride st, in place
@@ -1946,23 +1957,23 @@ __device__ __forceinline__ void redft01_line(T* L, int st, int t, const T* __res
         const Cx<T> o = cmul(Cx<T>{vk.r - vh.r, vk.i - vh.i}, cconj(twl(tw, M + k)));
         c[j] = {e.r - o.i, e.i + o.r};
     }
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int j = 0; j < V; j++) scx<T, SW>(L, st, t + 8 * j, c[j]);
-    __syncthreads();
+    line_sync();
     fft_line<T, true, M, SW>(L, st, t, tw);
     // y[2n] = v[n], y[2n+1] = v[N-1-n], v[2m] = Re z[m], v[2m+1] = Im z[m]
     Cx<T> z[V];
 #pragma unroll
     for (int k2 = 0; k2 < V; k2++) z[k2] = lcx<T, SW>(L, st, t + 8 * k2);
-    __syncthreads();
+    line_sync();
 #pragma unroll
     for (int k2 = 0; k2 < V; k2++) {
         const int m = t + 8 * k2;
         if (m < N / 4) { L[lp<SW>(4 * m) * st] = z[k2].r; L[lp<SW>(4 * m + 2) * st] = z[k2].i; }
         else { L[lp<SW>(2 * N - 1 - 4 * m) * st] = z[k2].r; L[lp<SW>(2 * N - 3 - 4 * m) * st] = z[k2].i; }
     }
-    __syncthreads();
+    line_sync();
 }
 
 // N = 128: the 2D transform of plane b in one block (the plane in LDS, row
@@ -2001,11 +2012,13 @@ __global__ __launch_bounds__(FdCfg<T>::threads) void k_dct_plane(const T* in, T*
         if (INV) redft01_line<T, FD_N, SW>(pl + line * FD_P, 1, t, tw);
         else redft10_line<T, FD_N, SW>(pl + line * FD_P, 1, t, tw);
     }
+    __syncthreads();   // every row transformed before the columns are read
     for (int l0 = 0; l0 < FD_N; l0 += LG) {
         const int line = l0 + (tid >> 3);
         if (INV) redft01_line<T, FD_N, SW>(pl + line, FD_P, t, tw);
         else redft10_line<T, FD_N, SW>(pl + line, FD_P, t, tw);
     }
+    __syncthreads();
     if (INV && po.phi) {
         const T norm = T(4 * po.ny * po.nz);
         for (int e = tid; e < FD_N * FD_N; e += FD_THREADS)
@@ -2160,6 +2173,7 @@ __global__ __launch_bounds__(8 * FL) void k_dct_lines(const T* in, T* out, const
     const int line = tid >> 3, t = tid & 7;
     if (INV) redft01_line<T, N>(ls + line * P, 1, t, tw);
     else redft10_line<T, N>(ls + line * P, 1, t, tw);
+    __syncthreads();   // the stores below read every line
     if (AX == 2 && INV && po.phi) {
         const T norm = T(4 * po.ny * po.nz);
 #pragma unroll 2
