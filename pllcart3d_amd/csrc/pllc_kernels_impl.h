@@ -1842,7 +1842,8 @@ __device__ __forceinline__ void scx(T* L, int st, int c, Cx<T> v)
 // steps with M = P x 8: thread t takes the P values n = 8 n1 + t, DFTs them
 // over n1, twiddles by e^{-2 pi i t k1/M} and stores at 8 k1 + t; then the
 // 8-point DFTs over the contiguous 8 k1 .. 8 k1 + 7 (P/8 of them per thread)
-// give X[k1 + P k2].  Every thread of the block calls it (block barriers).
+// give X[k1 + P k2].  The line's 8 threads call it together (line_sync
+// between the steps); the block's lines are independent of each other.
 // The 8 threads of a line lie in one wave (consecutive thread indices), so
 // their exchanges through LDS need the wave's accesses ordered, not the
 // block's: a wave-scope fence on both sides of a wave barrier (the writes
