@@ -20,6 +20,8 @@ enum FieldId {
     F_CC,             // Thomas factors of the current sweep
     F_SRC2, F_CC2,    // second and third set for the batched momentum sweeps
     F_SRC3, F_CC3,
+    F_SRC4,           // u's explicit momentum source while the momentum explicit
+                      // parts run beside the concentration update (F_SRC is busy)
     NFIELDS
 };
 
@@ -90,6 +92,11 @@ struct Ctx {
     // named kernel class), used for HIP-event timing of one kernel class
     void (*hook)(void* user, const char* name, int phase) = nullptr;
     void* hook_user = nullptr;
+    // the momentum explicit parts may run beside the concentration update:
+    // they read neither c_new (no body force, Fx = Fy = 0) nor anything the
+    // concentration update and the viscosity write (set by the runtime for
+    // a simulation of its own; off in ensembles)
+    int mom_fork = 0;
 };
 
 // Operator launchers.  g, r are the RK coefficients (rk_gamma, rk_rho) the
@@ -104,6 +111,11 @@ struct Ops {
     // xmom, ymom and zmom together (needs F_SRC2/3, F_CC2/3)
     // ghosts: the x-sweeps also write update_velo_boundary's ghost copies
     static void momentum3(Ctx<T>& c, double g, double r, int slot, bool ghosts = false);
+    // momentum3 in two parts: the three explicit parts (sources in F_SRC4,
+    // F_SRC2, F_SRC3; c.st may be a side stream), then the sweeps of all
+    // three on c.st after them; together equal to momentum3
+    static void momentum3_explicit(Ctx<T>& c, double g, double r);
+    static void momentum3_implicit(Ctx<T>& c, double g, double r, int slot, bool ghosts);
     static void velo_ghost(Ctx<T>& c);
     static void divergence(Ctx<T>& c, double g, double r, bool write_div, bool pack);
     // ghosts: phi's face ghosts too (update_phi_ghosts) in the same pass

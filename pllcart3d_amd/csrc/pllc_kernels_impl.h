@@ -3518,29 +3518,34 @@ static void rhs_mom(Ctx<T>& c, T g, T r, T* S)
 }
 
 // comps: bit 0 u, bit 1 v, bit 2 w.  With several components their sources
-// go to F_SRC, F_SRC2, F_SRC3 and the sweeps of all of them share launches;
-// the components are independent (each reads only old velocities, c_new and
-// the viscosities), so this equals xmom; ymom; zmom (179-181).
+// go to F_SRC, F_SRC2, F_SRC3 (u's to F_SRC4 when split: F_SRC then holds
+// the concentration's) and the sweeps of all of them share launches; the
+// components are independent (each reads only old velocities, c_new and the
+// viscosities), so this equals xmom; ymom; zmom (179-181).  part: 1 the
+// explicit parts, 2 the sweeps, 3 both.
 template <typename T>
-static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot, bool vghost = false)
+static void momentum_set(Ctx<T>& c, int comps, double gd, double rd, int slot, bool vghost = false, int part = 3,
+                         bool split = false)
 {
     const T g = T(gd), r = T(rd);
     int F[3], n = 0;
     T *S[3], *CC[3], *qo[3], *qn[3], *N[3], *Nm[3];
-    const int src[3] = {F_SRC, F_SRC2, F_SRC3}, ccs[3] = {F_CC, F_CC2, F_CC3};
+    const int src[3] = {split ? F_SRC4 : F_SRC, F_SRC2, F_SRC3}, ccs[3] = {F_CC, F_CC2, F_CC3};
     for (int q = 0; q < 3; q++) {
         if (!(comps & (1 << q))) continue;
         const int f = FU + q, b = q == 0 ? U_OLD : q == 1 ? V_OLD : W_OLD;
         T* s = c.f[src[n]] ? c.f[src[n]] : c.f[F_SRC];
         T* cc = c.f[ccs[n]] ? c.f[ccs[n]] : c.f[F_CC];
-        if (f == FU) rhs_mom<T, FU>(c, g, r, s);
-        else if (f == FV) rhs_mom<T, FV>(c, g, r, s);
-        else rhs_mom<T, FW>(c, g, r, s);
+        if (part & 1) {
+            if (f == FU) rhs_mom<T, FU>(c, g, r, s);
+            else if (f == FV) rhs_mom<T, FV>(c, g, r, s);
+            else rhs_mom<T, FW>(c, g, r, s);
+        }
         F[n] = f; S[n] = s; CC[n] = cc;
         qo[n] = c.f[b]; qn[n] = c.f[b + 1]; N[n] = c.f[b + 3]; Nm[n] = c.f[b + 4];
         n++;
     }
-    adi_sets<T>(c, g, r, slot, F, n, S, CC, mun(c), qo, qn, N, Nm, vghost);
+    if (part & 2) adi_sets<T>(c, g, r, slot, F, n, S, CC, mun(c), qo, qn, N, Nm, vghost);
 }
 
 template <typename T>
@@ -3553,6 +3558,18 @@ template <typename T>
 void Ops<T>::momentum3(Ctx<T>& c, double g, double r, int slot, bool ghosts)
 {
     momentum_set<T>(c, 7, g, r, slot, ghosts);
+}
+
+template <typename T>
+void Ops<T>::momentum3_explicit(Ctx<T>& c, double g, double r)
+{
+    momentum_set<T>(c, 7, g, r, 0, false, 1, true);
+}
+
+template <typename T>
+void Ops<T>::momentum3_implicit(Ctx<T>& c, double g, double r, int slot, bool ghosts)
+{
+    momentum_set<T>(c, 7, g, r, slot, ghosts, 2, true);
 }
 
 template <typename T>
