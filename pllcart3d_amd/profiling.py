@@ -4,6 +4,8 @@
         set-up, one UPDATE_OLD (12 whole-field copies of known size: the
         calibration of the HBM counters) and `steps` fused steps; run it
         under rocprofv3 --kernel-trace --stats or --pmc FETCH_SIZE / WRITE_SIZE.
+        The steps run on one stream (PLLC_FORK=0) unless --fork is given, so
+        that each kernel's duration is its own.
 
     python pllcart3d_amd/profiling.py run --closure [--steps 20]
         the 2x-coarser mixed-precision grid of the configuration with the
@@ -40,6 +42,11 @@ def _short(name):
 
 def run(args):
     sys.path.insert(0, ROOT)
+    if not args.fork:
+        # one stream per step (no side stream for the momentum explicit
+        # parts), so each kernel's duration is its own, as the bench's
+        # per-kernel timing measures it
+        os.environ["PLLC_FORK"] = "0"
     import pllcart3d_amd as P
     from pllcart3d_amd.configs import CONFIGS, config_params
     prm = config_params(args.config)
@@ -194,6 +201,8 @@ def main():
     r.add_argument("--tune", action="store_true", help="autotune the sweeps' block shapes first (as bench.py)")
     r.add_argument("--closure", action="store_true", help="the coarse grid with the learned closure instead")
     r.add_argument("--ensemble", action="store_true", help="the 64-member mixed-precision ensemble instead")
+    r.add_argument("--fork", action="store_true",
+                   help="keep the side stream of the momentum explicit parts (default: one stream)")
     s = sub.add_parser("summary")
     s.add_argument("--trace", required=True)
     s.add_argument("--fetch")
