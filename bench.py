@@ -150,9 +150,10 @@ PEAK_TFLOPS = {8.0: 78.6, 4.0: 157.3}
 PROFILES = os.path.join(ROOT, "profiles")
 CLASS_KERNELS = {
     "sweep_x": ["k_sweep_part<{T}, 0>"], "sweep_y": ["k_sweep_part<{T}, 1>"],
-    "sweep_z": ["k_sweep_part_z<{T}>", "k_sweep_part_zt<{T}>"], "corr_pg": ["k_corr_pg<{T}>"],
-    "rhs_c": ["k_rhs_c<{T}, true>"], "rhs_u": ["k_rhs_mom<{T}, 1, true>"], "rhs_v": ["k_rhs_mom<{T}, 2, true>"],
-    "rhs_w": ["k_rhs_mom<{T}, 3, true>"], "div": ["k_div<{T}>"], "visc": ["k_visc<{T}>"],
+    "sweep_z": ["k_sweep_part_z<{T}>", "k_sweep_part_zt<{T}>"], "corr_pg": ["k_corr_pg<{T}, false>"],
+    "rhs_c": ["k_rhs_c<{T}, true, false>"], "rhs_u": ["k_rhs_mom<{T}, 1, true, false>"],
+    "rhs_v": ["k_rhs_mom<{T}, 2, true, false>"], "rhs_w": ["k_rhs_mom<{T}, 3, true, false>"],
+    "div": ["k_div<{T}, false>"], "visc": ["k_visc<{T}, false>"],
     "dct": ["k_dct_plane<{T}, false>", "k_dct_plane<{T}, true>"],
 }
 
