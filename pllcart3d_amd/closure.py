@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import Sim, ic_fill
+from . import Ensemble, Sim, ic_fill
 from ._lib import FIELDS, IC_MULTIMODE, Params, last_error, load
 from .configs import make_params
 
@@ -362,25 +362,37 @@ def draw_training_runs(n, seed0=500):
 def collect(fine_shape, runs, steps, every, warmup, log=print, horizon=1):
     """(features, targets) of box-filtered fine runs: one sample every
     `every` steps from step `warmup` to `steps` of each run, each target the
-    coarse solver's error over `horizon` steps."""
-    X, T = [], []
+    coarse solver's error over `horizon` steps.  The fine runs advance
+    together as one ensemble (plain FP64, each member a captured graph
+    replayed as a branch of one graph; equal to the runs done alone bit for
+    bit); each sample advances its member by `horizon` steps, so the members
+    stay in step.  Samples are returned run by run."""
+    fps, c0s = [], []
     for R, Pe, seed in runs:
         fp = make_params(*fine_shape, Vratio=R, Pe=Pe, Inf_thick=0.02)
-        c0 = ic_fill(fp, IC_MULTIMODE, seed, 0.01)
-        fine = Sim(fp, "plain")
-        fine.init(c0)
+        fps.append(fp)
+        c0s.append(ic_fill(fp, IC_MULTIMODE, seed, 0.01))
+    ens = Ensemble(fps, "plain")
+    try:
+        for m, c0 in zip(ens.members, c0s):
+            m.init(c0)
+        Xr = [[] for _ in runs]
+        Tr = [[] for _ in runs]
         done = 0
         for target_step in range(warmup, steps + 1, every):
-            fine.run(target_step - done)
-            x, t = sample_one_step(fine, fp, horizon)
+            ens.run(target_step - done)
+            for r, (fp, m) in enumerate(zip(fps, ens.members)):
+                x, t = sample_one_step(m, fp, horizon)
+                Xr[r].append(x.astype(np.float32))
+                Tr[r].append(t.astype(np.float32))
             done = target_step + horizon
-            X.append(x.astype(np.float32))
-            T.append(t.astype(np.float32))
-        rms = float(np.sqrt(np.mean(T[-1].astype(np.float64) ** 2)))
-        log("[closure data] R=%.2f Pe=%.0f seed %d: samples up to step %d, last target rms %.3e"
-            % (R, Pe, seed, done, rms))
-        fine.close()
-    return X, T
+        for (R, Pe, seed), Tl in zip(runs, Tr):
+            rms = float(np.sqrt(np.mean(Tl[-1].astype(np.float64) ** 2)))
+            log("[closure data] R=%.2f Pe=%.0f seed %d: samples up to step %d, last target rms %.3e"
+                % (R, Pe, seed, done, rms))
+    finally:
+        ens.close()
+    return [x for xs in Xr for x in xs], [t for ts in Tr for t in ts]
 
 
 def normalisation(X, T, mask):
