@@ -521,13 +521,13 @@ extern "C" void pllc_sim_destroy(pllc_sim* s)
 {
     if (!s) return;
     if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->side) (void)hipStreamSynchronize(s->side);
     std::lock_guard<std::recursive_mutex> lk(pllc_api_mutex());
     graph_drop(s);
     for (void* p : s->owned) (void)hipFree(p);
     for (hipEvent_t e : s->tev) (void)hipEventDestroy(e);
     if (s->e0) (void)hipEventDestroy(s->e0);
     if (s->e1) (void)hipEventDestroy(s->e1);
-    if (s->side) (void)hipStreamSynchronize(s->side);
     if (s->fork_ev) (void)hipEventDestroy(s->fork_ev);
     if (s->join_ev) (void)hipEventDestroy(s->join_ev);
     if (s->side) (void)hipStreamDestroy(s->side);
@@ -809,13 +809,14 @@ static void fused_step(pllc_sim* s, Ctx<T>& c)
             Ctx<T> cs = c;
             cs.st = s->side;
             Ops<T>::momentum3_explicit(cs, g, r);
-            (void)hipEventRecord(s->join_ev, s->side);
+            if (!pllc_hip_ok("fused step", hipEventRecord(s->join_ev, s->side)))
+                (void)hipStreamSynchronize(s->side);   // no join event: the side stream's work is done first
         }
         Ops<T>::concentration(c, g, r, rk, c.f[U_OLD], c.f[V_OLD], c.f[W_OLD]);
         T* mun[4] = {c.f[MUN_CELL], c.f[MUN_XY], c.f[MUN_XZ], c.f[MUN_YZ]};
         Ops<T>::viscosity(c, c.f[C_NEW], mun);
         if (fork) {
-            (void)hipStreamWaitEvent(c.st, s->join_ev, 0);
+            (void)pllc_hip_ok("fused step", hipStreamWaitEvent(c.st, s->join_ev, 0));
             Ops<T>::momentum3_implicit(c, g, r, rk, true);
         } else {
             Ops<T>::momentum3(c, g, r, rk, true);   // with update_velo_boundary
