@@ -298,7 +298,7 @@ def large_line(a):
 
 def closure_line(a):
     """The learned closure (cfg: coarse 128x64x64 FP32 run with the 3-layer
-    3x3x3 CNN, R=3, Pe=2000, judged against the fine 256x128x128 plain FP64
+    3x3x3 CNN, R=3, Pe=2000, compared with the fine 256x128x128 plain FP64
     run): training data from 4 box-filtered fine runs with R ~ U[2,4],
     Pe ~ U[1000,3000] (seeds 500..503; held-out run seed 600), training on
     the GPU with the library's BF16 kernels, then the fine run of the
@@ -366,7 +366,7 @@ def closure_line(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py: runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import pllcart3d_amd as P

@@ -1,8 +1,8 @@
 """FP64 CPU reference of the learned closure's network: a 3-layer 3x3x3
 convolutional network on the coarse grid's cells (numpy).
 
-TEST INFRASTRUCTURE ONLY: the checker of the GPU's BF16 forward and
-backward kernels (tests/), never the product.  The reference has no learned
+TEST INFRASTRUCTURE ONLY: the tests (tests/) compare the GPU's BF16 forward
+and backward kernels with it; it is never the product.  The reference has no learned
 closure; this network corrects the explicit advective term of Hc_op
 (pllcart3d.c:1539-1644), N = -(u c_x + v c_y + w c_z), on a 2x coarser grid.
 

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (pllcart3d_amd/,
  * include/) includes, links or loads this code; only tests/, the smoke check
- * and the CPU-baseline leg of bench.py call it, and only as the checker.
+ * and the CPU-baseline leg of bench.py call it, and only to check results.
  *
  * The oracle follows the reference's arithmetic operation by operation
  * (same operand order, no FMA contraction: build with -ffp-contract=off) so

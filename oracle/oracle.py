@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
-baseline of bench.py, always as the checker, never as the product.
+baseline of bench.py, always to check results, never as the product.
 """
 import ctypes as C
 import os

@@ -13,7 +13,7 @@ dt times it to c; the prediction is made anew every `every` steps
 (set_closure).  A guarded run falls back to plain WENO where c leaves
 [0, 1] or the correction injects mass (pllc_sim_run_guarded).  The
 mixing-zone length, from the y-z average of c along x (the profile
-conc_average writes to cavgyz.bin, pllcart3d.c:3691-3724), judges the
+conc_average writes to cavgyz.bin, pllcart3d.c:3691-3724), measures the
 coarse run against the fine one.
 """
 import ctypes as C

@@ -1,4 +1,4 @@
-"""The run configurations of this build (box 4x1x1, Nmvp = 0, Fx = Fy = 0).
+"""The run configurations of the benchmark and tests (box 4x1x1, Nmvp = 0, Fx = Fy = 0).
 
 The reference keeps c with the displacing fluid at c = 0 and
 mu = exp(Vratio * c) (pllcart3d.c:502, 853); a front written as

@@ -6,7 +6,7 @@
 namespace pllc {
 
 // Field ids: the state of the reference's Conc/Velo/Visc structs plus phi
-// and div (structures_3d.h:24-53), then the scratch fields of this build.
+// and div (structures_3d.h:24-53), then scratch fields of the solver.
 enum FieldId {
     C_OLD = 0, C_NEW, C_N, C_NM1,
     U_OLD, U_NEW, U_PG, U_N, U_NM1,

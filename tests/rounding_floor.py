@@ -18,7 +18,7 @@ ten times it.  FLOORS holds the measured values (make them again with
 `python -m tests.rounding_floor`); tests/test_rounding_floor.py re-measures
 them and checks the table.
 
-Test infrastructure: imports the oracle, as the checker.
+Test infrastructure: imports the oracle, to check results.
 """
 import numpy as np
 
@@ -30,7 +30,7 @@ SUBSTAGE = ["CONCENTRATION", "VISCOSITY", "XMOM", "YMOM", "ZMOM", "VELO_BOUNDARY
 QUANTITIES = {"c": ["C_OLD"], "velocity": ["U_OLD", "V_OLD", "W_OLD"], "phi": ["PHI"],
               "pg": ["U_PG", "V_PG", "W_PG"], "U_OLD": ["U_OLD"], "V_OLD": ["V_OLD"], "W_OLD": ["W_OLD"]}
 FLOOR_MARGIN = 10.0
-NORTH_STAR_TOL = 1e-11
+PARITY_TOL = 1e-11
 
 # case -> quantity -> largest relative L2 gap over the case's steps, max of
 # the "fft" and "ulp" measurements
@@ -57,11 +57,11 @@ FLOORS = {
 
 def bar(case, q):
     """The GPU bar for quantity q of a case: FLOOR_MARGIN x its floor, at
-    least north_star's 1e-11 (c, the velocity and phi are held to exactly
+    least the required 1e-11 (c, the velocity and phi are held to exactly
     1e-11, whatever the floor)."""
     if q in ("c", "velocity", "phi"):
-        return NORTH_STAR_TOL
-    return max(NORTH_STAR_TOL, FLOOR_MARGIN * FLOORS[case][q])
+        return PARITY_TOL
+    return max(PARITY_TOL, FLOOR_MARGIN * FLOORS[case][q])
 
 
 def rel_l2(a, b):
@@ -172,7 +172,7 @@ def live_bars(oracle_mod, prm, c0, nsteps, modes=("fft", "ulp")):
     for mode in modes:
         for q, v in measure(oracle_mod, prm, c0, nsteps, mode).items():
             worst[q] = max(worst.get(q, 0.0), v)
-    return {q: NORTH_STAR_TOL if q in ("c", "velocity", "phi") else max(NORTH_STAR_TOL, FLOOR_MARGIN * v)
+    return {q: PARITY_TOL if q in ("c", "velocity", "phi") else max(PARITY_TOL, FLOOR_MARGIN * v)
             for q, v in worst.items()}, worst
 
 

@@ -1,4 +1,4 @@
-"""Pins of the FP64 reference network (oracle/cnn_ref.py), the checker of
+"""Pins of the FP64 reference network (oracle/cnn_ref.py), the reference for
 the GPU closure kernels: the convolution against a brute-force sum, its
 two adjoints by the inner-product identity, and the parameter gradients by
 central finite differences."""

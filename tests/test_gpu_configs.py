@@ -1,7 +1,7 @@
 """The plain FP64 path at the run configurations' own sizes and states, vs
 the FP64 oracle (OpenMP on the host's cores), through the C-ABI.
 
-Bar (north_star's): after every step, relative L2 <= 1e-11 in c, in the
+Bar (the required parity): after every step, relative L2 <= 1e-11 in c, in the
 velocity (u, v, w as one vector field) and in phi.  Each velocity
 component on its own and the pressure gradient (one vector field) are held
 to ten times the rounding floor the oracle itself shows for the same case
@@ -107,7 +107,7 @@ def test_front_256x128x128_tracks_oracle(oracle_mod, threads):
 
 def test_mixed_front_256x128x128_tracks_fp64_oracle(oracle_mod, threads):
     """The benchmark's workload in mixed precision against the FP64 oracle
-    for 20 steps: the concentration within 1e-5 relative L2 (north_star's
+    for 20 steps: the concentration within 1e-5 relative L2 (the required
     bar) and the velocity vector within 1e-5 after every 5th step; each
     step's dt (an FP64 reduction over the FP32 velocities) within 1e-5 of
     the oracle's."""

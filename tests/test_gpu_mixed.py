@@ -2,8 +2,8 @@
 flux and bulk reductions and the (0,0) Poisson pencil) vs the FP64 oracle.
 
 The bar, written here: after 10 steps the concentration is within 1e-5
-relative L2 of the FP64 solution (north_star's bar) and the velocity (u, v,
-w as one vector field) within 1e-5.  phi, which north_star gives no bar, is
+relative L2 of the FP64 solution (the required bar) and the velocity (u, v,
+w as one vector field) within 1e-5.  phi, for which no bar is required, is
 held to 1e-3 as a sanity bound (measured up to 1.9e-4, at 24x256x256).
 Every field's own error is printed so its size is on record; the spanwise
 component w is ~1e-4 of u at these states, so its own relative error

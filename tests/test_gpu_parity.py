@@ -109,7 +109,7 @@ def test_each_operator_from_shared_state(oracle_mod, nsteps_before):
             orc.op(op, dt, rk, vb_o)
             sim.op(op, dt, rk, vb_o)
             _compare(orc, sim, written[op], what="rk%d %s" % (rk, op))
-            # keep the two states identical so each operator is judged alone
+            # keep the two states identical so each operator is compared alone
             _sync_state(orc, sim)
 
 
