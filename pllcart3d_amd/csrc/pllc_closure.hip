@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_loss(long n, const float* __restrict__ 
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) lpart[blockIdx.x] = s[0];
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_ysum(G3 g, const float* __restrict__ y,
     s[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_ymean(G3 g, const double* __restrict__ 
     s[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_guard_stats(pllc::Dims d, const T* __re
     ssum[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
+        if ((int)threadIdx.x < w) {
             smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
             smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
             ssum[threadIdx.x] += ssum[threadIdx.x + w];
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_corr_sum(pllc::Dims d, const T* __restr
     s[threadIdx.x] = sm;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];

@@ -1691,11 +1691,12 @@ __global__ __launch_bounds__(256) void k_dct_planes(int ny_, int nz_, int nx, in
         for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int k = 0; k < KN; k += 4) {
-            if (!N && k >= nz) break;
+            if (N || k < nz) {
 #pragma unroll
-            for (int u = 0; u < TW; u++)
-                if (tok[u])
-                    acc[u] = Mma<T>::run(P[(ty0[u] + li) * pz + k + lk], Z[(tz0[u] + li) * pz + k + lk], acc[u]);
+                for (int u = 0; u < TW; u++)
+                    if (tok[u])
+                        acc[u] = Mma<T>::run(P[(ty0[u] + li) * pz + k + lk], Z[(tz0[u] + li) * pz + k + lk], acc[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < TW; u++)
@@ -1710,11 +1711,12 @@ __global__ __launch_bounds__(256) void k_dct_planes(int ny_, int nz_, int nx, in
         for (int u = 0; u < TW; u++) acc[u] = acc_t{T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int k = 0; k < KN; k += 4) {
-            if (!N && k >= ny) break;
+            if (N || k < ny) {
 #pragma unroll
-            for (int u = 0; u < TW; u++)
-                if (tok[u])
-                    acc[u] = Mma<T>::run(Y[(ty0[u] + li) * py + k + lk], Q[(k + lk) * pz + tz0[u] + li], acc[u]);
+                for (int u = 0; u < TW; u++)
+                    if (tok[u])
+                        acc[u] = Mma<T>::run(Y[(ty0[u] + li) * py + k + lk], Q[(k + lk) * pz + tz0[u] + li], acc[u]);
+            }
         }
         const int m = p / nx, x = p - m * nx;
         PhiOut<T> pm = po;
