@@ -154,7 +154,38 @@ def ensemble_summary(args):
     sys.stdout.write("\n".join(out) + "\n")
 
 
+def issue_summary(args):
+    """Vector-issue table from a --pmc SQ_INSTS_VALU ... SQ_WAVES run: per
+    kernel its average duration, VALU instructions and waves per dispatch,
+    and the share of the device's vector-issue slots its instructions fill
+    (SQ_INSTS_VALU x 4 cycles / (CUs x 4 SIMDs x duration x clock))."""
+    con = sqlite3.connect(args.trace)
+    dur = collections.defaultdict(list)
+    for name, a, b in con.execute("select name, start, end from kernels"):
+        dur[_short(name)].append((b - a) / 1e3)
+    cnt = collections.defaultdict(lambda: collections.defaultdict(list))
+    for name, c, v in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        cnt[_short(name)][c].append(v)
+    simds = 4 * args.cus
+    out = ["| kernel | avg us | VALU instructions per launch | waves | per wave | VALU issue share |",
+           "|---|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, cs in cnt.items():
+        if not k.startswith("k_") or "SQ_INSTS_VALU" not in cs or "SQ_WAVES" not in cs or k not in dur:
+            continue
+        t = sum(dur[k]) / len(dur[k])
+        ins = sum(cs["SQ_INSTS_VALU"]) / len(cs["SQ_INSTS_VALU"])
+        wav = sum(cs["SQ_WAVES"]) / len(cs["SQ_WAVES"])
+        share = ins * 4 / (simds * t * 1e-6 * args.clock_ghz * 1e9)
+        rows.append((t * len(dur[k]), "| `%s` | %.1f | %.3g | %d | %.0f | %.0f%% |" % (k[:60], t, ins, wav, ins / wav,
+                                                                                   100 * share)))
+    out += [r for _, r in sorted(rows, key=lambda r: -r[0])]
+    sys.stdout.write("\n".join(out) + "\n")
+
+
 def summary(args):
+    if getattr(args, "issue", False):
+        return issue_summary(args)
     if getattr(args, "ensemble", False):
         return ensemble_summary(args)
     out = []
@@ -211,6 +242,9 @@ def main():
     s.add_argument("--bytes", type=int, default=8, help="bytes per value")
     s.add_argument("--ensemble", action="store_true", help="an ensemble run's trace (every stream, per grid)")
     s.add_argument("--steps", type=int, default=4, help="the ensemble run's profiled steps")
+    s.add_argument("--issue", action="store_true", help="a --pmc SQ_INSTS_VALU ... SQ_WAVES run: vector-issue table")
+    s.add_argument("--cus", type=int, default=256, help="compute units of the device (issue table)")
+    s.add_argument("--clock-ghz", type=float, default=2.4, help="engine clock (issue table)")
     a = ap.parse_args()
     run(a) if a.cmd == "run" else summary(a)
 

@@ -93,3 +93,23 @@ def test_ensemble_summary_takes_every_stream_after_the_last_copy(tmp_path, capsy
     assert line and line[0].split("|")[3].strip() == "2" and abs(float(line[0].split("|")[5]) - 4.0) < 1e-9, out
     line = [l for l in out.splitlines() if l.startswith("| `k_corr_pg<float, true>` | 64 x 48 x 3328 |")]
     assert line and abs(float(line[0].split("|")[5]) - 1.5) < 1e-9, out
+
+
+def test_issue_summary_shares(tmp_path, capsys):
+    con = sqlite3.connect(str(tmp_path / "p.db"))
+    con.execute("create table kernels (name text, start int, end int)")
+    con.execute("create table counters_collection (kernel_name text, counter_name text, value real)")
+    k = "void pllc::k_rhs_c<float, true, false>(pllc::Dims)"
+    # two dispatches of 50 us; 3.84e7 instructions each: 3.84e7 x 4 / (1024 x 50e-6 x 2.4e9) = 1.25
+    con.executemany("insert into kernels values (?, ?, ?)", [(k, 0, 50000), (k, 60000, 110000)])
+    con.executemany("insert into counters_collection values (?, ?, ?)",
+                    [(k, "SQ_INSTS_VALU", 3.84e7), (k, "SQ_INSTS_VALU", 3.84e7), (k, "SQ_WAVES", 65536.0),
+                     (k, "SQ_WAVES", 65536.0)])
+    con.commit()
+    con.close()
+    profiling.summary(argparse.Namespace(trace=str(tmp_path / "p.db"), issue=True, cus=256, clock_ghz=2.4))
+    out = capsys.readouterr().out
+    line = [l for l in out.splitlines() if l.startswith("| `k_rhs_c<float, true, false>`")]
+    assert line, out
+    cells = [c.strip() for c in line[0].strip("|").split("|")]
+    assert cells[1] == "50.0" and cells[3] == "65536" and cells[4] == "586" and cells[5] == "125%", cells
