@@ -40,7 +40,7 @@ SWEEP_TOUCHES_PER_SUBSTAGE = {"sweep_y": 11.0, "sweep_z": 11.0, "sweep_x": 15.0}
 # the coarse runs' Timestep caps (the fine run's 2e-3, and larger)
 CLOSURE_HORIZON = 4
 CLOSURE_ALPHA = 0.5
-CLOSURE_EVERY = 8
+CLOSURE_EVERY = 16
 CLOSURE_TIMESTEPS = (2e-3, 2.25e-3, 2.5e-3)
 
 # initial-condition recipes, as the workload names them
